@@ -1,0 +1,175 @@
+/*
+ * mrfg.h -- C ABI of the MI355X-native monolithic interface-coupled solver
+ * (libmrfg.so, built from multiregionfoam_amd/csrc for gfx950).
+ *
+ * The boundary is the one multiRegionFoam crosses when it solves a
+ * monolithically coupled field:
+ *
+ *   multiRegionSystem::assembleAndSolveCoupledMatrix  (src/multiRegionSystem/
+ *   multiRegionSystem.C:61-191) collects each region's assembled fvMatrix
+ *   into a coupledFvMatrix and calls coupledEqns.solve(dict) with region 0's
+ *   "<field>coupled" solver dictionary (multiRegionSystem.C:150-153).
+ *
+ * foam-extend's coupledFvMatrix<scalar>::solve then hands the coupled
+ * solver, per region, the LDU matrix with addBoundaryDiag applied, the
+ * source with addBoundarySource(couples=false) applied, and the interface
+ * boundary coefficients; the monolithic interfaces contribute to A*x through
+ * monolithicCouplingFvPatchField::initInterfaceMatrixUpdate /
+ * updateInterfaceMatrix (monolithicCouplingFvPatchField.C:381-464):
+ *     result[fc[i]] -= coeffs[i] * sum_p w[p] * psi_other[fc_other[idx[p]]].
+ * The entry points below take exactly that data, as plain arrays.
+ *
+ * Conventions: FP64 values, int32 addressing (the reference's -DWM_DP,
+ * -DWM_LABEL_SIZE=32, compile_commands.json:9-10); faces in OpenFOAM
+ * upper-triangular order (l_addr[f] < u_addr[f], sorted by owner then
+ * neighbour).  Every function returns MRFG_OK (0) or a negative code and
+ * never throws; non-convergence is not an error (mrfg_perf.converged = 0).
+ * A context is not thread-safe; several contexts may coexist.
+ */
+#ifndef MRFG_H
+#define MRFG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRFG_OK 0
+#define MRFG_ERR_ARG (-1)      /* bad argument or inconsistent addressing */
+#define MRFG_ERR_DEVICE (-2)   /* HIP runtime error (no device, out of memory, ...) */
+#define MRFG_ERR_DICT (-3)     /* dictionary syntax or unknown keyword/solver */
+#define MRFG_ERR_STATE (-4)    /* call out of sequence (e.g. solve before commit) */
+
+typedef struct mrfg_ctx mrfg_ctx;
+typedef struct mrfg_case mrfg_case;
+
+/* lduSolverPerformance of one coupled solve */
+typedef struct {
+    double initial_residual;   /* sum|b-Ax0| / normFactor */
+    double final_residual;     /* sum|b-Ax| / normFactor (FP64 in every mode) */
+    int n_iterations;          /* Krylov iterations (mixed: total inner iterations) */
+    int converged;
+    int singular;              /* breakdown (rho or (t,t) or (w,p) zero) */
+    int n_outer;               /* mixed: FP64 refinement steps; fp64: 0 */
+    double seconds;            /* wall time of the solve on the device */
+    double bytes_moved;        /* algorithmic bytes of the solve */
+} mrfg_perf;
+
+enum { MRFG_BICGSTAB = 0, MRFG_PCG = 1 };
+enum { MRFG_FP64 = 0, MRFG_MIXED = 1 };
+
+/* Solver controls: the keys of an fvSolution solver entry
+   (flowOverHeatedPlate/system/fluid/fvSolution:43-57) plus the GPU keys
+   precision / refinement / checkInterval. */
+typedef struct {
+    int solver;            /* MRFG_BICGSTAB (BiCGStab) or MRFG_PCG (PCG/CG) */
+    int precision;         /* MRFG_FP64 or MRFG_MIXED */
+    double tolerance;
+    double rel_tol;
+    int min_iter;
+    int max_iter;
+    double inner_rel_tol;  /* mixed: FP32 inner solve relative tolerance */
+    int inner_max_iter;
+    int max_outer;
+    int check_interval;    /* iterations between host convergence checks */
+} mrfg_controls;
+
+/* ---- context --------------------------------------------------------- */
+
+/* Replaces the run-time selected coupledLduSolver object built per call by
+   coupledFvMatrix::solve (multiRegionSystem.C:150-153); one context holds
+   one or more coupled systems resident on `device`. */
+int mrfg_create(mrfg_ctx **ctx, int device);
+void mrfg_destroy(mrfg_ctx *ctx);
+const char *mrfg_last_error(const mrfg_ctx *ctx);
+const char *mrfg_version(void);
+
+/* Start a coupled system (one coupledFvMatrix: multiRegionSystem.C:85). */
+int mrfg_begin_system(mrfg_ctx *ctx, int n_regions, int n_interfaces, int *sys_id);
+
+/* One region's fvMatrix as the coupled solver receives it
+   (coupledEqns.set(nReg, ...) at multiRegionSystem.C:143-145): diag with
+   all patch internalCoeffs, source with non-coupled boundaryCoeffs, psi0 the
+   current field.  lower == NULL means a symmetric matrix. */
+int mrfg_set_region(mrfg_ctx *ctx, int sys, int region, int n_cells, int n_faces,
+                    const int32_t *l_addr, const int32_t *u_addr,
+                    const double *diag, const double *lower, const double *upper,
+                    const double *source, const double *psi0);
+
+/* One monolithic interface (a coupledPatchPair of regionInterfaceProperties;
+   monolithicCouplingFvPatchField.C:381-464).  Side A faces gather B face
+   cells with weights w_a over pairs [off_a[i], off_a[i+1]) (idx_a = B face
+   index), and vice versa; coeff_* are the coupled patches' boundaryCoeffs. */
+int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int region_b,
+                       int n_faces_a, const int32_t *fc_a, const double *coeff_a,
+                       const int32_t *off_a, const int32_t *idx_a, const double *w_a,
+                       int n_faces_b, const int32_t *fc_b, const double *coeff_b,
+                       const int32_t *off_b, const int32_t *idx_b, const double *w_b);
+
+/* Build the device layout of every system defined so far and upload it. */
+int mrfg_commit(mrfg_ctx *ctx);
+
+/* coupledFvMatrix<scalar>::solve(dict) for every committed system
+   (multiRegionSystem.C:150-153).  perf receives one entry per system. */
+int mrfg_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_perf *perf);
+
+/* Same, with the controls read from an fvSolution text: `entry` names the
+   solver dictionary under `solvers` (e.g. "Tcoupled"). */
+int mrfg_solve_dict(mrfg_ctx *ctx, const char *fv_solution, const char *entry, mrfg_perf *perf);
+
+/* Parse a solver entry of an fvSolution text (lduSolver::New selection by
+   name, keys solver / preconditioner / tolerance / relTol / minIter /
+   maxIter).  err receives the message on MRFG_ERR_DICT. */
+int mrfg_parse_controls(const char *fv_solution, const char *entry, mrfg_controls *out,
+                        char *err, int err_len);
+
+/* Field in / out in the region's own cell order (the psi that
+   coupledFvMatrix::solve writes back into each region's T). */
+int mrfg_get_psi(mrfg_ctx *ctx, int sys, int region, double *psi);
+int mrfg_set_psi(mrfg_ctx *ctx, int sys, int region, const double *psi);
+
+/* ---- case builder ------------------------------------------------------ */
+
+/* Assemble a case dictionary (regions of transportTemperature /
+   conductTemperature type and monolithicCoupledPatches) into its coupled
+   system: the work of regionType::setCoupledEqns
+   (conductTemperature.C:135-152, transportTemperature.C:129-150) and of the
+   monolithic patch fields (monolithicThermalDiffusivity...C:167-277,
+   GGI weights of ggiInterfaceToInterfaceMapping.C:62-77). */
+int mrfg_case_build(const char *dict_text, mrfg_case **out, char *err, int err_len);
+void mrfg_case_free(mrfg_case *c);
+int mrfg_case_sizes(const mrfg_case *c, int *n_regions, int *n_interfaces);
+int mrfg_case_region_sizes(const mrfg_case *c, int region, int *n_cells, int *n_faces, int *symmetric);
+int mrfg_case_region(const mrfg_case *c, int region, int32_t *l_addr, int32_t *u_addr,
+                     double *diag, double *lower, double *upper, double *source, double *psi0);
+int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int *region_b,
+                              int *n_faces_a, int *n_faces_b, int *n_pairs_a, int *n_pairs_b);
+int mrfg_case_interface(const mrfg_case *c, int iface,
+                        int32_t *fc_a, double *coeff_a, int32_t *off_a, int32_t *idx_a, double *w_a,
+                        int32_t *fc_b, double *coeff_b, int32_t *off_b, int32_t *idx_b, double *w_b);
+/* Add a built case as a new system of the context (begin_system +
+   set_region + set_interface). */
+int mrfg_add_case(mrfg_ctx *ctx, const mrfg_case *c, int *sys_id);
+
+/* ---- kernel-level entry points (tests, profiling) ---------------------- */
+
+/* y = A x over all regions of one system (lduMatrix::Amul plus the
+   interface updates); x, y in concatenated region order. */
+int mrfg_spmv(mrfg_ctx *ctx, int sys, const double *x, double *y);
+/* Reciprocal DILU diagonal (coupled Cholesky calcPreconDiag). */
+int mrfg_dilu_factor(mrfg_ctx *ctx, int sys, double *rD);
+/* w = M^-1 r with the DILU preconditioner (coupled Cholesky precondition). */
+int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w);
+/* Average device time (ms) of `reps` launches: which 0 = SpMV, 1 = DILU apply. */
+int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms);
+/* info[0..5] = cells, internal faces, GGI pairs, rows incl. padding,
+   64-row slices, DILU levels. */
+int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info);
+/* Canonical algorithmic bytes at value size 8 (fp64) or 4 (fp32). */
+int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b_spmv, double *b_dilu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,378 @@
+"""MI355X-native monolithic interface-coupled solve (multiRegionFoam hot path).
+
+Python mirror of the reference's coupled-solve interface over the C ABI of
+``libmrfg.so`` (include/mrfg.h):
+
+* :class:`CoupledFvMatrix` plays ``coupledFvMatrix<scalar>`` -- one or more
+  coupled systems resident on a GPU, ``solve(dict)`` with the controls of an
+  fvSolution solver entry (multiRegionSystem.C:150-153).
+* :class:`Case` assembles a case dictionary natively (regionType
+  ``setCoupledEqns`` and the monolithic patch coefficients).
+* :func:`parse_controls` reads an fvSolution solver entry.
+
+The HIP library is required: every entry point raises if ``libmrfg.so`` is
+missing or no GPU is present; nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import cases  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmrfg.so")
+
+MRFG_OK = 0
+ERRORS = {-1: "argument", -2: "device", -3: "dictionary", -4: "state"}
+
+
+class MrfgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mrfg %s error (%d): %s" % (ERRORS.get(code, "unknown"), code, msg))
+        self.code = code
+
+
+class Perf(C.Structure):
+    _fields_ = [("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("n_iterations", C.c_int), ("converged", C.c_int), ("singular", C.c_int),
+                ("n_outer", C.c_int), ("seconds", C.c_double), ("bytes_moved", C.c_double)]
+
+
+class Controls(C.Structure):
+    _fields_ = [("solver", C.c_int), ("precision", C.c_int), ("tolerance", C.c_double),
+                ("rel_tol", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
+                ("inner_rel_tol", C.c_double), ("inner_max_iter", C.c_int),
+                ("max_outer", C.c_int), ("check_interval", C.c_int)]
+
+
+_lib = None
+
+I32P = C.POINTER(C.c_int32)
+F64P = C.POINTER(C.c_double)
+
+
+def lib():
+    """Load libmrfg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libmrfg.so is not built (run __graft_entry__.build() or make -C "
+                          "multiregionfoam_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "mrfg_version": ([], C.c_char_p),
+        "mrfg_create": ([C.POINTER(vp), C.c_int], C.c_int),
+        "mrfg_destroy": ([vp], None),
+        "mrfg_last_error": ([vp], C.c_char_p),
+        "mrfg_begin_system": ([vp, C.c_int, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "mrfg_set_region": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, F64P, F64P, F64P, F64P, F64P],
+                            C.c_int),
+        "mrfg_set_interface": ([vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_int, I32P, F64P, I32P, I32P, F64P,
+                                C.c_int, I32P, F64P, I32P, I32P, F64P], C.c_int),
+        "mrfg_commit": ([vp], C.c_int),
+        "mrfg_solve": ([vp, C.POINTER(Controls), C.POINTER(Perf)], C.c_int),
+        "mrfg_solve_dict": ([vp, C.c_char_p, C.c_char_p, C.POINTER(Perf)], C.c_int),
+        "mrfg_parse_controls": ([C.c_char_p, C.c_char_p, C.POINTER(Controls), C.c_char_p, C.c_int], C.c_int),
+        "mrfg_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
+        "mrfg_set_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
+        "mrfg_case_build": ([C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int], C.c_int),
+        "mrfg_case_free": ([vp], None),
+        "mrfg_case_sizes": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "mrfg_case_region_sizes": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+                                   C.c_int),
+        "mrfg_case_region": ([vp, C.c_int, I32P, I32P, F64P, F64P, F64P, F64P, F64P], C.c_int),
+        "mrfg_case_interface_sizes": ([vp, C.c_int] + [C.POINTER(C.c_int)] * 6, C.c_int),
+        "mrfg_case_interface": ([vp, C.c_int, I32P, F64P, I32P, I32P, F64P, I32P, F64P, I32P, I32P, F64P],
+                                C.c_int),
+        "mrfg_add_case": ([vp, vp, C.POINTER(C.c_int)], C.c_int),
+        "mrfg_spmv": ([vp, C.c_int, F64P, F64P], C.c_int),
+        "mrfg_dilu_factor": ([vp, C.c_int, F64P], C.c_int),
+        "mrfg_precondition": ([vp, C.c_int, F64P, F64P], C.c_int),
+        "mrfg_time_kernel": ([vp, C.c_int, C.c_int, F64P], C.c_int),
+        "mrfg_layout_info": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
+        "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+EXPORTED = (
+    "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region "
+    "mrfg_set_interface mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi "
+    "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
+    "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
+    "mrfg_precondition mrfg_time_kernel mrfg_layout_info mrfg_bytes").split()
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def _text(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+# ---------------------------------------------------------------------------
+# Controls
+# ---------------------------------------------------------------------------
+
+SOLVER_IDS = {"BiCGStab": 0, "hipBiCGStab": 0, "PBiCGStab": 0, "PCG": 1, "CG": 1, "hipPCG": 1}
+
+
+def parse_controls(fv_solution: str, entry: str) -> Controls:
+    """Read ``solvers { <entry> { ... } }`` of an fvSolution text."""
+    c = Controls()
+    err = C.create_string_buffer(512)
+    rc = lib().mrfg_parse_controls(_text(fv_solution), _text(entry), C.byref(c), err, 512)
+    if rc != MRFG_OK:
+        raise MrfgError(rc, err.value.decode())
+    return c
+
+
+def controls(solver="BiCGStab", tolerance=1e-10, relTol=0.0, minIter=0, maxIter=1000,
+             precision="fp64", innerRelTol=1e-4, innerMaxIter=1000, maxOuter=50, checkInterval=8):
+    """Build a Controls record from keyword values (same keys as fvSolution)."""
+    text = ("solvers { X { solver %s; preconditioner { preconditioner Cholesky; } tolerance %r; "
+            "relTol %r; minIter %d; maxIter %d; precision %s; checkInterval %d; "
+            "refinement { innerRelTol %r; innerMaxIter %d; maxOuter %d; } } }"
+            % (solver, float(tolerance), float(relTol), minIter, maxIter, precision, checkInterval,
+               float(innerRelTol), innerMaxIter, maxOuter))
+    return parse_controls(text, "X")
+
+
+@dataclass
+class SolverPerformance:
+    """lduSolverPerformance of one coupled solve."""
+    initial_residual: float
+    final_residual: float
+    n_iterations: int
+    converged: bool
+    singular: bool
+    n_outer: int
+    seconds: float
+    bytes_moved: float
+
+    def line(self, solver="BiCGStab", field="T"):
+        return ("%s:  Solving for %s, Initial residual = %.6g, Final residual = %.6g, No Iterations %d"
+                % (solver, field, self.initial_residual, self.final_residual, self.n_iterations))
+
+
+# ---------------------------------------------------------------------------
+# Case (native assembly)
+# ---------------------------------------------------------------------------
+
+class Case:
+    """A case dictionary assembled by the native case builder."""
+
+    def __init__(self, case):
+        text = case if isinstance(case, str) else cases.to_dict_text(case)
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mrfg_case_build(_text(text), C.byref(h), err, 1024)
+        if rc != MRFG_OK:
+            raise MrfgError(rc, err.value.decode())
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mrfg_case_free(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sizes(self):
+        nr, ni = C.c_int(), C.c_int()
+        lib().mrfg_case_sizes(self._h, C.byref(nr), C.byref(ni))
+        return nr.value, ni.value
+
+    def region(self, r):
+        n, f, sym = C.c_int(), C.c_int(), C.c_int()
+        if lib().mrfg_case_region_sizes(self._h, r, C.byref(n), C.byref(f), C.byref(sym)) != MRFG_OK:
+            raise IndexError(r)
+        n, f = n.value, f.value
+        d = dict(nCells=n, nFaces=f, symmetric=bool(sym.value),
+                 lAddr=np.zeros(f, np.int32), uAddr=np.zeros(f, np.int32), diag=np.zeros(n),
+                 lower=np.zeros(f), upper=np.zeros(f), source=np.zeros(n), psi=np.zeros(n))
+        lib().mrfg_case_region(self._h, r, _p(d["lAddr"], C.c_int32), _p(d["uAddr"], C.c_int32),
+                               _p(d["diag"], C.c_double), _p(d["lower"], C.c_double),
+                               _p(d["upper"], C.c_double), _p(d["source"], C.c_double),
+                               _p(d["psi"], C.c_double))
+        return d
+
+    def interface(self, i):
+        v = [C.c_int() for _ in range(6)]
+        if lib().mrfg_case_interface_sizes(self._h, i, *[C.byref(x) for x in v]) != MRFG_OK:
+            raise IndexError(i)
+        ra, rb, na, nb, pa, pb = [x.value for x in v]
+        d = dict(regionA=ra, regionB=rb, nA=na, nB=nb,
+                 fcA=np.zeros(na, np.int32), coeffA=np.zeros(na), offA=np.zeros(na + 1, np.int32),
+                 idxA=np.zeros(pa, np.int32), wA=np.zeros(pa),
+                 fcB=np.zeros(nb, np.int32), coeffB=np.zeros(nb), offB=np.zeros(nb + 1, np.int32),
+                 idxB=np.zeros(pb, np.int32), wB=np.zeros(pb))
+        lib().mrfg_case_interface(self._h, i, _p(d["fcA"], C.c_int32), _p(d["coeffA"], C.c_double),
+                                  _p(d["offA"], C.c_int32), _p(d["idxA"], C.c_int32), _p(d["wA"], C.c_double),
+                                  _p(d["fcB"], C.c_int32), _p(d["coeffB"], C.c_double),
+                                  _p(d["offB"], C.c_int32), _p(d["idxB"], C.c_int32), _p(d["wB"], C.c_double))
+        return d
+
+    def regions(self):
+        return [self.region(r) for r in range(self.sizes()[0])]
+
+    def interfaces(self):
+        return [self.interface(i) for i in range(self.sizes()[1])]
+
+
+# ---------------------------------------------------------------------------
+# Coupled matrix (GPU-resident systems)
+# ---------------------------------------------------------------------------
+
+class CoupledFvMatrix:
+    """One or more coupled block systems resident on a GPU.
+
+    ``add_case`` / ``add_system`` define systems, ``commit`` uploads them,
+    ``solve`` runs the coupled solver on all of them at once.
+    """
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().mrfg_create(C.byref(h), device)
+        if rc != MRFG_OK:
+            raise MrfgError(rc, "cannot create a context on device %d (is a GPU present?)" % device)
+        self._h = h
+        self._sizes = []
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mrfg_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def close(self):
+        self.__del__()
+
+    def _check(self, rc):
+        if rc != MRFG_OK:
+            raise MrfgError(rc, lib().mrfg_last_error(self._h).decode())
+
+    def add_case(self, case):
+        if not isinstance(case, Case):
+            case = Case(case)
+        sid = C.c_int()
+        self._check(lib().mrfg_add_case(self._h, case.handle, C.byref(sid)))
+        self._sizes.append([r["nCells"] for r in case.regions()])
+        return sid.value
+
+    def add_system(self, regions, interfaces):
+        """Define a system from LDU arrays (dicts with the keys of Case.region
+        / Case.interface)."""
+        sid = C.c_int()
+        self._check(lib().mrfg_begin_system(self._h, len(regions), len(interfaces), C.byref(sid)))
+        s = sid.value
+        keep = []
+        for r, R in enumerate(regions):
+            arr = {k: np.ascontiguousarray(R[k], dtype=np.int32 if k in ("lAddr", "uAddr") else np.float64)
+                   for k in ("lAddr", "uAddr", "diag", "upper", "source", "psi")}
+            lower = None if R.get("symmetric", False) else np.ascontiguousarray(R["lower"], dtype=np.float64)
+            keep.append((arr, lower))
+            self._check(lib().mrfg_set_region(
+                self._h, s, r, int(R["nCells"]), int(R["nFaces"]), _p(arr["lAddr"], C.c_int32),
+                _p(arr["uAddr"], C.c_int32), _p(arr["diag"], C.c_double), _p(lower, C.c_double),
+                _p(arr["upper"], C.c_double), _p(arr["source"], C.c_double), _p(arr["psi"], C.c_double)))
+        for i, I in enumerate(interfaces):
+            a = {k: np.ascontiguousarray(I[k], dtype=np.int32 if k[:2] in ("fc", "of", "id") else np.float64)
+                 for k in ("fcA", "coeffA", "offA", "idxA", "wA", "fcB", "coeffB", "offB", "idxB", "wB")}
+            keep.append(a)
+            self._check(lib().mrfg_set_interface(
+                self._h, s, i, int(I["regionA"]), int(I["regionB"]),
+                int(I["nA"]), _p(a["fcA"], C.c_int32), _p(a["coeffA"], C.c_double), _p(a["offA"], C.c_int32),
+                _p(a["idxA"], C.c_int32), _p(a["wA"], C.c_double),
+                int(I["nB"]), _p(a["fcB"], C.c_int32), _p(a["coeffB"], C.c_double), _p(a["offB"], C.c_int32),
+                _p(a["idxB"], C.c_int32), _p(a["wB"], C.c_double)))
+        self._sizes.append([int(R["nCells"]) for R in regions])
+        return s
+
+    def commit(self):
+        self._check(lib().mrfg_commit(self._h))
+
+    @property
+    def n_systems(self):
+        return len(self._sizes)
+
+    def solve(self, ctrl=None, fv_solution=None, entry="Tcoupled", **kw):
+        """Solve every system.  ``ctrl``: Controls; or ``fv_solution`` text +
+        ``entry``; or keyword controls (see :func:`controls`)."""
+        perf = (Perf * max(1, self.n_systems))()
+        if fv_solution is not None:
+            self._check(lib().mrfg_solve_dict(self._h, _text(fv_solution), _text(entry), perf))
+        else:
+            if ctrl is None:
+                ctrl = controls(**kw)
+            self._check(lib().mrfg_solve(self._h, C.byref(ctrl), perf))
+        return [SolverPerformance(p.initial_residual, p.final_residual, p.n_iterations, bool(p.converged),
+                                  bool(p.singular), p.n_outer, p.seconds, p.bytes_moved)
+                for p in perf[:self.n_systems]]
+
+    def psi(self, sys=0):
+        out = []
+        for r, n in enumerate(self._sizes[sys]):
+            a = np.zeros(n)
+            self._check(lib().mrfg_get_psi(self._h, sys, r, _p(a, C.c_double)))
+            out.append(a)
+        return out
+
+    def set_psi(self, sys, psi):
+        for r, a in enumerate(psi):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self._check(lib().mrfg_set_psi(self._h, sys, r, _p(a, C.c_double)))
+
+    # ---- kernel-level -----------------------------------------------------
+    def spmv(self, x, sys=0):
+        x = np.ascontiguousarray(np.concatenate(x) if isinstance(x, (list, tuple)) else x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._check(lib().mrfg_spmv(self._h, sys, _p(x, C.c_double), _p(y, C.c_double)))
+        return y
+
+    def dilu_factor(self, sys=0):
+        n = sum(self._sizes[sys])
+        rD = np.zeros(n)
+        self._check(lib().mrfg_dilu_factor(self._h, sys, _p(rD, C.c_double)))
+        return rD
+
+    def precondition(self, r, sys=0):
+        r = np.ascontiguousarray(np.concatenate(r) if isinstance(r, (list, tuple)) else r, dtype=np.float64)
+        w = np.zeros_like(r)
+        self._check(lib().mrfg_precondition(self._h, sys, _p(r, C.c_double), _p(w, C.c_double)))
+        return w
+
+    def time_kernel(self, which, reps=20):
+        ms = C.c_double()
+        self._check(lib().mrfg_time_kernel(self._h, which, reps, C.byref(ms)))
+        return ms.value
+
+    def layout_info(self, sys=0):
+        info = (C.c_longlong * 6)()
+        self._check(lib().mrfg_layout_info(self._h, sys, info))
+        keys = ("cells", "faces", "pairs", "rows", "slices", "levels")
+        return dict(zip(keys, [int(v) for v in info]))
+
+    def bytes_model(self, sys=0, value_size=8):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().mrfg_bytes(self._h, sys, value_size, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(iter=a.value, spmv=b.value, dilu=c.value)

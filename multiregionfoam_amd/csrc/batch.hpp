@@ -1,0 +1,151 @@
+// Device-resident batch of coupled block systems and the solve drivers.
+//
+// A batch holds one or more independent coupled systems (each: regions +
+// monolithic interfaces) concatenated in one position space.  Kernels run
+// over all systems at once; Krylov scalars and convergence are per system,
+// so a single system is a batch of one and many uneven systems share every
+// launch.  Matrices, factors and fields stay resident between solves.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "coupled_system.hpp"
+#include "device.hpp"
+#include "dictionary.hpp"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace mrfg {
+
+// Solver controls of an fvSolution solver entry, e.g.
+//   Tcoupled { solver BiCGStab; preconditioner { preconditioner Cholesky; }
+//              tolerance 1e-10; relTol 0; minIter 0; maxIter 1000;
+//              precision mixed; refinement { innerRelTol 1e-4; ... } }
+struct SolverControls {
+    enum Kind { BiCGStab = 0, PCG = 1 } solver = BiCGStab;
+    enum Precision { FP64 = 0, Mixed = 1 } precision = FP64;
+    double tolerance = 1e-6;
+    double relTol = 0.0;
+    int minIter = 0;
+    int maxIter = 1000;
+    double innerRelTol = 1e-4;
+    int innerMaxIter = 1000;
+    int maxOuter = 50;
+    int checkInterval = 8;    // iterations between host convergence checks
+};
+
+SolverControls parseControls(const Dict &d);
+
+struct SolvePerf {
+    double initialResidual = 0, finalResidual = 0;
+    int nIterations = 0, converged = 0, singular = 0, nOuter = 0;
+    double seconds = 0;       // device time of the solve (setup + iterations)
+    double bytes = 0;         // algorithmic bytes moved (canonical model)
+};
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count);
+    void upload(const std::vector<T> &h, hipStream_t st);
+    void zero(hipStream_t st);
+};
+
+template <typename T>
+struct DevMatrix {
+    DevBuf<T> ellVal, diag, ifCoeff, ifPairW, lVal, lProd, uVal;
+};
+
+class Batch {
+public:
+    explicit Batch(int device);
+    ~Batch();
+
+    // Replace the batch content.  Builds layouts, uploads FP64 (and, on
+    // demand, FP32) matrices and the initial guesses.
+    void setSystems(const std::vector<const CoupledSystem *> &systems);
+    int nSystems() const { return (int)sys_.size(); }
+
+    // Solve every system; perf gets one entry per system.
+    void solve(const SolverControls &c, std::vector<SolvePerf> &perf);
+
+    // Per-region initial guess / solution of one system (original cell order).
+    void setPsi(int sys, int region, const double *psi);
+    void getPsi(int sys, int region, double *psi);
+
+    // Kernel-level entry points (used by tests and the profiler harness).
+    void spmv(const double *xHost, double *yHost, int sys);           // y = A x
+    void diluFactor(int sys, double *rDHost);                        // rD per cell
+    void precondition(const double *rHost, double *wHost, int sys);   // w = M^-1 r
+
+    // Time `reps` repetitions of one kernel family on resident data; returns
+    // the average device time per launch (ms).  which: 0 SpMV, 1 DILU fwd+bwd.
+    double timeKernel(int which, int reps);
+
+    const SystemLayout &layout(int sys) const { return *lay_[sys]; }
+    hipStream_t stream() const { return stream_; }
+    // Canonical algorithmic bytes of one iteration / one SpMV / one DILU
+    // apply for system `sys` at value size sv.
+    double bytesIter(int sys, int sv) const;
+    double bytesSpmv(int sys, int sv) const;
+    double bytesDilu(int sys, int sv) const;
+
+private:
+    void ensureFloat();
+    void factorise();
+    void initState(const SolverControls &c);
+    void readState(std::vector<SysState> &out);
+    void runFp64(const SolverControls &c);
+    void runMixed(const SolverControls &c, std::vector<int> &innerIters);
+    template <typename T>
+    MatView<T> view(const DevMatrix<T> &m) const;
+    template <typename T>
+    void bindKrylov(Krylov<T> &K, bool fp32);
+    int posBegin(int sys) const { return sliceBegin_[sys] * kSlice; }
+
+    int device_;
+    hipStream_t stream_ = nullptr;
+    std::vector<std::unique_ptr<SystemLayout>> lay_;
+    std::vector<const CoupledSystem *> sys_;
+    std::vector<int> sliceBegin_, levelBase_;
+    int nSlices_ = 0, nPos_ = 0, nBlk_ = 0, nLevelsTotal_ = 0, nOrder_ = 0;
+    bool factorised_ = false, haveFloat_ = false;
+
+    // structure (shared by both precisions)
+    DevBuf<int32_t> ellOff_, ellCol_, ifBase_, ifRowFace_, ifPairOff_, ifPairCol_, lOff_, lCol_, uOff_, uCol_;
+    DevBuf<unsigned long long> ifMask_;
+    DevBuf<int32_t> cellOfPos_;
+    DevBuf<BlockEntry> blocks_;
+    DevBuf<SysInfo> info_;
+    DevBuf<int32_t> sliceSys_, sliceLevel_, levelCount_, order_;
+    DevBuf<int> cntA_, cntB_, cntC_, err_;
+    void checkDeviceError();
+    // host copies of the FP64 values, kept to derive the FP32 copies
+    std::vector<double> hEllVal_, hDiag_, hIfCoeff_, hIfPairW_, hLVal_, hLProd_, hUVal_;
+    DevMatrix<double> m64_;
+    DevMatrix<float> m32_;
+    DevBuf<double> sumA_, b64_, x64_, d64_;
+    DevBuf<double> v64_[12];
+    DevBuf<float> v32_[14];
+    DevBuf<SysState> st64_, st32_;
+    DevBuf<double> partials_;
+    DevBuf<int> tickets_;
+    std::vector<SysInfo> hInfo_;
+};
+
+}  // namespace mrfg

@@ -1,0 +1,470 @@
+// extern "C" entry points of libmrfg.so (declared in include/mrfg.h).
+#include "../../include/mrfg.h"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "coupled_system.hpp"
+#include "dictionary.hpp"
+
+struct mrfg_ctx {
+    int device = 0;
+    std::string err;
+    std::vector<std::unique_ptr<mrfg::CoupledSystem>> systems;
+    std::unique_ptr<mrfg::Batch> batch;
+    bool committed = false;
+};
+
+struct mrfg_case {
+    mrfg::CoupledSystem sys;
+};
+
+namespace {
+
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+template <typename F>
+int guard(mrfg_ctx *ctx, F &&f)
+{
+    try {
+        f();
+        if (ctx) ctx->err.clear();
+        return MRFG_OK;
+    } catch (const mrfg::DictError &e) {
+        if (ctx) ctx->err = e.what();
+        return MRFG_ERR_DICT;
+    } catch (const StateError &e) {
+        if (ctx) ctx->err = e.what();
+        return MRFG_ERR_STATE;
+    } catch (const ArgError &e) {
+        if (ctx) ctx->err = e.what();
+        return MRFG_ERR_ARG;
+    } catch (const std::out_of_range &e) {
+        if (ctx) ctx->err = std::string("index out of range: ") + e.what();
+        return MRFG_ERR_ARG;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        std::string w = e.what();
+        if (w.rfind("HIP error", 0) == 0) return MRFG_ERR_DEVICE;
+        return MRFG_ERR_ARG;
+    } catch (...) {
+        if (ctx) ctx->err = "unknown error";
+        return MRFG_ERR_ARG;
+    }
+}
+
+void copyErr(const std::string &m, char *err, int len)
+{
+    if (err && len > 0) {
+        std::strncpy(err, m.c_str(), (size_t)len - 1);
+        err[len - 1] = '\0';
+    }
+}
+
+mrfg::CoupledSystem &systemOf(mrfg_ctx *ctx, int sys)
+{
+    if (sys < 0 || sys >= (int)ctx->systems.size()) throw ArgError("system index out of range");
+    return *ctx->systems[sys];
+}
+
+mrfg::Batch &committedBatch(mrfg_ctx *ctx)
+{
+    if (!ctx->committed || !ctx->batch) throw StateError("mrfg_commit has not been called");
+    return *ctx->batch;
+}
+
+void toControls(const mrfg::SolverControls &s, mrfg_controls *c)
+{
+    c->solver = s.solver == mrfg::SolverControls::PCG ? MRFG_PCG : MRFG_BICGSTAB;
+    c->precision = s.precision == mrfg::SolverControls::Mixed ? MRFG_MIXED : MRFG_FP64;
+    c->tolerance = s.tolerance;
+    c->rel_tol = s.relTol;
+    c->min_iter = s.minIter;
+    c->max_iter = s.maxIter;
+    c->inner_rel_tol = s.innerRelTol;
+    c->inner_max_iter = s.innerMaxIter;
+    c->max_outer = s.maxOuter;
+    c->check_interval = s.checkInterval;
+}
+
+mrfg::SolverControls fromControls(const mrfg_controls *c)
+{
+    mrfg::SolverControls s;
+    if (c->solver != MRFG_BICGSTAB && c->solver != MRFG_PCG) throw ArgError("unknown solver id");
+    if (c->precision != MRFG_FP64 && c->precision != MRFG_MIXED) throw ArgError("unknown precision id");
+    s.solver = c->solver == MRFG_PCG ? mrfg::SolverControls::PCG : mrfg::SolverControls::BiCGStab;
+    s.precision = c->precision == MRFG_MIXED ? mrfg::SolverControls::Mixed : mrfg::SolverControls::FP64;
+    s.tolerance = c->tolerance;
+    s.relTol = c->rel_tol;
+    s.minIter = c->min_iter;
+    s.maxIter = c->max_iter;
+    s.innerRelTol = c->inner_rel_tol;
+    s.innerMaxIter = c->inner_max_iter;
+    s.maxOuter = c->max_outer;
+    s.checkInterval = c->check_interval > 0 ? c->check_interval : 8;
+    if (s.maxIter < 0 || s.minIter < 0 || s.innerMaxIter < 1 || s.maxOuter < 1) throw ArgError("bad iteration limits");
+    return s;
+}
+
+mrfg::SolverControls controlsFromText(const char *text, const char *entry)
+{
+    if (!text || !entry) throw ArgError("null fvSolution text or entry");
+    mrfg::Dict d = mrfg::parseDict(text);
+    const mrfg::Dict &solvers = d.subDict("solvers");
+    return mrfg::parseControls(solvers.subDict(entry));
+}
+
+void fillPerf(const std::vector<mrfg::SolvePerf> &p, mrfg_perf *out)
+{
+    for (size_t i = 0; i < p.size(); ++i) {
+        out[i].initial_residual = p[i].initialResidual;
+        out[i].final_residual = p[i].finalResidual;
+        out[i].n_iterations = p[i].nIterations;
+        out[i].converged = p[i].converged;
+        out[i].singular = p[i].singular;
+        out[i].n_outer = p[i].nOuter;
+        out[i].seconds = p[i].seconds;
+        out[i].bytes_moved = p[i].bytes;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mrfg_version(void) { return "multiregionfoam_amd 0.1 (gfx950)"; }
+
+int mrfg_create(mrfg_ctx **ctx, int device)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    *ctx = nullptr;
+    auto c = std::make_unique<mrfg_ctx>();
+    c->device = device;
+    int rc = guard(c.get(), [&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("HIP error: no HIP device");
+        if (device < 0 || device >= n) throw ArgError("device index out of range");
+        c->batch = std::make_unique<mrfg::Batch>(device);
+    });
+    if (rc != MRFG_OK) return rc;
+    *ctx = c.release();
+    return MRFG_OK;
+}
+
+void mrfg_destroy(mrfg_ctx *ctx) { delete ctx; }
+
+const char *mrfg_last_error(const mrfg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mrfg_begin_system(mrfg_ctx *ctx, int n_regions, int n_interfaces, int *sys_id)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        if (n_regions < 1 || n_interfaces < 0) throw ArgError("need at least one region");
+        auto s = std::make_unique<mrfg::CoupledSystem>();
+        s->regions.resize(n_regions);
+        s->interfaces.resize(n_interfaces);
+        ctx->systems.push_back(std::move(s));
+        ctx->committed = false;
+        if (sys_id) *sys_id = (int)ctx->systems.size() - 1;
+    });
+}
+
+int mrfg_set_region(mrfg_ctx *ctx, int sys, int region, int n_cells, int n_faces, const int32_t *l_addr,
+                    const int32_t *u_addr, const double *diag, const double *lower, const double *upper,
+                    const double *source, const double *psi0)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (region < 0 || region >= (int)S.regions.size()) throw ArgError("region index out of range");
+        if (n_cells < 1 || n_faces < 0) throw ArgError("bad region sizes");
+        if (!diag || !source || !psi0 || (n_faces > 0 && (!l_addr || !u_addr || !upper)))
+            throw ArgError("null region array");
+        mrfg::LduRegion &R = S.regions[region];
+        R.name = "region" + std::to_string(region);
+        R.nCells = n_cells;
+        R.nFaces = n_faces;
+        R.lAddr.assign(l_addr, l_addr + n_faces);
+        R.uAddr.assign(u_addr, u_addr + n_faces);
+        R.diag.assign(diag, diag + n_cells);
+        R.upper.assign(upper, upper + n_faces);
+        R.symmetric = lower == nullptr;
+        if (lower) R.lower.assign(lower, lower + n_faces);
+        else R.lower = R.upper;
+        R.source.assign(source, source + n_cells);
+        R.psi.assign(psi0, psi0 + n_cells);
+        ctx->committed = false;
+    });
+}
+
+int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int region_b, int n_faces_a,
+                       const int32_t *fc_a, const double *coeff_a, const int32_t *off_a, const int32_t *idx_a,
+                       const double *w_a, int n_faces_b, const int32_t *fc_b, const double *coeff_b,
+                       const int32_t *off_b, const int32_t *idx_b, const double *w_b)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (iface < 0 || iface >= (int)S.interfaces.size()) throw ArgError("interface index out of range");
+        if (n_faces_a < 0 || n_faces_b < 0) throw ArgError("bad interface sizes");
+        if (!off_a || !off_b) throw ArgError("null GGI offsets");
+        mrfg::CoupledInterface &I = S.interfaces[iface];
+        I.name = "interface" + std::to_string(iface);
+        auto fill = [](mrfg::InterfaceSide &s, int region, int n, const int32_t *fc, const double *coeff,
+                       const int32_t *off, const int32_t *idx, const double *w) {
+            if (n > 0 && (!fc || !coeff)) throw ArgError("null interface array");
+            s.region = region;
+            s.fc.assign(fc, fc + n);
+            s.coeff.assign(coeff, coeff + n);
+            s.magSf.assign(n, 0.0);
+            s.off.assign(off, off + n + 1);
+            int np = s.off.back();
+            if (np < 0 || s.off.front() != 0) throw ArgError("bad GGI offsets");
+            for (int i = 0; i < n; ++i)
+                if (s.off[i + 1] < s.off[i]) throw ArgError("GGI offsets must be non-decreasing");
+            if (np > 0 && (!idx || !w)) throw ArgError("null GGI pair array");
+            s.idx.assign(idx, idx + np);
+            s.w.assign(w, w + np);
+        };
+        fill(I.a, region_a, n_faces_a, fc_a, coeff_a, off_a, idx_a, w_a);
+        fill(I.b, region_b, n_faces_b, fc_b, coeff_b, off_b, idx_b, w_b);
+        ctx->committed = false;
+    });
+}
+
+int mrfg_commit(mrfg_ctx *ctx)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        if (ctx->systems.empty()) throw StateError("no system defined");
+        std::vector<const mrfg::CoupledSystem *> ptrs;
+        for (auto &s : ctx->systems) {
+            try {
+                s->check();
+            } catch (const std::runtime_error &e) {
+                throw ArgError(e.what());
+            }
+            ptrs.push_back(s.get());
+        }
+        ctx->batch->setSystems(ptrs);
+        ctx->committed = true;
+    });
+}
+
+int mrfg_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_perf *perf)
+{
+    if (!ctx || !controls || !perf) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::Batch &B = committedBatch(ctx);
+        std::vector<mrfg::SolvePerf> p;
+        B.solve(fromControls(controls), p);
+        fillPerf(p, perf);
+    });
+}
+
+int mrfg_solve_dict(mrfg_ctx *ctx, const char *fv_solution, const char *entry, mrfg_perf *perf)
+{
+    if (!ctx || !perf) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::SolverControls c = controlsFromText(fv_solution, entry);
+        mrfg::Batch &B = committedBatch(ctx);
+        std::vector<mrfg::SolvePerf> p;
+        B.solve(c, p);
+        fillPerf(p, perf);
+    });
+}
+
+int mrfg_parse_controls(const char *fv_solution, const char *entry, mrfg_controls *out, char *err, int err_len)
+{
+    if (!out) return MRFG_ERR_ARG;
+    mrfg_ctx tmp;
+    int rc = guard(&tmp, [&] { toControls(controlsFromText(fv_solution, entry), out); });
+    if (rc != MRFG_OK) copyErr(tmp.err, err, err_len);
+    return rc;
+}
+
+int mrfg_get_psi(mrfg_ctx *ctx, int sys, int region, double *psi)
+{
+    if (!ctx || !psi) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).getPsi(sys, region, psi);
+    });
+}
+
+int mrfg_set_psi(mrfg_ctx *ctx, int sys, int region, const double *psi)
+{
+    if (!ctx || !psi) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (region < 0 || region >= (int)S.regions.size()) throw ArgError("region index out of range");
+        S.regions[region].psi.assign(psi, psi + S.regions[region].nCells);
+        if (ctx->committed) ctx->batch->setPsi(sys, region, psi);
+    });
+}
+
+// ---- case builder ----------------------------------------------------------
+
+int mrfg_case_build(const char *dict_text, mrfg_case **out, char *err, int err_len)
+{
+    if (!dict_text || !out) return MRFG_ERR_ARG;
+    *out = nullptr;
+    mrfg_ctx tmp;
+    auto c = std::make_unique<mrfg_case>();
+    int rc = guard(&tmp, [&] { c->sys = mrfg::buildCase(dict_text); });
+    if (rc != MRFG_OK) {
+        copyErr(tmp.err, err, err_len);
+        return rc;
+    }
+    *out = c.release();
+    return MRFG_OK;
+}
+
+void mrfg_case_free(mrfg_case *c) { delete c; }
+
+int mrfg_case_sizes(const mrfg_case *c, int *n_regions, int *n_interfaces)
+{
+    if (!c) return MRFG_ERR_ARG;
+    if (n_regions) *n_regions = (int)c->sys.regions.size();
+    if (n_interfaces) *n_interfaces = (int)c->sys.interfaces.size();
+    return MRFG_OK;
+}
+
+int mrfg_case_region_sizes(const mrfg_case *c, int region, int *n_cells, int *n_faces, int *symmetric)
+{
+    if (!c || region < 0 || region >= (int)c->sys.regions.size()) return MRFG_ERR_ARG;
+    const auto &R = c->sys.regions[region];
+    if (n_cells) *n_cells = R.nCells;
+    if (n_faces) *n_faces = R.nFaces;
+    if (symmetric) *symmetric = R.symmetric ? 1 : 0;
+    return MRFG_OK;
+}
+
+int mrfg_case_region(const mrfg_case *c, int region, int32_t *l_addr, int32_t *u_addr, double *diag, double *lower,
+                     double *upper, double *source, double *psi0)
+{
+    if (!c || region < 0 || region >= (int)c->sys.regions.size()) return MRFG_ERR_ARG;
+    const auto &R = c->sys.regions[region];
+    auto cp = [](auto *dst, const auto &src) {
+        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    cp(l_addr, R.lAddr);
+    cp(u_addr, R.uAddr);
+    cp(diag, R.diag);
+    cp(lower, R.lower);
+    cp(upper, R.upper);
+    cp(source, R.source);
+    cp(psi0, R.psi);
+    return MRFG_OK;
+}
+
+int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int *region_b, int *n_faces_a,
+                              int *n_faces_b, int *n_pairs_a, int *n_pairs_b)
+{
+    if (!c || iface < 0 || iface >= (int)c->sys.interfaces.size()) return MRFG_ERR_ARG;
+    const auto &I = c->sys.interfaces[iface];
+    if (region_a) *region_a = I.a.region;
+    if (region_b) *region_b = I.b.region;
+    if (n_faces_a) *n_faces_a = I.a.nFaces();
+    if (n_faces_b) *n_faces_b = I.b.nFaces();
+    if (n_pairs_a) *n_pairs_a = (int)I.a.w.size();
+    if (n_pairs_b) *n_pairs_b = (int)I.b.w.size();
+    return MRFG_OK;
+}
+
+int mrfg_case_interface(const mrfg_case *c, int iface, int32_t *fc_a, double *coeff_a, int32_t *off_a,
+                        int32_t *idx_a, double *w_a, int32_t *fc_b, double *coeff_b, int32_t *off_b, int32_t *idx_b,
+                        double *w_b)
+{
+    if (!c || iface < 0 || iface >= (int)c->sys.interfaces.size()) return MRFG_ERR_ARG;
+    const auto &I = c->sys.interfaces[iface];
+    auto cp = [](auto *dst, const auto &src) {
+        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    cp(fc_a, I.a.fc); cp(coeff_a, I.a.coeff); cp(off_a, I.a.off); cp(idx_a, I.a.idx); cp(w_a, I.a.w);
+    cp(fc_b, I.b.fc); cp(coeff_b, I.b.coeff); cp(off_b, I.b.off); cp(idx_b, I.b.idx); cp(w_b, I.b.w);
+    return MRFG_OK;
+}
+
+int mrfg_add_case(mrfg_ctx *ctx, const mrfg_case *c, int *sys_id)
+{
+    if (!ctx || !c) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        ctx->systems.push_back(std::make_unique<mrfg::CoupledSystem>(c->sys));
+        ctx->committed = false;
+        if (sys_id) *sys_id = (int)ctx->systems.size() - 1;
+    });
+}
+
+// ---- kernel-level -----------------------------------------------------------
+
+int mrfg_spmv(mrfg_ctx *ctx, int sys, const double *x, double *y)
+{
+    if (!ctx || !x || !y) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).spmv(x, y, sys);
+    });
+}
+
+int mrfg_dilu_factor(mrfg_ctx *ctx, int sys, double *rD)
+{
+    if (!ctx || !rD) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).diluFactor(sys, rD);
+    });
+}
+
+int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w)
+{
+    if (!ctx || !r || !w) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).precondition(r, w, sys);
+    });
+}
+
+int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms)
+{
+    if (!ctx || !ms || reps < 1 || which < 0 || which > 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] { *ms = committedBatch(ctx).timeKernel(which, reps); });
+}
+
+int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info)
+{
+    if (!ctx || !info) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        const mrfg::SystemLayout &L = committedBatch(ctx).layout(sys);
+        info[0] = L.nCells;
+        info[1] = L.nFaces;
+        info[2] = L.nPairs;
+        info[3] = L.nPos;
+        info[4] = L.nSlices;
+        info[5] = L.nLevels;
+    });
+}
+
+int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b_spmv, double *b_dilu)
+{
+    if (!ctx || (value_size != 4 && value_size != 8)) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        mrfg::Batch &B = committedBatch(ctx);
+        if (b_iter) *b_iter = B.bytesIter(sys, value_size);
+        if (b_spmv) *b_spmv = B.bytesSpmv(sys, value_size);
+        if (b_dilu) *b_dilu = B.bytesDilu(sys, value_size);
+    });
+}
+
+}  // extern "C"

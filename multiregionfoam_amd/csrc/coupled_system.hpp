@@ -1,0 +1,81 @@
+// The coupled block system handed to the solver: one LDU matrix per region
+// plus monolithic interfaces with GGI gather weights.  This is what
+// coupledFvMatrix<scalar>::solve hands to the coupled solver after
+// addBoundaryDiag / addBoundarySource (call site multiRegionSystem.C:150-153).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mrfg {
+
+struct LduRegion {
+    std::string name;
+    int nCells = 0, nFaces = 0;
+    std::vector<int32_t> lAddr, uAddr;      // owner / neighbour, upper-triangular order
+    std::vector<double> diag, lower, upper; // diag includes patch internalCoeffs
+    std::vector<double> source;             // includes non-coupled boundaryCoeffs
+    std::vector<double> psi;                // initial guess / solution
+    bool symmetric = false;
+};
+
+// One side of a monolithic interface: faces of this side gather the other
+// side's face-cell values, pnf_i = sum_p w[p] * psi_other[fcOther[idx[p]]],
+// and Amul does result[fc[i]] -= coeff[i] * pnf_i
+// (monolithicCouplingFvPatchField.C:381-464).
+struct InterfaceSide {
+    int region = -1;
+    std::vector<int32_t> fc;
+    std::vector<double> coeff;
+    std::vector<double> magSf;
+    std::vector<int32_t> off, idx;
+    std::vector<double> w;
+    int nFaces() const { return (int)fc.size(); }
+};
+
+struct CoupledInterface {
+    std::string name;
+    InterfaceSide a, b;
+};
+
+struct CoupledSystem {
+    std::vector<LduRegion> regions;
+    std::vector<CoupledInterface> interfaces;
+    long long nCells() const
+    {
+        long long n = 0;
+        for (auto &r : regions) n += r.nCells;
+        return n;
+    }
+    long long nFaces() const
+    {
+        long long n = 0;
+        for (auto &r : regions) n += r.nFaces;
+        return n;
+    }
+    long long nPairs() const
+    {
+        long long n = 0;
+        for (auto &i : interfaces) n += (long long)i.a.w.size() + (long long)i.b.w.size();
+        return n;
+    }
+    // Throws std::runtime_error if addressing is inconsistent.
+    void check() const;
+};
+
+// Build and assemble a case from its dictionary text (regions +
+// monolithicCoupledPatches).  Extra per-region data (k, cell sizes) are
+// returned for the interface flux check.
+struct CaseExtras {
+    std::vector<std::vector<double>> k;
+    std::vector<std::vector<double>> cellSize;   // 3 per region
+    std::vector<int> ifaceNormalAxis;            // per interface
+};
+
+CoupledSystem buildCase(const std::string &dictText, CaseExtras *extras = nullptr);
+
+// splitmix64 hash -> uniform [0,1)
+double hashU01(uint64_t seed, uint64_t index);
+
+}  // namespace mrfg

@@ -1,0 +1,298 @@
+"""ctypes binding to the CPU oracle (oracle/_build/libmrf_oracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "libmrf_oracle.so")
+
+SIDES = ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")
+PTYPE = {"zeroGradient": 0, "fixedValue": 1, "empty": 2, "monolithicTemperature": 3}
+FLUX = {"wall": 0, "inlet": 1, "outlet": 2}
+
+
+class RegionSpec(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 3), ("origin", C.c_double * 3), ("length", C.c_double * 3),
+        ("transport", C.c_int), ("rhoCp", C.c_double),
+        ("k_uniform", C.c_double), ("k_block", C.c_int), ("k_min", C.c_double),
+        ("k_max", C.c_double), ("k_seed", C.c_uint64),
+        ("flow_axis", C.c_int), ("profile_mask", C.c_int), ("Umax", C.c_double),
+        ("noise", C.c_double), ("vel_seed", C.c_uint64),
+        ("patch_type", C.c_int * 6), ("patch_value", C.c_double * 6),
+        ("patch_flux", C.c_int * 6), ("T0", C.c_double),
+    ]
+
+
+class IfaceSpec(C.Structure):
+    _fields_ = [("regionA", C.c_int), ("sideA", C.c_int), ("regionB", C.c_int), ("sideB", C.c_int)]
+
+
+class Ldu(C.Structure):
+    _fields_ = [
+        ("nCells", C.c_int), ("nFaces", C.c_int),
+        ("lAddr", C.POINTER(C.c_int32)), ("uAddr", C.POINTER(C.c_int32)),
+        ("diag", C.POINTER(C.c_double)), ("lower", C.POINTER(C.c_double)),
+        ("upper", C.POINTER(C.c_double)), ("source", C.POINTER(C.c_double)),
+        ("psi", C.POINTER(C.c_double)), ("symmetric", C.c_int),
+        ("k", C.POINTER(C.c_double)), ("cellSize", C.POINTER(C.c_double)),
+    ]
+
+
+class Iface(C.Structure):
+    _fields_ = [
+        ("regionA", C.c_int), ("regionB", C.c_int), ("nA", C.c_int), ("nB", C.c_int),
+        ("fcA", C.POINTER(C.c_int32)), ("fcB", C.POINTER(C.c_int32)),
+        ("coeffA", C.POINTER(C.c_double)), ("coeffB", C.POINTER(C.c_double)),
+        ("offA", C.POINTER(C.c_int32)), ("idxA", C.POINTER(C.c_int32)),
+        ("wA", C.POINTER(C.c_double)),
+        ("offB", C.POINTER(C.c_int32)), ("idxB", C.POINTER(C.c_int32)),
+        ("wB", C.POINTER(C.c_double)),
+        ("magSfA", C.POINTER(C.c_double)), ("magSfB", C.POINTER(C.c_double)),
+    ]
+
+
+class System(C.Structure):
+    _fields_ = [("nRegions", C.c_int), ("nInterfaces", C.c_int),
+                ("regions", C.POINTER(Ldu)), ("interfaces", C.POINTER(Iface))]
+
+
+class Perf(C.Structure):
+    _fields_ = [("initialResidual", C.c_double), ("finalResidual", C.c_double),
+                ("nIterations", C.c_int), ("converged", C.c_int), ("singular", C.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_build.argtypes = [C.c_int, C.POINTER(RegionSpec), C.c_int, C.POINTER(IfaceSpec),
+                                C.POINTER(System)]
+        L.orc_build.restype = C.c_int
+        L.orc_free.argtypes = [C.POINTER(System)]
+        L.orc_solve.argtypes = [C.POINTER(System), C.c_int, C.c_double, C.c_double, C.c_int,
+                                C.c_int, C.POINTER(Perf)]
+        PP = C.POINTER(C.POINTER(C.c_double))
+        L.orc_amul.argtypes = [C.POINTER(System), PP, PP]
+        L.orc_sumA.argtypes = [C.POINTER(System), PP]
+        L.orc_norm_factor.argtypes = [C.POINTER(System), PP, PP, PP, PP]
+        L.orc_norm_factor.restype = C.c_double
+        L.orc_dilu_factor.argtypes = [C.POINTER(System), PP]
+        L.orc_dilu_precondition.argtypes = [C.POINTER(System), PP, PP, PP]
+        L.orc_interface_flux.argtypes = [C.POINTER(System), C.c_int, PP] + [C.POINTER(C.c_double)] * 4
+        L.orc_u01.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_u01.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _spec(r):
+    s = RegionSpec()
+    for a in range(3):
+        s.n[a] = r["nCells"][a]
+        s.origin[a] = r["origin"][a]
+        s.length[a] = r["length"][a]
+    s.transport = 1 if r["type"] == "transportTemperature" else 0
+    s.rhoCp = r["rhoCp"]
+    k = r["k"]
+    if "uniform" in k:
+        s.k_uniform = float(k["uniform"])
+        s.k_block = 0
+    else:
+        s.k_block = int(k["block"])
+        s.k_min, s.k_max, s.k_seed = float(k["min"]), float(k["max"]), int(k["seed"])
+    v = r["velocity"]
+    if v:
+        s.flow_axis = int(v["axis"])
+        s.profile_mask = sum(1 << a for a in v.get("profile", ()))
+        s.Umax = float(v["Umax"])
+        s.noise = float(v.get("noise", 0.0))
+        s.vel_seed = int(v.get("seed", 0))
+    for i, side in enumerate(SIDES):
+        p = r["patches"][side]
+        s.patch_type[i] = PTYPE[p["type"]]
+        s.patch_value[i] = float(p.get("value", 0.0) or 0.0)
+        s.patch_flux[i] = FLUX[p.get("flux", "wall")]
+    s.T0 = r["T0"]
+    return s
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class OracleSystem:
+    """An assembled coupled system held by the oracle library."""
+
+    def __init__(self, case):
+        L = lib()
+        names = [r["name"] for r in case["regions"]]
+        specs = (RegionSpec * len(names))(*[_spec(r) for r in case["regions"]])
+        ifs = (IfaceSpec * max(1, len(case["interfaces"])))()
+        for i, it in enumerate(case["interfaces"]):
+            ifs[i].regionA = names.index(it["A"][0])
+            ifs[i].sideA = SIDES.index(it["A"][1])
+            ifs[i].regionB = names.index(it["B"][0])
+            ifs[i].sideB = SIDES.index(it["B"][1])
+        self.sys = System()
+        rc = L.orc_build(len(names), specs, len(case["interfaces"]), ifs, C.byref(self.sys))
+        if rc != 0:
+            raise RuntimeError("orc_build failed (%d)" % rc)
+        self.names = names
+        self._load()
+
+    def _load(self):
+        s = self.sys
+        self.regions = []
+        for r in range(s.nRegions):
+            R = s.regions[r]
+            n, f = R.nCells, R.nFaces
+            self.regions.append(dict(
+                nCells=n, nFaces=f, lAddr=_arr(R.lAddr, f, np.int32), uAddr=_arr(R.uAddr, f, np.int32),
+                diag=_arr(R.diag, n, np.float64), lower=_arr(R.lower, f, np.float64),
+                upper=_arr(R.upper, f, np.float64), source=_arr(R.source, n, np.float64),
+                psi=_arr(R.psi, n, np.float64), symmetric=bool(R.symmetric),
+                k=_arr(R.k, n, np.float64), cellSize=_arr(R.cellSize, 3, np.float64)))
+        self.interfaces = []
+        for i in range(s.nInterfaces):
+            I = s.interfaces[i]
+            nA, nB = I.nA, I.nB
+            offA = _arr(I.offA, nA + 1, np.int32)
+            offB = _arr(I.offB, nB + 1, np.int32)
+            self.interfaces.append(dict(
+                regionA=I.regionA, regionB=I.regionB, nA=nA, nB=nB,
+                fcA=_arr(I.fcA, nA, np.int32), fcB=_arr(I.fcB, nB, np.int32),
+                coeffA=_arr(I.coeffA, nA, np.float64), coeffB=_arr(I.coeffB, nB, np.float64),
+                offA=offA, idxA=_arr(I.idxA, int(offA[-1]), np.int32), wA=_arr(I.wA, int(offA[-1]), np.float64),
+                offB=offB, idxB=_arr(I.idxB, int(offB[-1]), np.int32), wB=_arr(I.wB, int(offB[-1]), np.float64),
+                magSfA=_arr(I.magSfA, nA, np.float64), magSfB=_arr(I.magSfB, nB, np.float64)))
+
+    def __del__(self):
+        try:
+            lib().orc_free(C.byref(self.sys))
+        except Exception:
+            pass
+
+    # ---- vector helpers: a coupled vector is a list of per-region arrays ----
+    def _pp(self, vecs):
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in vecs]
+        ptrs = (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+        return arrs, ptrs
+
+    def zeros(self):
+        return [np.zeros(r["nCells"]) for r in self.regions]
+
+    def amul(self, x):
+        xa, xp = self._pp(x)
+        ya, yp = self._pp(self.zeros())
+        lib().orc_amul(C.byref(self.sys), xp, yp)
+        return ya
+
+    def sumA(self):
+        sa, sp = self._pp(self.zeros())
+        lib().orc_sumA(C.byref(self.sys), sp)
+        return sa
+
+    def norm_factor(self, x, b, Ax):
+        xa, xp = self._pp(x)
+        ba, bp = self._pp(b)
+        aa, ap = self._pp(Ax)
+        ta, tp = self._pp(self.zeros())
+        return lib().orc_norm_factor(C.byref(self.sys), xp, bp, ap, tp)
+
+    def dilu_factor(self):
+        ra, rp = self._pp(self.zeros())
+        lib().orc_dilu_factor(C.byref(self.sys), rp)
+        return ra
+
+    def dilu_precondition(self, rD, r):
+        da, dp = self._pp(rD)
+        ra, rp = self._pp(r)
+        wa, wp = self._pp(self.zeros())
+        lib().orc_dilu_precondition(C.byref(self.sys), dp, wp, rp)
+        return wa
+
+    def interface_flux(self, i, psi):
+        pa, pp = self._pp(psi)
+        out = [C.c_double() for _ in range(4)]
+        lib().orc_interface_flux(C.byref(self.sys), i, pp, *[C.byref(o) for o in out])
+        return tuple(o.value for o in out)
+
+    def solve(self, solver="BiCGStab", tol=1e-10, relTol=0.0, minIter=0, maxIter=1000, psi0=None):
+        """Solve in place from psi0 (default: the case's T0); returns
+        (psi list, perf dict)."""
+        s = self.sys
+        for r in range(s.nRegions):
+            src = psi0[r] if psi0 is not None else self.regions[r]["psi"]
+            dst = np.ctypeslib.as_array(s.regions[r].psi, shape=(s.regions[r].nCells,))
+            dst[:] = src
+        perf = Perf()
+        kind = 1 if solver in ("PCG", "CG", "hipPCG") else 0
+        lib().orc_solve(C.byref(s), kind, tol, relTol, minIter, maxIter, C.byref(perf))
+        psi = [_arr(s.regions[r].psi, s.regions[r].nCells, np.float64) for r in range(s.nRegions)]
+        return psi, dict(initialResidual=perf.initialResidual, finalResidual=perf.finalResidual,
+                         nIterations=perf.nIterations, converged=bool(perf.converged),
+                         singular=bool(perf.singular))
+
+    # ---- global (concatenated) sparse matrix, for brute-force checks ----
+    def offsets(self):
+        off = [0]
+        for r in self.regions:
+            off.append(off[-1] + r["nCells"])
+        return off
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        off = self.offsets()
+        rows, cols, vals = [], [], []
+        for g, R in enumerate(self.regions):
+            o = off[g]
+            n = R["nCells"]
+            rows.append(np.arange(n) + o); cols.append(np.arange(n) + o); vals.append(R["diag"])
+            rows.append(R["uAddr"] + o); cols.append(R["lAddr"] + o); vals.append(R["lower"])
+            rows.append(R["lAddr"] + o); cols.append(R["uAddr"] + o); vals.append(R["upper"])
+        for I in self.interfaces:
+            for side in (0, 1):
+                n = I["nB"] if side else I["nA"]
+                off_ = I["offB"] if side else I["offA"]
+                idx = I["idxB"] if side else I["idxA"]
+                w = I["wB"] if side else I["wA"]
+                fc = I["fcB"] if side else I["fcA"]
+                fcN = I["fcA"] if side else I["fcB"]
+                coeff = I["coeffB"] if side else I["coeffA"]
+                rO = off[I["regionB"] if side else I["regionA"]]
+                rN = off[I["regionA"] if side else I["regionB"]]
+                cnt = np.diff(off_)
+                face = np.repeat(np.arange(n), cnt)
+                rows.append(fc[face] + rO)
+                cols.append(fcN[idx] + rN)
+                vals.append(-coeff[face] * w)
+        N = off[-1]
+        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+
+    def source_vector(self):
+        return np.concatenate([r["source"] for r in self.regions])
+
+    def split(self, v):
+        off = self.offsets()
+        return [np.array(v[off[g]:off[g + 1]]) for g in range(len(self.regions))]

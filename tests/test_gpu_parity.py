@@ -1,0 +1,169 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Kernel-level checks are bit-exact (FP contraction is off in both builds and
+each row is evaluated in the face-loop order of the oracle).  Solves are
+compared on converged fields: relative L2 <= 1e-10 to the oracle's field
+with the same iteration count +-2 (FP64), <= 1e-8 for mixed precision.
+"""
+import numpy as np
+import pytest
+
+from multiregionfoam_amd import cases
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-10
+MIXED_TOL = 1e-8
+
+
+def small_cases():
+    return {
+        "plate2d": cases.plate2d(),
+        "duct_ggi": cases.duct3d(nf=(24, 12, 12), ns=(30, 15, 5)),
+        "duct_conformal": cases.duct3d(nf=(20, 10, 8), ns=(20, 10, 4), conformal=True),
+        "exchanger3": cases.exchanger3(n=(20, 8, 6)),
+    }
+
+
+def _ctx(m, case):
+    A = m.CoupledFvMatrix(0)
+    A.add_case(case)
+    A.commit()
+    return A
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", list(small_cases()))
+def test_spmv_bitexact(gpu, name):
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    A = _ctx(gpu, case)
+    rng = np.random.default_rng(1)
+    x = [rng.uniform(250, 400, r["nCells"]) for r in O.regions]
+    y = A.spmv(x)
+    y_ref = np.concatenate(O.amul(x))
+    assert np.array_equal(y, y_ref), np.abs(y - y_ref).max()
+
+
+@pytest.mark.parametrize("name", list(small_cases()))
+def test_dilu_bitexact(gpu, name):
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    A = _ctx(gpu, case)
+    rD_ref = O.dilu_factor()
+    rD = A.dilu_factor()
+    assert np.array_equal(rD, np.concatenate(rD_ref))
+    rng = np.random.default_rng(2)
+    r = [rng.standard_normal(n["nCells"]) for n in O.regions]
+    w = A.precondition(r)
+    w_ref = np.concatenate(O.dilu_precondition(rD_ref, r))
+    assert np.array_equal(w, w_ref), np.abs(w - w_ref).max()
+
+
+@pytest.mark.parametrize("name", list(small_cases()))
+def test_bicgstab_fp64_matches_oracle(gpu, name):
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    psi_ref, perf_ref = O.solve(tol=1e-10, maxIter=1000)
+    A = _ctx(gpu, case)
+    perf = A.solve(solver="BiCGStab", tolerance=1e-10, maxIter=1000)[0]
+    assert perf.converged and perf_ref["converged"]
+    assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
+    assert perf.initial_residual == pytest.approx(perf_ref["initialResidual"], rel=1e-12)
+    err = rel(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
+    assert err <= FIELD_TOL, err
+
+
+def test_pcg_symmetric(gpu):
+    # two conducting slabs: symmetric block system -> PCG
+    case = cases.duct3d(nf=(16, 8, 8), ns=(20, 10, 4))
+    case["regions"][0]["type"] = "conductTemperature"
+    case["regions"][0]["velocity"] = None
+    case["regions"][0]["patches"]["xmin"]["flux"] = "wall"
+    case["regions"][0]["patches"]["xmax"]["flux"] = "wall"
+    O = ob.OracleSystem(case)
+    psi_ref, perf_ref = O.solve(solver="PCG", tol=1e-10, maxIter=1000)
+    A = _ctx(gpu, case)
+    perf = A.solve(solver="PCG", tolerance=1e-10, maxIter=1000)[0]
+    assert perf.converged and perf_ref["converged"]
+    assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
+    assert rel(np.concatenate(A.psi(0)), np.concatenate(psi_ref)) <= FIELD_TOL
+
+
+@pytest.mark.parametrize("name", ["plate2d", "duct_ggi", "exchanger3"])
+def test_mixed_precision_refinement(gpu, name):
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    psi_ref, perf_ref = O.solve(tol=1e-12, maxIter=2000)
+    A = _ctx(gpu, case)
+    perf = A.solve(solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
+                   innerRelTol=1e-4)[0]
+    assert perf.converged and perf.n_outer >= 1
+    err = rel(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
+    assert err <= MIXED_TOL, err
+
+
+def test_batch_equals_alone(gpu):
+    """Each system's result is bit-identical alone and inside a batch."""
+    cs = [cases.box_pair(4000, 3000, 20.0, 11), cases.box_pair(9000, 2000, 300.0, 12),
+          cases.box_pair(1500, 6000, 3.0, 13, ggi=True)]
+    alone = []
+    for c in cs:
+        A = _ctx(gpu, c)
+        p = A.solve(tolerance=1e-10, maxIter=1000)[0]
+        alone.append((p, np.concatenate(A.psi(0))))
+    B = gpu.CoupledFvMatrix(0)
+    for c in cs:
+        B.add_case(c)
+    B.commit()
+    perfs = B.solve(tolerance=1e-10, maxIter=1000)
+    for q, (p0, x0) in enumerate(alone):
+        assert perfs[q].n_iterations == p0.n_iterations
+        assert perfs[q].final_residual == p0.final_residual
+        assert np.array_equal(np.concatenate(B.psi(q)), x0)
+
+
+def test_fvsolution_dictionary_selects_solver(gpu):
+    case = small_cases()["plate2d"]
+    fvs = """
+    solvers
+    {
+        Tcoupled
+        {
+            solver           hipBiCGStab;
+            preconditioner   { preconditioner Cholesky; }
+            tolerance        1e-10;
+            relTol           0;
+            minIter          0;
+            maxIter          300;
+        };
+    }
+    """
+    A = _ctx(gpu, case)
+    perf = A.solve(fv_solution=fvs, entry="Tcoupled")[0]
+    assert perf.converged
+    O = ob.OracleSystem(case)
+    _, perf_ref = O.solve(tol=1e-10, maxIter=300)
+    assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
+
+
+def test_full_size_duct_residual(gpu):
+    """1.4M-cell GGI duct: SpMV bit-exact at full size and the GPU's
+    converged field has a true (oracle-evaluated) residual below tolerance."""
+    case = cases.duct3d()
+    O = ob.OracleSystem(case)
+    A = _ctx(gpu, case)
+    x = [r["psi"] + np.linspace(0.0, 1.0, r["nCells"]) for r in O.regions]
+    assert np.array_equal(A.spmv(x), np.concatenate(O.amul(x)))
+    perf = A.solve(tolerance=1e-10, maxIter=1000)[0]
+    assert perf.converged
+    psi = A.psi(0)
+    b = [r["source"] for r in O.regions]
+    Ax = O.amul(psi)
+    nf = O.norm_factor([r["psi"] for r in O.regions], b, O.amul([r["psi"] for r in O.regions]))
+    res = sum(np.abs(bb - aa).sum() for bb, aa in zip(b, Ax)) / nf
+    assert res < 1e-10 * 1.5, res
