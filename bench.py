@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: coupled solves/s of the monolithic interface-coupled solve.
+
+Workload: the 3-D two-region coupled temperature solve -- fluid duct
+160x80x80 (1.024M cells) over a solid slab 200x100x19 (380k cells, face grid
+1.25x refined and offset 0.37 fluid cells), non-conformal GGI interface,
+1.404M cells; DILU-preconditioned BiCGStab to tolerance 1e-10 (normalised
+L1 residual), synthetic seeded fields.  One step = one coupled solve from
+the initial field (factorisation + setup + iterations) on data resident in
+HBM.  `value` is FP64 solves/s; the mixed FP32/FP64 refinement result is
+reported beside it with its error against the FP64 oracle.
+
+Prints ONE JSON line.  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pct(v, q):
+    v = sorted(v)
+    if not v:
+        return None
+    k = (len(v) - 1) * q
+    f = int(k)
+    c = min(f + 1, len(v) - 1)
+    return v[f] + (v[c] - v[f]) * (k - f)
+
+
+def run_steps(A, ctrl, warmup, steps):
+    for _ in range(warmup):
+        A.reset_psi()
+        A.solve(ctrl)
+    lat, perf = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        A.reset_psi()
+        perf = A.solve(ctrl)[0]
+        lat.append(time.perf_counter() - s0)
+    total = time.perf_counter() - t0
+    return total, lat, perf
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus %d is not run "
+              "(the multi-GPU sharding is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import multiregionfoam_amd as m
+    from multiregionfoam_amd import cases
+
+    case = cases.duct3d()
+    A = m.CoupledFvMatrix(0)
+    A.add_case(case)
+    A.commit()
+    info = A.layout_info(0)
+    bm64 = A.bytes_model(0, 8)
+    bm32 = A.bytes_model(0, 4)
+
+    fp64 = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=1000)
+    total, lat, perf = run_steps(A, fp64, args.warmup, args.steps)
+    psi64 = np.concatenate(A.psi(0))
+    ms_step = 1e3 * total / args.steps
+    solves_s = args.steps / total
+    gbs = perf.n_iterations * bm64["iter"] / (total / args.steps) / 1e9
+
+    mixed = None
+    if not args.no_mixed:
+        mctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
+                           innerRelTol=1e-4)
+        mt, mlat, mperf = run_steps(A, mctrl, args.warmup, args.steps)
+        psim = np.concatenate(A.psi(0))
+        mixed = dict(value=args.steps / mt, ms_per_step=1e3 * mt / args.steps,
+                     speedup_vs_fp64=(args.steps / mt) / solves_s,
+                     inner_iterations=mperf.n_iterations, outer_steps=mperf.n_outer,
+                     final_residual=mperf.final_residual, converged=mperf.converged,
+                     p50_ms=1e3 * pct(mlat, 0.5), p99_ms=1e3 * pct(mlat, 0.99))
+
+    # ---- per-kernel timing (HIP events on the library stream) ----
+    A.reset_psi()
+    k_spmv = A.time_kernel(0, args.kernel_reps)
+    k_fwd = A.time_kernel(1, args.kernel_reps)
+    k_bwd = A.time_kernel(2, args.kernel_reps)
+    per_iter = {"spmv": 2 * k_spmv, "dilu_fwd": 2 * k_fwd, "dilu_bwd": 2 * k_bwd}
+    dominant = max(per_iter, key=per_iter.get)
+    launch_ms = {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd}[dominant]
+    launch_bytes = bm64["spmv"] if dominant == "spmv" else 0.5 * bm64["dilu"]
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    kernel_names = {"spmv": "k_spmv_plain<double>", "dilu_fwd": "k_dilu<double, 0>",
+                    "dilu_bwd": "k_dilu<double, 1>"}
+
+    # ---- CPU baseline: the oracle (C restatement), one full FP64 solve ----
+    cpu = None
+    err64 = errm = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind
+        O = oracle_bind.OracleSystem(case)
+        t0 = time.perf_counter()
+        psi_ref, pref = O.solve(tol=1e-10, maxIter=1000)
+        tc = time.perf_counter() - t0
+        psi_ref = np.concatenate(psi_ref)
+        err64 = float(np.linalg.norm(psi64 - psi_ref) / np.linalg.norm(psi_ref))
+        if mixed is not None:
+            errm = float(np.linalg.norm(psim - psi_ref) / np.linalg.norm(psi_ref))
+            mixed["rel_l2_vs_fp64_oracle"] = errm
+        cpu = dict(value=1.0 / tc, unit="solves/s", cores=1, kind="port",
+                   sample="one full FP64 DILU-BiCGStab solve of the 1.404M-cell duct on one host core "
+                          "(%d iterations, %.2f s)" % (pref["nIterations"], tc),
+                   iterations=pref["nIterations"])
+
+    out = {
+        "metric": "coupled solves/s + HBM GB/s at 1.4M cells; p99 latency; mixed vs FP64",
+        "value": solves_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded case generator)",
+        "config": {"workload": "duct3d: fluid 160x80x80 + solid 200x100x19, GGI interface "
+                               "(0.37-cell offset, 1.25x refined), DILU-BiCGStab tol 1e-10",
+                   "cells": info["cells"], "internal_faces": info["faces"], "ggi_pairs": info["pairs"],
+                   "dilu_levels": info["levels"]},
+        "iterations": perf.n_iterations,
+        "final_residual": perf.final_residual,
+        "converged": perf.converged,
+        "hbm_gbs_algorithmic": gbs,
+        "p50_ms": 1e3 * pct(lat, 0.5),
+        "p99_ms": 1e3 * pct(lat, 0.99),
+        "rel_l2_vs_fp64_oracle": err64,
+        "mixed": mixed,
+        "kernels_ms": launch_ms and {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd},
+        "roofline": {"bound": "hbm", "kernel": kernel_names[dominant], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": launch_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -124,6 +124,10 @@ int mrfg_solve_dict(mrfg_ctx *ctx, const char *fv_solution, const char *entry, m
 int mrfg_parse_controls(const char *fv_solution, const char *entry, mrfg_controls *out,
                         char *err, int err_len);
 
+/* Reset every system's field to the psi0 given at definition (a device
+   copy; repeated solves from the same initial field). */
+int mrfg_reset_psi(mrfg_ctx *ctx);
+
 /* Field in / out in the region's own cell order (the psi that
    coupledFvMatrix::solve writes back into each region's T). */
 int mrfg_get_psi(mrfg_ctx *ctx, int sys, int region, double *psi);
@@ -161,7 +165,9 @@ int mrfg_spmv(mrfg_ctx *ctx, int sys, const double *x, double *y);
 int mrfg_dilu_factor(mrfg_ctx *ctx, int sys, double *rD);
 /* w = M^-1 r with the DILU preconditioner (coupled Cholesky precondition). */
 int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w);
-/* Average device time (ms) of `reps` launches: which 0 = SpMV, 1 = DILU apply. */
+/* Average device time (ms) of one launch over `reps` launches, from HIP
+   events on the context's stream: which 0 = SpMV, 1 = DILU forward sweep,
+   2 = DILU backward sweep, 3 = DILU factorisation. */
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms);
 /* info[0..5] = cells, internal faces, GGI pairs, rows incl. padding,
    64-row slices, DILU levels. */

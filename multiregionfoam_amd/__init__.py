@@ -81,6 +81,7 @@ def lib():
         "mrfg_solve_dict": ([vp, C.c_char_p, C.c_char_p, C.POINTER(Perf)], C.c_int),
         "mrfg_parse_controls": ([C.c_char_p, C.c_char_p, C.POINTER(Controls), C.c_char_p, C.c_int], C.c_int),
         "mrfg_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
+        "mrfg_reset_psi": ([vp], C.c_int),
         "mrfg_set_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_case_build": ([C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int], C.c_int),
         "mrfg_case_free": ([vp], None),
@@ -109,7 +110,7 @@ def lib():
 
 EXPORTED = (
     "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region "
-    "mrfg_set_interface mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi "
+    "mrfg_set_interface mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_time_kernel mrfg_layout_info mrfg_bytes").split()
@@ -336,6 +337,10 @@ class CoupledFvMatrix:
             self._check(lib().mrfg_get_psi(self._h, sys, r, _p(a, C.c_double)))
             out.append(a)
         return out
+
+    def reset_psi(self):
+        """Reset every system's field to its initial guess (device copy)."""
+        self._check(lib().mrfg_reset_psi(self._h))
 
     def set_psi(self, sys, psi):
         for r, a in enumerate(psi):

@@ -249,6 +249,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     sumA_.upload(sumA, st);
     b64_.upload(source, st);
     x64_.upload(psi0, st);
+    x0_.upload(psi0, st);
     d64_.alloc(nPos_);
     d64_.zero(st);
     for (auto &v : v64_) {
@@ -496,10 +497,13 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
 {
     HIP_OK(hipSetDevice(device_));
     if (c.precision == SolverControls::Mixed) ensureFloat();
-    factorise();
     HIP_OK(hipStreamSynchronize(stream_));
-    initState(c);
     auto t0 = std::chrono::steady_clock::now();
+    // the preconditioner is built per solve call, as the coupled solver
+    // object is constructed per coupledFvMatrix::solve call
+    factorised_ = false;
+    factorise();
+    initState(c);
     std::vector<int> inner;
     if (c.precision == SolverControls::Mixed) runMixed(c, inner);
     else runFp64(c);
@@ -525,6 +529,11 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
             p.bytes = p.nIterations * bytesIter((int)q, 8);
         }
     }
+}
+
+void Batch::resetPsi()
+{
+    HIP_OK(hipMemcpyAsync(x64_.p, x0_.p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
 }
 
 void Batch::setPsi(int sys, int region, const double *psi)
@@ -603,31 +612,45 @@ double Batch::timeKernel(int which, int reps)
     HIP_OK(hipSetDevice(device_));
     factorise();
     std::vector<SysState> hs(sys_.size());
-    for (auto &s : hs) s.status = kRunning;
+    for (auto &x : hs) x.status = kRunning;
     HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), hs.size() * sizeof(SysState), hipMemcpyHostToDevice, stream_));
     LevelView LV = makeLevelView(sliceSys_.p, sliceLevel_.p, levelCount_.p, order_.p, cntA_.p, cntB_.p, err_.p,
                                  nLevelsTotal_, nOrder_);
     Krylov<double> K;
     bindKrylov(K, false);
     MatView<double> A = view(m64_);
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    auto launch = [&]() {
-        if (which == 0) HIP_OK(launchSpmvPlain<double>(A, blocks_.p, nBlk_, x64_.p, v64_[0].p, stream_));
-        else HIP_OK(launchPrecondition<double>(A, LV, K, b64_.p, v64_[9].p, stream_));
+    const int nE = 2 * reps + 2;
+    std::vector<hipEvent_t> ev(nE);
+    for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+    // DILU sweeps are timed as forward/backward pairs (the counters of one
+    // sweep are reset by the other), each sweep between its own events.
+    auto one = [&](int i) {
+        if (which == 0) {
+            HIP_OK(hipEventRecord(ev[2 * i], stream_));
+            HIP_OK(launchSpmvPlain<double>(A, blocks_.p, nBlk_, x64_.p, v64_[0].p, stream_));
+            HIP_OK(hipEventRecord(ev[2 * i + 1], stream_));
+        } else if (which == 3) {
+            HIP_OK(hipEventRecord(ev[2 * i], stream_));
+            HIP_OK(launchFactor<double>(A, LV, K, A, d64_.p, cellOfPos_.p, cntC_.p, stream_));
+            HIP_OK(hipEventRecord(ev[2 * i + 1], stream_));
+        } else {
+            HIP_OK(launchDiluSweeps<double>(A, LV, K, b64_.p, v64_[9].p, stream_, which == 1 ? ev[2 * i] : nullptr,
+                                             which == 1 ? ev[2 * i + 1] : nullptr,
+                                             which == 2 ? ev[2 * i] : nullptr, which == 2 ? ev[2 * i + 1] : nullptr));
+        }
     };
-    launch();
-    HIP_OK(hipEventRecord(e0, stream_));
-    for (int i = 0; i < reps; ++i) launch();
-    HIP_OK(hipEventRecord(e1, stream_));
-    HIP_OK(hipEventSynchronize(e1));
+    one(reps);   // warm-up into the spare event pair
+    for (int i = 0; i < reps; ++i) one(i);
+    HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
-    float ms = 0;
-    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return ms / reps;
+    double tot = 0;
+    for (int i = 0; i < reps; ++i) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        tot += ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return tot / reps;
 }
 
 double Batch::bytesSpmv(int sys, int sv) const

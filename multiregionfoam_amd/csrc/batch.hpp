@@ -84,6 +84,8 @@ public:
     // Solve every system; perf gets one entry per system.
     void solve(const SolverControls &c, std::vector<SolvePerf> &perf);
 
+    // Reset every system's field to the committed initial guess (device copy).
+    void resetPsi();
     // Per-region initial guess / solution of one system (original cell order).
     void setPsi(int sys, int region, const double *psi);
     void getPsi(int sys, int region, double *psi);
@@ -93,8 +95,10 @@ public:
     void diluFactor(int sys, double *rDHost);                        // rD per cell
     void precondition(const double *rHost, double *wHost, int sys);   // w = M^-1 r
 
-    // Time `reps` repetitions of one kernel family on resident data; returns
-    // the average device time per launch (ms).  which: 0 SpMV, 1 DILU fwd+bwd.
+    // Time `reps` repetitions of one kernel on resident data; returns the
+    // average device time per launch (ms), measured with HIP events on the
+    // library's stream.  which: 0 SpMV, 1 DILU forward sweep, 2 DILU
+    // backward sweep, 3 DILU factor.
     double timeKernel(int which, int reps);
 
     const SystemLayout &layout(int sys) const { return *lay_[sys]; }
@@ -139,7 +143,7 @@ private:
     std::vector<double> hEllVal_, hDiag_, hIfCoeff_, hIfPairW_, hLVal_, hLProd_, hUVal_;
     DevMatrix<double> m64_;
     DevMatrix<float> m32_;
-    DevBuf<double> sumA_, b64_, x64_, d64_;
+    DevBuf<double> sumA_, b64_, x64_, x0_, d64_;
     DevBuf<double> v64_[12];
     DevBuf<float> v32_[14];
     DevBuf<SysState> st64_, st32_;

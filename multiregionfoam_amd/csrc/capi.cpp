@@ -292,6 +292,12 @@ int mrfg_parse_controls(const char *fv_solution, const char *entry, mrfg_control
     return rc;
 }
 
+int mrfg_reset_psi(mrfg_ctx *ctx)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] { committedBatch(ctx).resetPsi(); });
+}
+
 int mrfg_get_psi(mrfg_ctx *ctx, int sys, int region, double *psi)
 {
     if (!ctx || !psi) return MRFG_ERR_ARG;
@@ -436,7 +442,7 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w)
 
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms)
 {
-    if (!ctx || !ms || reps < 1 || which < 0 || which > 1) return MRFG_ERR_ARG;
+    if (!ctx || !ms || reps < 1 || which < 0 || which > 3) return MRFG_ERR_ARG;
     return guard(ctx, [&] { *ms = committedBatch(ctx).timeKernel(which, reps); });
 }
 

@@ -585,6 +585,19 @@ hipError_t launchPrecondition(const MatView<T> &A, const LevelView &LV, const Kr
 }
 
 template <typename T>
+hipError_t launchDiluSweeps(const MatView<T> &A, const LevelView &LV, const Krylov<T> &K, const T *in, T *out,
+                            hipStream_t st, hipEvent_t f0, hipEvent_t f1, hipEvent_t b0, hipEvent_t b1)
+{
+    if (f0) (void)hipEventRecord(f0, st);
+    k_dilu<T, kDiluFwd><<<LV.nOrder, 64, 0, st>>>(A, LV, K.info, K.state, K.rD, in, out, LV.cntA, LV.cntB);
+    if (f1) (void)hipEventRecord(f1, st);
+    if (b0) (void)hipEventRecord(b0, st);
+    k_dilu<T, kDiluBwd><<<LV.nOrder, 64, 0, st>>>(A, LV, K.info, K.state, K.rD, out, out, LV.cntB, LV.cntA);
+    if (b1) (void)hipEventRecord(b1, st);
+    MRFG_CHECK_LAUNCH();
+}
+
+template <typename T>
 hipError_t launchBicgIteration(const MatView<T> &A, const LevelView &LV, const Krylov<T> &K, hipStream_t st)
 {
     k_bicg_p<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
@@ -657,6 +670,8 @@ hipError_t launchFillInt(int *a, int n, int v, hipStream_t st)
                                         const MatView<double> &, double *, const int32_t *, int *, hipStream_t);     \
     template hipError_t launchPrecondition<T>(const MatView<T> &, const LevelView &, const Krylov<T> &, const T *,   \
                                               T *, hipStream_t);                                                     \
+    template hipError_t launchDiluSweeps<T>(const MatView<T> &, const LevelView &, const Krylov<T> &, const T *, T *, \
+                                            hipStream_t, hipEvent_t, hipEvent_t, hipEvent_t, hipEvent_t);            \
     template hipError_t launchBicgIteration<T>(const MatView<T> &, const LevelView &, const Krylov<T> &,             \
                                                hipStream_t);                                                         \
     template hipError_t launchPcgIteration<T>(const MatView<T> &, const LevelView &, const Krylov<T> &, hipStream_t);

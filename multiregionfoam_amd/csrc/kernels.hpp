@@ -33,6 +33,10 @@ hipError_t launchFactor(const MatView<T> &A, const LevelView &LV, const Krylov<T
 template <typename T>
 hipError_t launchPrecondition(const MatView<T> &A, const LevelView &LV, const Krylov<T> &K, const T *in, T *out,
                               hipStream_t st);
+// Forward then backward sweep with optional events around each sweep.
+template <typename T>
+hipError_t launchDiluSweeps(const MatView<T> &A, const LevelView &LV, const Krylov<T> &K, const T *in, T *out,
+                            hipStream_t st, hipEvent_t f0, hipEvent_t f1, hipEvent_t b0, hipEvent_t b1);
 template <typename T>
 hipError_t launchBicgIteration(const MatView<T> &A, const LevelView &LV, const Krylov<T> &K, hipStream_t st);
 template <typename T>
