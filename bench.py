@@ -106,8 +106,8 @@ def main():
     launch_ms = {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd}[dominant]
     launch_bytes = bm64["spmv"] if dominant == "spmv" else 0.5 * bm64["dilu"]
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    kernel_names = {"spmv": "k_spmv_plain<double>", "dilu_fwd": "k_dilu<double, 0>",
-                    "dilu_bwd": "k_dilu<double, 1>"}
+    kernel_names = {"spmv": "k_spmv_plain<double>", "dilu_fwd": "k_dilu_wave<double, 3, 0>",
+                    "dilu_bwd": "k_dilu_wave<double, 3, 1>"}
 
     # ---- CPU baseline: the oracle (C restatement), one full FP64 solve ----
     cpu = None
@@ -145,7 +145,7 @@ def main():
         "config": {"workload": "duct3d: fluid 160x80x80 + solid 200x100x19, GGI interface "
                                "(0.37-cell offset, 1.25x refined), DILU-BiCGStab tol 1e-10",
                    "cells": info["cells"], "internal_faces": info["faces"], "ggi_pairs": info["pairs"],
-                   "dilu_levels": info["levels"]},
+                   "dilu_waves": info["waves"], "rows_padded": info["rows"]},
         "iterations": perf.n_iterations,
         "final_residual": perf.final_residual,
         "converged": perf.converged,

@@ -107,6 +107,13 @@ int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int regi
                        int n_faces_b, const int32_t *fc_b, const double *coeff_b,
                        const int32_t *off_b, const int32_t *idx_b, const double *w_b);
 
+/* DILU schedule selection for the next mrfg_commit: MRFG_SCHEDULE_AUTO
+   (structured hex regions as line tiles, others by levels) or
+   MRFG_SCHEDULE_LEVELS (levels everywhere).  Both compute the same factor
+   and sweeps; the choice only changes the parallel execution order. */
+enum { MRFG_SCHEDULE_AUTO = 0, MRFG_SCHEDULE_LEVELS = 1 };
+int mrfg_set_schedule(mrfg_ctx *ctx, int schedule);
+
 /* Build the device layout of every system defined so far and upload it. */
 int mrfg_commit(mrfg_ctx *ctx);
 
@@ -169,8 +176,13 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w);
    events on the context's stream: which 0 = SpMV, 1 = DILU forward sweep,
    2 = DILU backward sweep, 3 = DILU factorisation. */
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms);
+/* Trace one DILU forward/backward pair (tracing facility; no counterpart
+   in the reference): out receives, per wave in dispatch order, [start ns,
+   end ns, slow-path waits, steps], forward sweep then backward sweep
+   (8*min(n_waves, max_waves) values).  n_waves receives the wave count. */
+int mrfg_trace_dilu(mrfg_ctx *ctx, long long *out, int max_waves, int *n_waves);
 /* info[0..5] = cells, internal faces, GGI pairs, rows incl. padding,
-   64-row slices, DILU levels. */
+   64-row slices, DILU waves. */
 int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info);
 /* Canonical algorithmic bytes at value size 8 (fp64) or 4 (fp32). */
 int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b_spmv, double *b_dilu);

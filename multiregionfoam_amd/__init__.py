@@ -77,6 +77,7 @@ def lib():
                                 C.c_int, I32P, F64P, I32P, I32P, F64P,
                                 C.c_int, I32P, F64P, I32P, I32P, F64P], C.c_int),
         "mrfg_commit": ([vp], C.c_int),
+        "mrfg_set_schedule": ([vp, C.c_int], C.c_int),
         "mrfg_solve": ([vp, C.POINTER(Controls), C.POINTER(Perf)], C.c_int),
         "mrfg_solve_dict": ([vp, C.c_char_p, C.c_char_p, C.POINTER(Perf)], C.c_int),
         "mrfg_parse_controls": ([C.c_char_p, C.c_char_p, C.POINTER(Controls), C.c_char_p, C.c_int], C.c_int),
@@ -98,6 +99,7 @@ def lib():
         "mrfg_precondition": ([vp, C.c_int, F64P, F64P], C.c_int),
         "mrfg_time_kernel": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_layout_info": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
+        "mrfg_trace_dilu": ([vp, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)], C.c_int),
         "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
     }
     for name, (args, res) in sig.items():
@@ -110,10 +112,10 @@ def lib():
 
 EXPORTED = (
     "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region "
-    "mrfg_set_interface mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
+    "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_time_kernel mrfg_layout_info mrfg_bytes").split()
+    "mrfg_precondition mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes").split()
 
 
 def _p(a, t):
@@ -309,7 +311,10 @@ class CoupledFvMatrix:
         self._sizes.append([int(R["nCells"]) for R in regions])
         return s
 
-    def commit(self):
+    def commit(self, schedule="auto"):
+        """Build the device layout; schedule "auto" (line tiles on structured
+        hex regions) or "levels" (level schedule everywhere)."""
+        self._check(lib().mrfg_set_schedule(self._h, {"auto": 0, "levels": 1}[schedule]))
         self._check(lib().mrfg_commit(self._h))
 
     @property
@@ -371,10 +376,20 @@ class CoupledFvMatrix:
         self._check(lib().mrfg_time_kernel(self._h, which, reps, C.byref(ms)))
         return ms.value
 
+    def trace_dilu(self, max_waves=100000):
+        """Per-wave [start_ns, end_ns, slow_waits, steps] of one forward and
+        one backward DILU sweep (arrays of shape (n_waves, 4))."""
+        out = np.zeros(8 * max_waves, dtype=np.int64)
+        n = C.c_int()
+        self._check(lib().mrfg_trace_dilu(self._h, out.ctypes.data_as(C.POINTER(C.c_longlong)), max_waves,
+                                          C.byref(n)))
+        m = min(n.value, max_waves)
+        return out[:4 * m].reshape(m, 4), out[4 * m:8 * m].reshape(m, 4)
+
     def layout_info(self, sys=0):
         info = (C.c_longlong * 6)()
         self._check(lib().mrfg_layout_info(self._h, sys, info))
-        keys = ("cells", "faces", "pairs", "rows", "slices", "levels")
+        keys = ("cells", "faces", "pairs", "rows", "slices", "waves")
         return dict(zip(keys, [int(v) for v in info]))
 
     def bytes_model(self, sys=0, value_size=8):

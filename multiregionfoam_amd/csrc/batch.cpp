@@ -8,7 +8,9 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <cstring>
 #include <stdexcept>
 
 #include "kernels.hpp"
@@ -104,10 +106,18 @@ Batch::~Batch()
 namespace {
 
 template <typename T>
-void appendShift(std::vector<T> &dst, const std::vector<T> &src, T shift, bool keepNeg = false)
+void appendShift(std::vector<T> &dst, const std::vector<T> &src, T shift)
 {
     dst.reserve(dst.size() + src.size());
-    for (T v : src) dst.push_back(keepNeg && v < 0 ? v : (T)(v + shift));
+    for (T v : src) dst.push_back((T)(v + shift));
+}
+
+int roundK(int k)
+{
+    if (k <= 3) return 3;
+    if (k <= 6) return 6;
+    if (k <= 12) return 12;
+    throw std::runtime_error("a cell has more than 12 lower or upper neighbours; not supported by the DILU kernels");
 }
 
 }  // namespace
@@ -115,44 +125,35 @@ void appendShift(std::vector<T> &dst, const std::vector<T> &src, T shift, bool k
 void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
 {
     HIP_OK(hipSetDevice(device_));
+    HIP_OK(hipStreamSynchronize(stream_));
     sys_ = systems;
     lay_.clear();
-    for (auto *s : systems) lay_.emplace_back(new SystemLayout(buildLayout(*s)));
+    for (auto *s : systems) lay_.emplace_back(new SystemLayout(buildLayout(*s, preferLevels_)));
     const int nS = (int)systems.size();
     sliceBegin_.assign(nS, 0);
-    levelBase_.assign(nS, 0);
+    K_ = 3;
+    for (auto &L : lay_) K_ = std::max(K_, roundK(L->K));
 
-    std::vector<int32_t> ellOff, ellCol, ifBase, ifRowFace, ifPairOff, ifPairCol, lOff, lCol, uOff, uCol;
+    std::vector<int32_t> ellOff, ellCol, ifBase, ifRowFace, ifPairOff, ifPairCol;
     std::vector<unsigned long long> ifMask;
-    std::vector<int32_t> cellOfPos, sliceSys, sliceLevel, levelCount;
-    std::vector<double> diag, source, sumA, psi0;
-    hEllVal_.clear(); hDiag_.clear(); hIfCoeff_.clear(); hIfPairW_.clear();
-    hLVal_.clear(); hLProd_.clear(); hUVal_.clear();
+    std::vector<int32_t> cellOfPos, fCode, bCode, fHalo, bHalo, waveSlice, waveSteps, waveSys;
+    std::vector<double> diag, source, sumA, psi0, fProd;
+    std::vector<std::array<int, 3>> waveKey;   // (key, system, local index)
+    hEllVal_.clear(); hDiag_.clear(); hIfCoeff_.clear(); hIfPairW_.clear(); hFVal_.clear(); hBVal_.clear();
     std::vector<BlockEntry> blocks;
     hInfo_.assign(nS, SysInfo{});
 
-    int slice = 0, ellBase = 0, lBase = 0, uBase = 0, ifRowBase = 0, ifFaceBase = 0, ifPairBase = 0, lvlBase = 0;
-    int maxLevels = 0;
+    int slice = 0, ellBase = 0, ifRowBase = 0, ifFaceBase = 0, ifPairBase = 0, hfBase = 0, hbBase = 0;
     for (int q = 0; q < nS; ++q) {
         const SystemLayout &L = *lay_[q];
         const int pos0 = slice * kSlice;
         sliceBegin_[q] = slice;
-        levelBase_[q] = lvlBase;
         for (int s = 0; s < L.nSlices; ++s) {
             ellOff.push_back(L.ellOff[s] + ellBase);
-            lOff.push_back(L.lOff[s] + lBase);
-            uOff.push_back(L.uOff[s] + uBase);
             ifBase.push_back(L.ifBase[s] + ifRowBase);
             ifMask.push_back(L.ifMask[s]);
-            sliceSys.push_back(q);
-        }
-        for (int lv = 0; lv < L.nLevels; ++lv) {
-            levelCount.push_back(L.levelSliceBegin[lv + 1] - L.levelSliceBegin[lv]);
-            for (int s = L.levelSliceBegin[lv]; s < L.levelSliceBegin[lv + 1]; ++s) sliceLevel.push_back(lv);
         }
         appendShift(ellCol, L.ellCol, pos0);
-        appendShift(lCol, L.lCol, pos0, true);
-        appendShift(uCol, L.uCol, pos0, true);
         appendShift(ifPairCol, L.ifPairCol, pos0);
         for (size_t i = 0; i + 1 < L.ifRowFace.size(); ++i) ifRowFace.push_back(L.ifRowFace[i] + ifFaceBase);
         for (size_t i = 0; i + 1 < L.ifPairOff.size(); ++i) ifPairOff.push_back(L.ifPairOff[i] + ifPairBase);
@@ -164,51 +165,161 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
         hEllVal_.insert(hEllVal_.end(), L.ellVal.begin(), L.ellVal.end());
         hIfCoeff_.insert(hIfCoeff_.end(), L.ifCoeff.begin(), L.ifCoeff.end());
         hIfPairW_.insert(hIfPairW_.end(), L.ifPairW.begin(), L.ifPairW.end());
-        hLVal_.insert(hLVal_.end(), L.lVal.begin(), L.lVal.end());
-        hLProd_.insert(hLProd_.end(), L.lProd.begin(), L.lProd.end());
-        hUVal_.insert(hUVal_.end(), L.uVal.begin(), L.uVal.end());
-
+        // DILU dependency slots, re-padded to the batch width K_; halo codes
+        // and halo slots are shifted to the system's halo range
+        for (int s = 0; s < L.nSlices; ++s)
+            for (int k = 0; k < K_; ++k)
+                for (int l = 0; l < kSlice; ++l) {
+                    if (k < L.K) {
+                        size_t e = ((size_t)s * L.K + k) * kSlice + l;
+                        int fc = L.fCode[e], bc = L.bCode[e];
+                        fCode.push_back(fc >= 0 ? fc + hfBase : fc);
+                        bCode.push_back(bc >= 0 ? bc + hbBase : bc);
+                        hFVal_.push_back(L.fVal[e]);
+                        hBVal_.push_back(L.bVal[e]);
+                        fProd.push_back(L.fProd[e]);
+                    } else {
+                        fCode.push_back(kNoDep);
+                        bCode.push_back(kNoDep);
+                        hFVal_.push_back(0.0);
+                        hBVal_.push_back(0.0);
+                        fProd.push_back(0.0);
+                    }
+                }
+        for (int p = 0; p < L.nPos; ++p) {
+            fHalo.push_back(L.fHalo[p] >= 0 ? L.fHalo[p] + hfBase : -1);
+            bHalo.push_back(L.bHalo[p] >= 0 ? L.bHalo[p] + hbBase : -1);
+        }
         SysInfo &I = hInfo_[q];
+        I.waveBegin = (int)waveSlice.size();
+        for (int w = 0; w < L.nWaves; ++w) {
+            waveKey.push_back({L.waveKey[w], q, (int)waveSlice.size()});
+            waveSlice.push_back(L.waveSlice[w] + slice);
+            waveSteps.push_back(L.waveSteps[w]);
+            waveSys.push_back(q);
+        }
+        I.nWaves = L.nWaves;
         I.blkBegin = (int)blocks.size();
         for (int s = 0; s < L.nSlices; s += kSlicesPerBlock)
             blocks.push_back(BlockEntry{q, slice + s, slice + std::min(L.nSlices, s + kSlicesPerBlock), 0});
         I.nBlk = (int)blocks.size() - I.blkBegin;
         I.sliceBegin = slice;
         I.nSlices = L.nSlices;
-        I.levelBase = lvlBase;
-        I.nLevels = L.nLevels;
         I.nCells = (int)L.nCells;
 
         slice += L.nSlices;
         ellBase += L.ellOff[L.nSlices];
-        lBase += L.lOff[L.nSlices];
-        uBase += L.uOff[L.nSlices];
         ifRowBase += (int)L.ifRowFace.size() - 1;
         ifFaceBase += (int)L.ifCoeff.size();
         ifPairBase += (int)L.ifPairW.size();
-        lvlBase += L.nLevels;
-        maxLevels = std::max(maxLevels, L.nLevels);
+        hfBase += L.nHaloF;
+        hbBase += L.nHaloB;
     }
     ellOff.push_back(ellBase);
-    lOff.push_back(lBase);
-    uOff.push_back(uBase);
     ifRowFace.push_back(ifFaceBase);
+    // pack the DILU records: values [c_0..c_{K-1}, rD | diag] in chunks of
+    // Pack<T>::N; ints [byte codes, out_0..out_{K-1}] in chunks of 4.  A
+    // cross-wave dependency is delivered to the consumer's inbox slot
+    // (slice, kk, lane); its producer's outs list those inbox addresses.
+    {
+        const int nSl = slice;
+        const int G = packChunks(K_ + 1, 2), G32 = packChunks(K_ + 1, 4);
+        const int NC = packChunks(K_, 4), H = packChunks(codeInts(K_), 4);
+        int rawF = 0, rawB = 0;
+        for (size_t e = 0; e < fCode.size(); ++e) {
+            if (fCode[e] >= 0) rawF |= 1 << (int)((e / kSlice) % K_);
+            if (bCode[e] >= 0) rawB |= 1 << (int)((e / kSlice) % K_);
+        }
+        haloMaskF_ = rawF;
+        haloMaskB_ = rawB;
+        const int mF = effectiveHaloMask(K_, rawF), mB = effectiveHaloMask(K_, rawB);
+        const int KIF = __builtin_popcount(mF), KIB = __builtin_popcount(mB);
+        auto kkOf = [](int m, int k) { return __builtin_popcount(m & ((1 << k) - 1)); };
+        hFPack_.assign((size_t)nSl * G * 64 * 2, 0.0);
+        hBPack_.assign((size_t)nSl * G * 64 * 2, 0.0);
+        hFPack32_.assign((size_t)nSl * G32 * 64 * 4, 0.0f);
+        hBPack32_.assign((size_t)nSl * G32 * 64 * 4, 0.0f);
+        std::vector<double> prod((size_t)nSl * G * 64 * 2, 0.0);
+        std::vector<int32_t> fc((size_t)nSl * H * 64 * 4, -1), bc((size_t)nSl * H * 64 * 4, -1);
+        // consumers of every published halo value (slot id -> inbox addresses)
+        std::vector<std::vector<int32_t>> fOuts(hfBase), bOuts(hbBase);
+        auto intAt = [&](int s, int j, int l) { return (((size_t)s * H + j / 4) * 64 + l) * 4 + j % 4; };
+        for (int s = 0; s < nSl; ++s)
+            for (int l = 0; l < kSlice; ++l) {
+                const int p = s * kSlice + l;
+                uint32_t fw[4] = {0, 0, 0, 0}, bw[4] = {0, 0, 0, 0};
+                for (int k = 0; k < K_; ++k) {
+                    size_t e = ((size_t)s * K_ + k) * kSlice + l;
+                    auto byteOf = [&](int code, int mask, int KI, std::vector<std::vector<int32_t>> &outs) {
+                        if (code == kNoDep) return kCodeNone;
+                        if (code < 0) return -code - 1;            // LDS ring slot
+                        if (!(mask & (1 << k))) throw std::runtime_error("DILU inbox mask misses a dependency slot");
+                        outs[code].push_back(((s * KI + kkOf(mask, k)) << 6) + l);
+                        return kCodeInbox;
+                    };
+                    uint32_t fb = (uint32_t)byteOf(fCode[e], mF, KIF, fOuts);
+                    uint32_t bb = (uint32_t)byteOf(bCode[e], mB, KIB, bOuts);
+                    fw[k / 4] |= fb << (8 * (k % 4));
+                    bw[k / 4] |= bb << (8 * (k % 4));
+                    size_t e64 = (((size_t)s * G + k / 2) * 64 + l) * 2 + k % 2;
+                    size_t e32 = (((size_t)s * G32 + k / 4) * 64 + l) * 4 + k % 4;
+                    hFPack_[e64] = hFVal_[e];
+                    hBPack_[e64] = hBVal_[e];
+                    hFPack32_[e32] = (float)hFVal_[e];
+                    hBPack32_[e32] = (float)hBVal_[e];
+                    prod[e64] = fProd[e];
+                }
+                // padding slots of the last code int are "none"
+                for (int k = K_; k < NC * 4; ++k) {
+                    fw[k / 4] |= (uint32_t)kCodeNone << (8 * (k % 4));
+                    bw[k / 4] |= (uint32_t)kCodeNone << (8 * (k % 4));
+                }
+                for (int j = 0; j < NC; ++j) {
+                    fc[intAt(s, j, l)] = (int32_t)fw[j];
+                    bc[intAt(s, j, l)] = (int32_t)bw[j];
+                }
+                prod[(((size_t)s * G + K_ / 2) * 64 + l) * 2 + K_ % 2] = diag[p];
+            }
+        for (int s = 0; s < nSl; ++s)
+            for (int l = 0; l < kSlice; ++l) {
+                const int p = s * kSlice + l;
+                auto putOuts = [&](int h, std::vector<std::vector<int32_t>> &outs, std::vector<int32_t> &dst) {
+                    if (h < 0) return;
+                    if ((int)outs[h].size() > K_) throw std::runtime_error("DILU value has more consumers than slots");
+                    for (size_t j = 0; j < outs[h].size(); ++j) dst[intAt(s, NC + (int)j, l)] = outs[h][j];
+                };
+                putOuts(fHalo[p], fOuts, fc);
+                putOuts(bHalo[p], bOuts, bc);
+            }
+        nHaloF_ = nSl * KIF * 64;
+        nHaloB_ = nSl * KIB * 64;
+        auto asVec = [](const std::vector<int32_t> &a) {
+            std::vector<int4> v(a.size() / 4);
+            std::memcpy(v.data(), a.data(), a.size() * sizeof(int32_t));
+            return v;
+        };
+        auto asD2 = [](const std::vector<double> &a) {
+            std::vector<double2> v(a.size() / 2);
+            std::memcpy(v.data(), a.data(), a.size() * sizeof(double));
+            return v;
+        };
+        fCodes_.upload(asVec(fc), stream_);
+        bCodes_.upload(asVec(bc), stream_);
+        prodPack_.upload(asD2(prod), stream_);
+        m64_.fPack.upload(asD2(hFPack_), stream_);
+        m64_.bPack.upload(asD2(hBPack_), stream_);
+    }
     ifPairOff.push_back(ifPairBase);
     nSlices_ = slice;
     nPos_ = slice * kSlice;
     nBlk_ = (int)blocks.size();
-    nLevelsTotal_ = lvlBase;
+    nWaves_ = (int)waveSlice.size();
 
-    // DILU launch order: level-major, systems interleaved
-    std::vector<int32_t> order;
-    order.reserve(nSlices_);
-    for (int lv = 0; lv < maxLevels; ++lv)
-        for (int q = 0; q < nS; ++q) {
-            const SystemLayout &L = *lay_[q];
-            if (lv >= L.nLevels) continue;
-            for (int s = L.levelSliceBegin[lv]; s < L.levelSliceBegin[lv + 1]; ++s) order.push_back(sliceBegin_[q] + s);
-        }
-    nOrder_ = (int)order.size();
+    // forward dispatch order: by key (tile anti-diagonal / level), systems
+    // interleaved; every wave's producers precede it
+    std::sort(waveKey.begin(), waveKey.end());
+    std::vector<int32_t> order(nWaves_);
+    for (int i = 0; i < nWaves_; ++i) order[i] = waveKey[i][2];
 
     hipStream_t st = stream_;
     ellOff_.upload(ellOff, st);
@@ -218,23 +329,19 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     ifRowFace_.upload(ifRowFace, st);
     ifPairOff_.upload(ifPairOff, st);
     ifPairCol_.upload(ifPairCol, st);
-    lOff_.upload(lOff, st);
-    lCol_.upload(lCol, st);
-    uOff_.upload(uOff, st);
-    uCol_.upload(uCol, st);
     cellOfPos_.upload(cellOfPos, st);
     blocks_.upload(blocks, st);
     info_.upload(hInfo_, st);
-    sliceSys_.upload(sliceSys, st);
-    sliceLevel_.upload(sliceLevel, st);
-    levelCount_.upload(levelCount, st);
     order_.upload(order, st);
-    cntA_.alloc(std::max(1, nLevelsTotal_));
-    cntB_.alloc(std::max(1, nLevelsTotal_));
-    cntC_.alloc(std::max(1, nLevelsTotal_));
-    cntA_.zero(st);
-    cntB_.zero(st);
-    cntC_.zero(st);
+    waveSlice_.upload(waveSlice, st);
+    waveSteps_.upload(waveSteps, st);
+    waveSys_.upload(waveSys, st);
+    haloF64_.alloc(std::max(1, nHaloF_));
+    haloB64_.alloc(std::max(1, nHaloB_));
+    haloC64_.alloc(std::max(1, nHaloF_));
+    HIP_OK(fillHalo64(haloF64_.p, (int)haloF64_.n, st));
+    HIP_OK(fillHalo64(haloB64_.p, (int)haloB64_.n, st));
+    HIP_OK(fillHalo64(haloC64_.p, (int)haloC64_.n, st));
     err_.alloc(1);
     err_.zero(st);
 
@@ -243,15 +350,10 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     hDiag_ = diag;
     m64_.ifCoeff.upload(hIfCoeff_, st);
     m64_.ifPairW.upload(hIfPairW_, st);
-    m64_.lVal.upload(hLVal_, st);
-    m64_.lProd.upload(hLProd_, st);
-    m64_.uVal.upload(hUVal_, st);
     sumA_.upload(sumA, st);
     b64_.upload(source, st);
     x64_.upload(psi0, st);
     x0_.upload(psi0, st);
-    d64_.alloc(nPos_);
-    d64_.zero(st);
     for (auto &v : v64_) {
         v.alloc(nPos_);
         v.zero(st);
@@ -259,11 +361,11 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     st64_.alloc(nS);
     st32_.alloc(nS);
     partials_.alloc((size_t)std::max(1, nBlk_) * 4);
-    tickets_.alloc(std::max(1, nS));
-    tickets_.zero(st);
     factorised_ = false;
     haveFloat_ = false;
     for (auto &v : v32_) v.release();
+    haloF32_.release();
+    haloB32_.release();
     HIP_OK(hipStreamSynchronize(st));
 }
 
@@ -280,13 +382,21 @@ void Batch::ensureFloat()
     m32_.diag.upload(toF(hDiag_), st);
     m32_.ifCoeff.upload(toF(hIfCoeff_), st);
     m32_.ifPairW.upload(toF(hIfPairW_), st);
-    m32_.lVal.upload(toF(hLVal_), st);
-    m32_.lProd.upload(toF(hLProd_), st);
-    m32_.uVal.upload(toF(hUVal_), st);
+    {
+        std::vector<float4> f(hFPack32_.size() / 4), b(hBPack32_.size() / 4);
+        std::memcpy(f.data(), hFPack32_.data(), hFPack32_.size() * sizeof(float));
+        std::memcpy(b.data(), hBPack32_.data(), hBPack32_.size() * sizeof(float));
+        m32_.fPack.upload(f, st);
+        m32_.bPack.upload(b, st);
+    }
     for (auto &v : v32_) {
         v.alloc(nPos_);
         v.zero(st);
     }
+    haloF32_.alloc(std::max(1, nHaloF_));
+    haloB32_.alloc(std::max(1, nHaloB_));
+    HIP_OK(fillHalo32(haloF32_.p, (int)haloF32_.n, st));
+    HIP_OK(fillHalo32(haloB32_.p, (int)haloB32_.n, st));
     haveFloat_ = true;
     factorised_ = false;   // rD32 is derived from the FP64 factor
     HIP_OK(hipStreamSynchronize(st));
@@ -307,25 +417,46 @@ MatView<T> Batch::view(const DevMatrix<T> &m) const
     A.ifPairOff = ifPairOff_.p;
     A.ifPairCol = ifPairCol_.p;
     A.ifPairW = m.ifPairW.p;
-    A.lOff = lOff_.p;
-    A.lCol = lCol_.p;
-    A.lVal = m.lVal.p;
-    A.lProd = m.lProd.p;
-    A.uOff = uOff_.p;
-    A.uCol = uCol_.p;
-    A.uVal = m.uVal.p;
     return A;
 }
 
+template <typename T>
+DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
+{
+    DiluView<T> D;
+    D.order = order_.p;
+    D.waveSlice = waveSlice_.p;
+    D.waveSteps = waveSteps_.p;
+    D.waveSys = waveSys_.p;
+    D.nWaves = nWaves_;
+    D.K = K_;
+    D.fCodes = fCodes_.p;
+    D.bCodes = bCodes_.p;
+    D.fPack = m.fPack.p;
+    D.bPack = m.bPack.p;
+    D.prodPack = prodPack_.p;
+    D.rDout = v64_[11].p;
+    D.cellOfPos = cellOfPos_.p;
+    D.haloMaskF = haloMaskF_;
+    D.haloMaskB = haloMaskB_;
+    D.haloF = hF;
+    D.haloB = hB;
+    D.haloC = haloC64_.p;
+    D.nHaloF = nHaloF_;
+    D.nHaloB = nHaloB_;
+    D.err = err_.p;
+    return D;
+}
+
 template <>
-void Batch::bindKrylov<double>(Krylov<double> &K, bool)
+void Batch::bindKrylov<double>(Krylov<double> &K)
 {
     K.blocks = blocks_.p;
     K.nBlk = nBlk_;
+    K.nSys = (int)sys_.size();
     K.info = info_.p;
     K.state = st64_.p;
     K.partials = partials_.p;
-    K.tickets = tickets_.p;
     K.sumA = sumA_.p;
     K.x = x64_.p;
     K.b = b64_.p;
@@ -335,14 +466,14 @@ void Batch::bindKrylov<double>(Krylov<double> &K, bool)
 }
 
 template <>
-void Batch::bindKrylov<float>(Krylov<float> &K, bool)
+void Batch::bindKrylov<float>(Krylov<float> &K)
 {
     K.blocks = blocks_.p;
     K.nBlk = nBlk_;
+    K.nSys = (int)sys_.size();
     K.info = info_.p;
     K.state = st32_.p;
     K.partials = partials_.p;
-    K.tickets = tickets_.p;
     K.sumA = sumA_.p;
     K.x = v32_[0].p; K.b = v32_[1].p;
     K.ax = v32_[2].p; K.r = v32_[3].p; K.rw = v32_[4].p; K.pv = v32_[5].p; K.v = v32_[6].p;
@@ -350,37 +481,14 @@ void Batch::bindKrylov<float>(Krylov<float> &K, bool)
     K.pA = v32_[12].p; K.rD = v32_[13].p;
 }
 
-LevelView makeLevelView(const int32_t *sliceSys, const int32_t *sliceLevel, const int32_t *levelCount,
-                        const int32_t *order, int *cntA, int *cntB, int *err, int nLevelsTotal, int nOrder)
-{
-    LevelView LV;
-    LV.err = err;
-    LV.sliceSys = sliceSys;
-    LV.sliceLevel = sliceLevel;
-    LV.levelCount = levelCount;
-    LV.order = order;
-    LV.cntA = cntA;
-    LV.cntB = cntB;
-    LV.nLevelsTotal = nLevelsTotal;
-    LV.nOrder = nOrder;
-    return LV;
-}
-
 void Batch::factorise()
 {
     if (factorised_) return;
-    LevelView LV = makeLevelView(sliceSys_.p, sliceLevel_.p, levelCount_.p, order_.p, cntA_.p, cntB_.p, err_.p,
-                                 nLevelsTotal_, nOrder_);
-    Krylov<double> K;
-    bindKrylov(K, false);
-    MatView<double> A = view(m64_);
-    HIP_OK(launchFactor<double>(A, LV, K, A, d64_.p, cellOfPos_.p, cntC_.p, stream_));
+    DiluView<double> D = diluView(m64_, haloF64_.p, haloB64_.p);
+    HIP_OK(mrfg::diluFactor(D, stream_));
     if (haveFloat_) {
-        Krylov<float> K32;
-        bindKrylov(K32, true);
-        MatView<float> A32 = view(m32_);
-        // FP32 rD from the FP64 factor: re-run the inversion into rD32
-        HIP_OK(launchFactor<float>(A32, LV, K32, A, d64_.p, cellOfPos_.p, cntC_.p, stream_));
+        DiluView<float> D32 = diluView(m32_, haloF32_.p, haloB32_.p);
+        HIP_OK(diluConvertRD(D32, v64_[11].p, nSlices_, stream_));
     }
     factorised_ = true;
 }
@@ -392,7 +500,8 @@ void Batch::checkDeviceError()
     HIP_OK(hipStreamSynchronize(stream_));
     if (e) {
         err_.zero(stream_);
-        throw std::runtime_error("HIP error: DILU level wait expired (device error flag set)");
+        throw std::runtime_error("HIP error: DILU dependency wait expired (device error flag " + std::to_string(e) +
+                                 ")");
     }
 }
 
@@ -410,6 +519,15 @@ void Batch::initState(const SolverControls &c)
         h[q] = s;
     }
     HIP_OK(hipMemcpyAsync(st64_.p, h.data(), h.size() * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Batch::markAllRunning()
+{
+    std::vector<SysState> hs(sys_.size());
+    for (auto &x : hs) x.status = kRunning;
+    HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), hs.size() * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
 }
 
 void Batch::readState(std::vector<SysState> &out)
@@ -428,10 +546,9 @@ static bool anyRunning(const std::vector<SysState> &s)
 
 void Batch::runFp64(const SolverControls &c)
 {
-    LevelView LV = makeLevelView(sliceSys_.p, sliceLevel_.p, levelCount_.p, order_.p, cntA_.p, cntB_.p, err_.p,
-                                 nLevelsTotal_, nOrder_);
     Krylov<double> K;
-    bindKrylov(K, false);
+    bindKrylov(K);
+    DiluView<double> D = diluView(m64_, haloF64_.p, haloB64_.p);
     MatView<double> A = view(m64_);
     const bool pcg = c.solver == SolverControls::PCG;
     HIP_OK(launchSetup<double>(A, K, pcg, stream_));
@@ -441,8 +558,8 @@ void Batch::runFp64(const SolverControls &c)
     while (anyRunning(hs) && done <= c.maxIter) {
         int chunk = std::min(c.checkInterval, c.maxIter - done + 1);
         for (int it = 0; it < chunk; ++it) {
-            if (pcg) HIP_OK(launchPcgIteration<double>(A, LV, K, stream_));
-            else HIP_OK(launchBicgIteration<double>(A, LV, K, stream_));
+            if (pcg) HIP_OK(launchPcgIteration<double>(A, D, K, stream_));
+            else HIP_OK(launchBicgIteration<double>(A, D, K, stream_));
         }
         done += chunk;
         readState(hs);
@@ -451,14 +568,11 @@ void Batch::runFp64(const SolverControls &c)
 
 void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters)
 {
-    ensureFloat();
-    factorise();
-    LevelView LV = makeLevelView(sliceSys_.p, sliceLevel_.p, levelCount_.p, order_.p, cntA_.p, cntB_.p, err_.p,
-                                 nLevelsTotal_, nOrder_);
     Krylov<double> K64;
-    bindKrylov(K64, false);
+    bindKrylov(K64);
     Krylov<float> K32;
-    bindKrylov(K32, true);
+    bindKrylov(K32);
+    DiluView<float> D32 = diluView(m32_, haloF32_.p, haloB32_.p);
     MatView<double> A64 = view(m64_);
     MatView<float> A32 = view(m32_);
     const bool pcg = c.solver == SolverControls::PCG;
@@ -479,8 +593,8 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters)
             if (!anyRunning(hi) || done > c.innerMaxIter) break;
             int chunk = std::min(c.checkInterval, c.innerMaxIter - done + 1);
             for (int it = 0; it < chunk; ++it) {
-                if (pcg) HIP_OK(launchPcgIteration<float>(A32, LV, K32, stream_));
-                else HIP_OK(launchBicgIteration<float>(A32, LV, K32, stream_));
+                if (pcg) HIP_OK(launchPcgIteration<float>(A32, D32, K32, stream_));
+                else HIP_OK(launchBicgIteration<float>(A32, D32, K32, stream_));
             }
             done += chunk;
         }
@@ -542,6 +656,7 @@ void Batch::setPsi(int sys, int region, const double *psi)
     const CoupledSystem &S = *sys_.at(sys);
     int n = S.regions.at(region).nCells, o = L.regionOffset[region];
     std::vector<double> x(L.nPos);
+    HIP_OK(hipStreamSynchronize(stream_));
     HIP_OK(hipMemcpy(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double), hipMemcpyDeviceToHost));
     for (int c = 0; c < n; ++c) x[L.posOfCell[o + c]] = psi[c];
     HIP_OK(hipMemcpy(x64_.p + posBegin(sys), x.data(), L.nPos * sizeof(double), hipMemcpyHostToDevice));
@@ -553,6 +668,7 @@ void Batch::getPsi(int sys, int region, double *psi)
     const CoupledSystem &S = *sys_.at(sys);
     int n = S.regions.at(region).nCells, o = L.regionOffset[region];
     std::vector<double> x(L.nPos);
+    HIP_OK(hipStreamSynchronize(stream_));
     HIP_OK(hipMemcpy(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double), hipMemcpyDeviceToHost));
     for (int c = 0; c < n; ++c) psi[c] = x[L.posOfCell[o + c]];
 }
@@ -573,6 +689,7 @@ void Batch::spmv(const double *xHost, double *yHost, int sys)
 
 void Batch::diluFactor(int sys, double *rDHost)
 {
+    factorised_ = false;
     factorise();
     const SystemLayout &L = *lay_.at(sys);
     std::vector<double> r(nPos_);
@@ -588,19 +705,15 @@ void Batch::precondition(const double *rHost, double *wHost, int sys)
     const SystemLayout &L = *lay_.at(sys);
     std::vector<double> r(nPos_, 0.0), w(nPos_);
     for (long long c = 0; c < L.nCells; ++c) r[posBegin(sys) + L.posOfCell[c]] = rHost[c];
-    // every system must be marked running for the sweeps to act
-    std::vector<SysState> hs(sys_.size());
-    for (auto &s : hs) s.status = kRunning;
-    HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), hs.size() * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+    markAllRunning();
     DevBuf<double> dr, dw;
     dr.upload(r, stream_);
     dw.alloc(nPos_);
     dw.zero(stream_);
-    LevelView LV = makeLevelView(sliceSys_.p, sliceLevel_.p, levelCount_.p, order_.p, cntA_.p, cntB_.p, err_.p,
-                                 nLevelsTotal_, nOrder_);
     Krylov<double> K;
-    bindKrylov(K, false);
-    HIP_OK(launchPrecondition<double>(view(m64_), LV, K, dr.p, dw.p, stream_));
+    bindKrylov(K);
+    DiluView<double> D = diluView(m64_, haloF64_.p, haloB64_.p);
+    HIP_OK(launchPrecondition<double>(D, K, dr.p, dw.p, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
     HIP_OK(hipMemcpy(w.data(), dw.p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
@@ -611,32 +724,33 @@ double Batch::timeKernel(int which, int reps)
 {
     HIP_OK(hipSetDevice(device_));
     factorise();
-    std::vector<SysState> hs(sys_.size());
-    for (auto &x : hs) x.status = kRunning;
-    HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), hs.size() * sizeof(SysState), hipMemcpyHostToDevice, stream_));
-    LevelView LV = makeLevelView(sliceSys_.p, sliceLevel_.p, levelCount_.p, order_.p, cntA_.p, cntB_.p, err_.p,
-                                 nLevelsTotal_, nOrder_);
+    markAllRunning();
     Krylov<double> K;
-    bindKrylov(K, false);
+    bindKrylov(K);
+    DiluView<double> D = diluView(m64_, haloF64_.p, haloB64_.p);
     MatView<double> A = view(m64_);
     const int nE = 2 * reps + 2;
     std::vector<hipEvent_t> ev(nE);
     for (auto &e : ev) HIP_OK(hipEventCreate(&e));
-    // DILU sweeps are timed as forward/backward pairs (the counters of one
-    // sweep are reset by the other), each sweep between its own events.
+    // sweeps run as forward/backward pairs (each resets the other's halo);
+    // only the requested one sits between its events
     auto one = [&](int i) {
+        hipEvent_t a = ev[2 * i], b = ev[2 * i + 1];
         if (which == 0) {
-            HIP_OK(hipEventRecord(ev[2 * i], stream_));
+            HIP_OK(hipEventRecord(a, stream_));
             HIP_OK(launchSpmvPlain<double>(A, blocks_.p, nBlk_, x64_.p, v64_[0].p, stream_));
-            HIP_OK(hipEventRecord(ev[2 * i + 1], stream_));
+            HIP_OK(hipEventRecord(b, stream_));
         } else if (which == 3) {
-            HIP_OK(hipEventRecord(ev[2 * i], stream_));
-            HIP_OK(launchFactor<double>(A, LV, K, A, d64_.p, cellOfPos_.p, cntC_.p, stream_));
-            HIP_OK(hipEventRecord(ev[2 * i + 1], stream_));
+            HIP_OK(hipEventRecord(a, stream_));
+            HIP_OK(mrfg::diluFactor(D, stream_));
+            HIP_OK(hipEventRecord(b, stream_));
         } else {
-            HIP_OK(launchDiluSweeps<double>(A, LV, K, b64_.p, v64_[9].p, stream_, which == 1 ? ev[2 * i] : nullptr,
-                                             which == 1 ? ev[2 * i + 1] : nullptr,
-                                             which == 2 ? ev[2 * i] : nullptr, which == 2 ? ev[2 * i + 1] : nullptr));
+            if (which == 1) HIP_OK(hipEventRecord(a, stream_));
+            HIP_OK(diluForward<double>(D, K.state, b64_.p, v64_[9].p, stream_));
+            if (which == 1) HIP_OK(hipEventRecord(b, stream_));
+            if (which == 2) HIP_OK(hipEventRecord(a, stream_));
+            HIP_OK(diluBackward<double>(D, K.state, v64_[9].p, stream_));
+            if (which == 2) HIP_OK(hipEventRecord(b, stream_));
         }
     };
     one(reps);   // warm-up into the spare event pair
@@ -651,6 +765,43 @@ double Batch::timeKernel(int which, int reps)
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return tot / reps;
+}
+
+int Batch::traceDilu(long long *out, int maxWaves)
+{
+    HIP_OK(hipSetDevice(device_));
+    factorise();
+    markAllRunning();
+    Krylov<double> K;
+    bindKrylov(K);
+    DiluView<double> D = diluView(m64_, haloF64_.p, haloB64_.p);
+    DevBuf<unsigned long long> tr;
+    tr.alloc((size_t)8 * nWaves_);
+    tr.zero(stream_);
+    int freq = 0;
+    HIP_OK(hipDeviceGetAttribute(&freq, hipDeviceAttributeWallClockRate, device_));   // kHz
+    D.trace = tr.p;
+    HIP_OK(diluForward<double>(D, K.state, b64_.p, v64_[9].p, stream_));
+    D.trace = tr.p + 4 * (size_t)nWaves_;
+    HIP_OK(diluBackward<double>(D, K.state, v64_[9].p, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    checkDeviceError();
+    std::vector<unsigned long long> h((size_t)8 * nWaves_);
+    HIP_OK(hipMemcpy(h.data(), tr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    const double nsPerTick = freq > 0 ? 1e6 / freq : 10.0;
+    for (int sw = 0; sw < 2; ++sw) {
+        unsigned long long t0 = ~0ull;
+        for (int w = 0; w < nWaves_; ++w) t0 = std::min(t0, h[(size_t)4 * (sw * nWaves_ + w)]);
+        for (int w = 0; w < nWaves_ && w < maxWaves; ++w) {
+            const unsigned long long *e = &h[(size_t)4 * (sw * nWaves_ + w)];
+            long long *o = out + (size_t)4 * (sw * std::min(nWaves_, maxWaves) + w);
+            o[0] = (long long)((e[0] - t0) * nsPerTick);
+            o[1] = (long long)((e[1] - t0) * nsPerTick);
+            o[2] = (long long)e[2];
+            o[3] = (long long)e[3];
+        }
+    }
+    return nWaves_;
 }
 
 double Batch::bytesSpmv(int sys, int sv) const

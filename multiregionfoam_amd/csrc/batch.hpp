@@ -68,7 +68,8 @@ struct DevBuf {
 
 template <typename T>
 struct DevMatrix {
-    DevBuf<T> ellVal, diag, ifCoeff, ifPairW, lVal, lProd, uVal;
+    DevBuf<T> ellVal, diag, ifCoeff, ifPairW;
+    DevBuf<typename Pack<T>::V> fPack, bPack;   // DILU records
 };
 
 class Batch {
@@ -100,6 +101,11 @@ public:
     // library's stream.  which: 0 SpMV, 1 DILU forward sweep, 2 DILU
     // backward sweep, 3 DILU factor.
     double timeKernel(int which, int reps);
+    // Run one forward/backward pair with per-wave tracing; out receives, per
+    // wave in dispatch order, [start, end, slow-path hits, steps] with times
+    // in ns relative to the earliest start (4*nWaves values per sweep,
+    // forward sweep first).  Returns the number of waves.
+    int traceDilu(long long *out, int maxWaves);
 
     const SystemLayout &layout(int sys) const { return *lay_[sys]; }
     hipStream_t stream() const { return stream_; }
@@ -109,6 +115,10 @@ public:
     double bytesSpmv(int sys, int sv) const;
     double bytesDilu(int sys, int sv) const;
 
+    // Force the level schedule for every region (general-mesh path); applies
+    // at the next setSystems.
+    void setPreferLevels(bool v) { preferLevels_ = v; }
+
 private:
     void ensureFloat();
     void factorise();
@@ -116,39 +126,49 @@ private:
     void readState(std::vector<SysState> &out);
     void runFp64(const SolverControls &c);
     void runMixed(const SolverControls &c, std::vector<int> &innerIters);
+    void checkDeviceError();
+    void markAllRunning();
     template <typename T>
     MatView<T> view(const DevMatrix<T> &m) const;
     template <typename T>
-    void bindKrylov(Krylov<T> &K, bool fp32);
+    DiluView<T> diluView(DevMatrix<T> &m, void *hF, void *hB);
+    template <typename T>
+    void bindKrylov(Krylov<T> &K);
     int posBegin(int sys) const { return sliceBegin_[sys] * kSlice; }
 
     int device_;
     hipStream_t stream_ = nullptr;
+    bool preferLevels_ = false;
     std::vector<std::unique_ptr<SystemLayout>> lay_;
     std::vector<const CoupledSystem *> sys_;
-    std::vector<int> sliceBegin_, levelBase_;
-    int nSlices_ = 0, nPos_ = 0, nBlk_ = 0, nLevelsTotal_ = 0, nOrder_ = 0;
+    std::vector<int> sliceBegin_;
+    int nSlices_ = 0, nPos_ = 0, nBlk_ = 0, nWaves_ = 0, K_ = 3, nHaloF_ = 0, nHaloB_ = 0;
     bool factorised_ = false, haveFloat_ = false;
 
     // structure (shared by both precisions)
-    DevBuf<int32_t> ellOff_, ellCol_, ifBase_, ifRowFace_, ifPairOff_, ifPairCol_, lOff_, lCol_, uOff_, uCol_;
+    DevBuf<int32_t> ellOff_, ellCol_, ifBase_, ifRowFace_, ifPairOff_, ifPairCol_;
     DevBuf<unsigned long long> ifMask_;
     DevBuf<int32_t> cellOfPos_;
     DevBuf<BlockEntry> blocks_;
     DevBuf<SysInfo> info_;
-    DevBuf<int32_t> sliceSys_, sliceLevel_, levelCount_, order_;
-    DevBuf<int> cntA_, cntB_, cntC_, err_;
-    void checkDeviceError();
+    // DILU schedule
+    DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_;
+    DevBuf<int4> fCodes_, bCodes_;
+    DevBuf<double2> prodPack_;
+    int haloMaskF_ = 0, haloMaskB_ = 0;
+    DevBuf<unsigned long long> haloF64_, haloB64_, haloC64_;
+    DevBuf<unsigned int> haloF32_, haloB32_;
+    DevBuf<int> err_;
     // host copies of the FP64 values, kept to derive the FP32 copies
-    std::vector<double> hEllVal_, hDiag_, hIfCoeff_, hIfPairW_, hLVal_, hLProd_, hUVal_;
+    std::vector<double> hEllVal_, hDiag_, hIfCoeff_, hIfPairW_, hFVal_, hBVal_, hFPack_, hBPack_;
+    std::vector<float> hFPack32_, hBPack32_;
     DevMatrix<double> m64_;
     DevMatrix<float> m32_;
-    DevBuf<double> sumA_, b64_, x64_, x0_, d64_;
+    DevBuf<double> sumA_, b64_, x64_, x0_;
     DevBuf<double> v64_[12];
     DevBuf<float> v32_[14];
     DevBuf<SysState> st64_, st32_;
     DevBuf<double> partials_;
-    DevBuf<int> tickets_;
     std::vector<SysInfo> hInfo_;
 };
 

@@ -241,6 +241,16 @@ int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int regi
     });
 }
 
+int mrfg_set_schedule(mrfg_ctx *ctx, int schedule)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        if (schedule != MRFG_SCHEDULE_AUTO && schedule != MRFG_SCHEDULE_LEVELS) throw ArgError("unknown schedule");
+        ctx->batch->setPreferLevels(schedule == MRFG_SCHEDULE_LEVELS);
+        ctx->committed = false;
+    });
+}
+
 int mrfg_commit(mrfg_ctx *ctx)
 {
     if (!ctx) return MRFG_ERR_ARG;
@@ -446,6 +456,15 @@ int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms)
     return guard(ctx, [&] { *ms = committedBatch(ctx).timeKernel(which, reps); });
 }
 
+int mrfg_trace_dilu(mrfg_ctx *ctx, long long *out, int max_waves, int *n_waves)
+{
+    if (!ctx || !out || max_waves < 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        int n = committedBatch(ctx).traceDilu(out, max_waves);
+        if (n_waves) *n_waves = n;
+    });
+}
+
 int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info)
 {
     if (!ctx || !info) return MRFG_ERR_ARG;
@@ -457,7 +476,7 @@ int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info)
         info[2] = L.nPairs;
         info[3] = L.nPos;
         info[4] = L.nSlices;
-        info[5] = L.nLevels;
+        info[5] = L.nWaves;
     });
 }
 

@@ -25,11 +25,11 @@ struct BlockEntry {
     int sys, slice0, slice1, pad;
 };
 
-// Per-system metadata for reductions and level synchronisation.
+// Per-system metadata for the reductions.
 struct SysInfo {
     int blkBegin, nBlk;        // range in the block table
     int sliceBegin, nSlices;
-    int levelBase, nLevels;    // range in the level arrays
+    int waveBegin, nWaves;     // range of the system's DILU waves
     int nCells, pad;
 };
 
@@ -46,25 +46,6 @@ struct MatView {
     const int32_t *ifPairOff;
     const int32_t *ifPairCol;
     const T *ifPairW;
-    const int32_t *lOff;
-    const int32_t *lCol;
-    const T *lVal;
-    const T *lProd;
-    const int32_t *uOff;
-    const int32_t *uCol;
-    const T *uVal;
-};
-
-struct LevelView {
-    const int32_t *sliceSys;      // per slice
-    const int32_t *sliceLevel;    // per slice, level index within its system
-    const int32_t *levelCount;    // per global level: slices in that level
-    const int32_t *order;         // DILU slice order (level-major, systems interleaved)
-    int *cntA;                    // per global level counters (two sets)
-    int *cntB;
-    int *err;                     // set to 1 if a bounded level wait expired
-    int nLevelsTotal;
-    int nOrder;
 };
 
 }  // namespace mrfg

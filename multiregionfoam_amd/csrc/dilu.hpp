@@ -1,0 +1,87 @@
+// DILU (coupled Cholesky preconditioner) executed by a wave schedule
+// (dilu.hip).  See layout.hpp for how the schedule and dependency codes are
+// built.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "layout.hpp"
+
+namespace mrfg {
+
+enum { kDiluFwd = 0, kDiluBwd = 1, kDiluFactor = 2 };
+
+// Halo slots hold the raw bits of a value; a slot equal to the sentinel has
+// not been published yet in the current sweep (signalling-NaN payloads that
+// arithmetic never produces).
+constexpr unsigned long long kHaloSentinel64 = 0xFFF4A5A5A5A5A5A5ull;
+constexpr unsigned int kHaloSentinel32 = 0xFFA5A5A5u;
+
+// Per-row DILU records, packed in 16-byte chunks so that a step of a wave
+// costs few vector-memory instructions.  Chunk g of lane l of slice s sits
+// at (s*G + g)*64 + l.
+//   values  [c_0..c_{K-1}, rD]: forward = lower coefficients, backward =
+//           upper coefficients, factor = upper*lower products with the
+//           diagonal in place of rD;
+//   ints    [codes, out_0..out_{K-1}]: one byte per dependency (kCodeNone,
+//           kCodeInbox, or an LDS ring slot 0..255), four per int, then the
+//           inbox addresses this row's value is delivered to (-1: none).
+// A dependency on another wave is read from the consumer's inbox slot
+// (slice, kk, lane), kk = rank of the dependency slot among the slots that
+// can cross waves, so its prefetch needs no other load first.
+template <typename T>
+struct Pack;
+template <>
+struct Pack<double> {
+    using V = double2;
+    static constexpr int N = 2;
+};
+template <>
+struct Pack<float> {
+    using V = float4;
+    static constexpr int N = 4;
+};
+constexpr int packChunks(int n, int per) { return (n + per - 1) / per; }
+constexpr int kCodeNone = 0xFF, kCodeInbox = 0xFE;
+// ints per row: packed codes then K outs
+constexpr int codeInts(int K) { return packChunks(K, 4) + K; }
+// Dependency slots that are given an inbox for a K-slot schedule whose
+// cross-wave dependencies occur in `mask`: structured-box schedules (K = 3)
+// only cross waves in the first two slots; anything else gets all slots.
+inline int effectiveHaloMask(int K, int mask) { return (K == 3 && (mask & ~3) == 0) ? 3 : (1 << K) - 1; }
+
+template <typename T>
+struct DiluView {
+    const int32_t *order = nullptr;      // waves in forward dispatch order
+    const int32_t *waveSlice = nullptr, *waveSteps = nullptr, *waveSys = nullptr;
+    int nWaves = 0, K = 3;
+    const int4 *fCodes = nullptr, *bCodes = nullptr;
+    typename Pack<T>::V *fPack = nullptr, *bPack = nullptr;
+    const double2 *prodPack = nullptr;   // factor records (FP64)
+    double *rDout = nullptr;             // factor: plain rD per row
+    const int32_t *cellOfPos = nullptr;  // factor: padding rows get rD = 0
+    int haloMaskF = 0, haloMaskB = 0;    // dependency slots that may cross waves
+    void *haloF = nullptr, *haloB = nullptr, *haloC = nullptr;   // inboxes
+    int nHaloF = 0, nHaloB = 0;          // inbox sizes (values)
+    int *err = nullptr;                  // bit 2: a halo wait expired
+    // optional trace: per wave (dispatch index) start / end realtime clock
+    // ticks, slow-path hits and steps (4 entries each)
+    unsigned long long *trace = nullptr;
+};
+
+// Forward sweep out = (D+L)^-1 in (with the D^-1 scaling of DILU) and the
+// backward sweep in place; systems whose status is not running are skipped.
+template <typename T>
+hipError_t diluForward(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s);
+template <typename T>
+hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipStream_t s);
+// Factorisation: rD = 1/d for every row of the batch (FP64), written to
+// rDout and into the forward and backward records.
+hipError_t diluFactor(const DiluView<double> &V, hipStream_t s);
+// rD of the FP32 records from the FP64 factor
+hipError_t diluConvertRD(const DiluView<float> &V32, const double *rD64, int nSlices, hipStream_t s);
+hipError_t fillHalo64(void *h, int n, hipStream_t s);
+hipError_t fillHalo32(void *h, int n, hipStream_t s);
+
+}  // namespace mrfg

@@ -1,6 +1,7 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <stdexcept>
 
 namespace mrfg {
@@ -13,9 +14,165 @@ struct RowDeps {
     std::vector<int32_t> up_off, up_nb, up_face;
 };
 
+// One DILU wave under construction: cells[step * 64 + lane] (global cell or -1)
+struct WaveBuild {
+    int key = 0;       // dispatch key: tiles on one anti-diagonal / one level share it
+    int region = 0;
+    int steps = 0;
+    std::vector<int32_t> cells;
+    // structured-box tile (box = false for level waves)
+    bool box = false;
+    int tb = 0, tc = 0, TB = 0, nA = 0, seqAxis = 0, actB = 0, actC = 0;
+};
+
+void tangential(int a, int &b, int &c)
+{
+    if (a == 0) { b = 1; c = 2; }
+    else if (a == 1) { b = 0; c = 2; }
+    else { b = 0; c = 1; }
+}
+
+// Model of one sweep's time (us) for a tiling: the dependency chain of a
+// tile (its steps plus one cross-tile hop per tile boundary crossed) against
+// the bandwidth time of all slots.
+double tileCost(const int n[3], int a, int TB, int TC)
+{
+    int b, c;
+    tangential(a, b, c);
+    int nTB = (n[b] + TB - 1) / TB, nTC = (n[c] + TC - 1) / TC;
+    int steps = n[a] + std::min(TB, n[b]) - 1 + std::min(TC, n[c]) - 1;
+    double slots = (double)nTB * nTC * steps * 64.0;
+    double chain = steps * 0.06 + (nTB + nTC - 2) * 2.0;
+    double bw = slots * 56.0 / 4.0e6;
+    return std::max(chain, bw) + 0.05 * bw;
+}
+
+void boxWaves(const int n[3], int regionIdx, int cellOffset, RegionSchedule &rs, std::vector<WaveBuild> &out)
+{
+    int a = 0;
+    for (int q = 1; q < 3; ++q)
+        if (n[q] > n[a]) a = q;
+    int b, c;
+    tangential(a, b, c);
+    int bestTB = 64;
+    double best = 1e300;
+    for (int TB = 1; TB <= 64; TB *= 2) {
+        int TC = 64 / TB;
+        if (TB > 1 && TB / 2 >= n[b]) continue;
+        if (TC > 1 && TC / 2 >= n[c]) continue;
+        double cst = tileCost(n, a, TB, TC);
+        if (cst < best) {
+            best = cst;
+            bestTB = TB;
+        }
+    }
+    const int TB = bestTB, TC = 64 / TB;
+    rs.kind = "box";
+    for (int q = 0; q < 3; ++q) rs.n[q] = n[q];
+    rs.seqAxis = a;
+    rs.tileB = TB;
+    rs.tileC = TC;
+    const int nTB = (n[b] + TB - 1) / TB, nTC = (n[c] + TC - 1) / TC;
+    for (int tc = 0; tc < nTC; ++tc)
+        for (int tb = 0; tb < nTB; ++tb) {
+            WaveBuild w;
+            w.key = tb + tc;
+            w.region = regionIdx;
+            int maxBl = std::min(TB, n[b] - tb * TB) - 1, maxCl = std::min(TC, n[c] - tc * TC) - 1;
+            w.box = true;
+            w.tb = tb;
+            w.tc = tc;
+            w.TB = TB;
+            w.nA = n[a];
+            w.seqAxis = a;
+            w.actB = maxBl + 1;
+            w.actC = maxCl + 1;
+            w.steps = n[a] + maxBl + maxCl;
+            w.cells.assign((size_t)w.steps * kSlice, -1);
+            for (int cl = 0; cl <= maxCl; ++cl)
+                for (int bl = 0; bl <= maxBl; ++bl) {
+                    int lane = bl + TB * cl;
+                    int idx[3];
+                    idx[b] = tb * TB + bl;
+                    idx[c] = tc * TC + cl;
+                    for (int ia = 0; ia < n[a]; ++ia) {
+                        idx[a] = ia;
+                        int cell = idx[0] + n[0] * (idx[1] + n[1] * idx[2]);
+                        w.cells[(size_t)(ia + bl + cl) * kSlice + lane] = cellOffset + cell;
+                    }
+                }
+            rs.maxSteps = std::max(rs.maxSteps, w.steps);
+            out.push_back(std::move(w));
+            rs.nWaves++;
+        }
+}
+
+void levelWaves(int regionIdx, int c0, int c1, const RowDeps &D, RegionSchedule &rs, std::vector<WaveBuild> &out)
+{
+    std::vector<int32_t> level(c1 - c0, 0);
+    int maxLevel = 0;
+    for (int c = c0; c < c1; ++c) {
+        int lv = 0;
+        for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q) lv = std::max(lv, level[D.lo_nb[q] - c0] + 1);
+        level[c - c0] = lv;
+        maxLevel = std::max(maxLevel, lv);
+    }
+    std::vector<std::vector<int32_t>> byLevel(maxLevel + 1);
+    for (int c = c0; c < c1; ++c) byLevel[level[c - c0]].push_back(c);
+    rs.kind = "levels";
+    for (int lv = 0; lv <= maxLevel; ++lv)
+        for (size_t i = 0; i < byLevel[lv].size(); i += kSlice) {
+            WaveBuild w;
+            w.key = lv;
+            w.region = regionIdx;
+            w.steps = 1;
+            w.cells.assign(kSlice, -1);
+            for (size_t l = 0; l < (size_t)kSlice && i + l < byLevel[lv].size(); ++l) w.cells[l] = byLevel[lv][i + l];
+            out.push_back(std::move(w));
+            rs.nWaves++;
+        }
+    rs.maxSteps = 1;
+}
+
 }  // namespace
 
-SystemLayout buildLayout(const CoupledSystem &sys)
+bool detectBox(const LduRegion &R, int n[3])
+{
+    const int N = R.nCells;
+    if (N < 2 || R.nFaces < 1) return false;
+    int d1 = 0, d2 = 0, d3 = 0;   // smallest distinct offsets
+    for (int f = 0; f < R.nFaces; ++f) {
+        int d = R.uAddr[f] - R.lAddr[f];
+        if (d == d1 || d == d2 || d == d3) continue;
+        if (!d1 || d < d1) { d3 = d2; d2 = d1; d1 = d; }
+        else if (!d2 || d < d2) { d3 = d2; d2 = d; }
+        else if (!d3 || d < d3) d3 = d;
+    }
+    int nx, ny, nz;
+    if (d1 != 1) return false;
+    if (!d2) { nx = N; ny = 1; nz = 1; }
+    else if (!d3) { nx = d2; ny = N / d2; nz = 1; }
+    else { nx = d2; if (d3 % d2) return false; ny = d3 / d2; nz = N / d3; }
+    if ((long long)nx * ny * nz != N || nx < 1 || ny < 1 || nz < 1) return false;
+    // verify the face list is exactly the lexicographic box pattern
+    int f = 0;
+    for (int c = 0; c < N; ++c) {
+        int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        int nb[3] = {i + 1 < nx ? c + 1 : -1, j + 1 < ny ? c + nx : -1, k + 1 < nz ? c + nx * ny : -1};
+        for (int q = 0; q < 3; ++q) {
+            if (nb[q] < 0) continue;
+            if (f >= R.nFaces || R.lAddr[f] != c || R.uAddr[f] != nb[q]) return false;
+            ++f;
+        }
+    }
+    if (f != R.nFaces) return false;
+    n[0] = nx;
+    n[1] = ny;
+    n[2] = nz;
+    return true;
+}
+
+SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
 {
     sys.check();
     SystemLayout L;
@@ -62,43 +219,101 @@ SystemLayout buildLayout(const CoupledSystem &sys)
             }
         }
     }
+    int K = 3;
+    for (int c = 0; c < N; ++c) K = std::max(K, std::max(nLo[c], nUp[c]));
+    L.K = K;
 
-    // ---- levels (merged over regions) ----
-    std::vector<int32_t> level(N, 0);
-    int maxLevel = 0;
-    for (int c = 0; c < N; ++c) {
-        int lv = 0;
-        for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q) lv = std::max(lv, level[D.lo_nb[q]] + 1);
-        level[c] = lv;
-        maxLevel = std::max(maxLevel, lv);
+    // ---- DILU schedule: waves of every region, ordered by dispatch key ----
+    std::vector<WaveBuild> waves;
+    L.schedules.resize(nR);
+    for (int r = 0; r < nR; ++r) {
+        int n[3];
+        if (!preferLevels && detectBox(sys.regions[r], n))
+            boxWaves(n, r, L.regionOffset[r], L.schedules[r], waves);
+        else
+            levelWaves(r, L.regionOffset[r], L.regionOffset[r + 1], D, L.schedules[r], waves);
     }
-    L.nLevels = N ? maxLevel + 1 : 0;
-    std::vector<int32_t> perLevel(L.nLevels + 1, 0);
-    for (int c = 0; c < N; ++c) perLevel[level[c] + 1]++;
-    L.levelSliceBegin.assign(L.nLevels + 1, 0);
-    std::vector<int32_t> levelPosBegin(L.nLevels + 1, 0);
-    for (int lv = 0; lv < L.nLevels; ++lv) {
-        int slices = (perLevel[lv + 1] + kSlice - 1) / kSlice;
-        L.levelSliceBegin[lv + 1] = L.levelSliceBegin[lv] + slices;
-        levelPosBegin[lv + 1] = L.levelSliceBegin[lv + 1] * kSlice;
+    std::vector<int> ord(waves.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return waves[x].key < waves[y].key; });
+    L.nWaves = (int)waves.size();
+    L.waveSlice.resize(L.nWaves);
+    L.waveSteps.resize(L.nWaves);
+    L.waveKey.resize(L.nWaves);
+    int slice = 0;
+    for (int wi = 0; wi < L.nWaves; ++wi) {
+        const WaveBuild &w = waves[ord[wi]];
+        L.waveSlice[wi] = slice;
+        L.waveSteps[wi] = w.steps;
+        L.waveKey[wi] = w.key;
+        slice += w.steps;
     }
-    L.nSlices = L.levelSliceBegin[L.nLevels];
-    L.nPos = L.nSlices * kSlice;
+    L.nSlices = slice;
+    L.nPos = slice * kSlice;
     L.cellOfPos.assign(L.nPos, -1);
-    L.posOfCell.resize(N);
+    L.posOfCell.assign(N, -1);
+    std::vector<int32_t> waveOfSlice(L.nSlices);
+    for (int wi = 0; wi < L.nWaves; ++wi) {
+        const WaveBuild &w = waves[ord[wi]];
+        for (int t = 0; t < w.steps; ++t) {
+            waveOfSlice[L.waveSlice[wi] + t] = wi;
+            for (int l = 0; l < kSlice; ++l) {
+                int c = w.cells[(size_t)t * kSlice + l];
+                if (c < 0) continue;
+                int p = (L.waveSlice[wi] + t) * kSlice + l;
+                L.cellOfPos[p] = c;
+                L.posOfCell[c] = p;
+            }
+        }
+    }
+    for (int c = 0; c < N; ++c)
+        if (L.posOfCell[c] < 0) throw std::runtime_error("DILU schedule does not cover every cell");
+
+    // structured-box waves: tile neighbours and fixed-slot records (slot =
+    // 2 - axis: z, y, x, the face order of a cell's lower neighbours and the
+    // reverse face order of its upper neighbours)
+    L.waveBox.assign(L.nWaves, 0);
+    L.boxMeta.assign((size_t)L.nWaves * kBoxMeta, -1);
     {
-        std::vector<int32_t> fill(levelPosBegin.begin(), levelPosBegin.end() - 1);
-        for (int c = 0; c < N; ++c) {
-            int p = fill[level[c]]++;
-            L.posOfCell[c] = p;
-            L.cellOfPos[p] = c;
+        std::vector<std::vector<int>> tileWave(nR);
+        std::vector<int> tileNB(nR, 0);
+        for (int wi = 0; wi < L.nWaves; ++wi) {
+            const WaveBuild &w = waves[ord[wi]];
+            if (!w.box) continue;
+            const RegionSchedule &rs = L.schedules[w.region];
+            int b, cc;
+            tangential(rs.seqAxis, b, cc);
+            int nTB = (rs.n[b] + rs.tileB - 1) / rs.tileB, nTC = (rs.n[cc] + rs.tileC - 1) / rs.tileC;
+            if (tileWave[w.region].empty()) tileWave[w.region].assign((size_t)nTB * nTC, -1);
+            tileNB[w.region] = nTB;
+            tileWave[w.region][(size_t)w.tc * nTB + w.tb] = wi;
+        }
+        for (int wi = 0; wi < L.nWaves; ++wi) {
+            const WaveBuild &w = waves[ord[wi]];
+            if (!w.box) continue;
+            L.waveBox[wi] = 1;
+            const RegionSchedule &rs = L.schedules[w.region];
+            int b, cc;
+            tangential(rs.seqAxis, b, cc);
+            const int nTB = tileNB[w.region], nTC = (rs.n[cc] + rs.tileC - 1) / rs.tileC;
+            auto tw = [&](int tb, int tc) {
+                return tb < 0 || tc < 0 || tb >= nTB || tc >= nTC ? -1 : tileWave[w.region][(size_t)tc * nTB + tb];
+            };
+            int32_t *m = &L.boxMeta[(size_t)wi * kBoxMeta];
+            m[kBmTB] = w.TB;
+            m[kBmNA] = w.nA;
+            m[kBmPrevB] = tw(w.tb - 1, w.tc);
+            m[kBmPrevC] = tw(w.tb, w.tc - 1);
+            m[kBmNextB] = tw(w.tb + 1, w.tc);
+            m[kBmNextC] = tw(w.tb, w.tc + 1);
+            m[kBmPerm] = (2 - rs.seqAxis) | ((2 - b) << 2) | ((2 - cc) << 4);
+            m[kBmActB] = w.actB;
+            m[kBmActC] = w.actC;
         }
     }
 
-    // per-position coefficient accessors
     auto regionOf = [&](int c) {
-        int r = int(std::upper_bound(L.regionOffset.begin(), L.regionOffset.end(), c) - L.regionOffset.begin()) - 1;
-        return r;
+        return int(std::upper_bound(L.regionOffset.begin(), L.regionOffset.end(), c) - L.regionOffset.begin()) - 1;
     };
 
     L.diag.assign(L.nPos, 0.0);
@@ -140,7 +355,7 @@ SystemLayout buildLayout(const CoupledSystem &sys)
     }
 
     // ---- interface faces per row (interface order, then face order) ----
-    std::vector<std::vector<std::pair<int, int>>> rowFaces(N);   // (iface*2+side, face)
+    std::vector<std::vector<std::pair<int, int>>> rowFaces(N);
     for (size_t i = 0; i < sys.interfaces.size(); ++i) {
         const CoupledInterface &I = sys.interfaces[i];
         const InterfaceSide *sides[2] = {&I.a, &I.b};
@@ -151,43 +366,48 @@ SystemLayout buildLayout(const CoupledSystem &sys)
         }
     }
 
-    // ---- per-slice ELL (SpMV) and DILU arrays ----
+    // ---- per-slice ELL (SpMV) ----
     L.ellOff.assign(L.nSlices + 1, 0);
-    L.lOff.assign(L.nSlices + 1, 0);
-    L.uOff.assign(L.nSlices + 1, 0);
     L.ifMask.assign(L.nSlices, 0);
     L.ifBase.assign(L.nSlices, 0);
     L.ifRowFace.assign(1, 0);
     L.ifPairOff.assign(1, 0);
     for (int s = 0; s < L.nSlices; ++s) {
-        int wE = 0, wL = 0, wU = 0;
+        int wE = 0;
         for (int l = 0; l < kSlice; ++l) {
             int c = L.cellOfPos[s * kSlice + l];
             if (c < 0) continue;
-            int nl = D.lo_off[c + 1] - D.lo_off[c], nu = D.up_off[c + 1] - D.up_off[c];
-            wE = std::max(wE, nl + nu);
-            wL = std::max(wL, nl);
-            wU = std::max(wU, nu);
+            wE = std::max(wE, (D.lo_off[c + 1] - D.lo_off[c]) + (D.up_off[c + 1] - D.up_off[c]));
         }
         L.ellOff[s + 1] = L.ellOff[s] + wE;
-        L.lOff[s + 1] = L.lOff[s] + wL;
-        L.uOff[s + 1] = L.uOff[s] + wU;
     }
     L.ellCol.assign((size_t)L.ellOff[L.nSlices] * kSlice, 0);
     L.ellVal.assign((size_t)L.ellOff[L.nSlices] * kSlice, 0.0);
-    L.lCol.assign((size_t)L.lOff[L.nSlices] * kSlice, -1);
-    L.lVal.assign((size_t)L.lOff[L.nSlices] * kSlice, 0.0);
-    L.lProd.assign((size_t)L.lOff[L.nSlices] * kSlice, 0.0);
-    L.uCol.assign((size_t)L.uOff[L.nSlices] * kSlice, -1);
-    L.uVal.assign((size_t)L.uOff[L.nSlices] * kSlice, 0.0);
+    const size_t nDep = (size_t)L.nSlices * K * kSlice;
+    L.fCode.assign(nDep, kNoDep);
+    L.bCode.assign(nDep, kNoDep);
+    L.fVal.assign(nDep, 0.0);
+    L.fProd.assign(nDep, 0.0);
+    L.bVal.assign(nDep, 0.0);
+    L.fHalo.assign(L.nPos, -1);
+    L.bHalo.assign(L.nPos, -1);
+    L.boxF.assign((size_t)L.nSlices * 3 * kSlice, 0.0);
+    L.boxB.assign((size_t)L.nSlices * 3 * kSlice, 0.0);
+    L.boxP.assign((size_t)L.nSlices * 3 * kSlice, 0.0);
+
+    auto waveStep = [&](int p, int &w, int &t, int &lane) {
+        int s = p / kSlice;
+        lane = p % kSlice;
+        w = waveOfSlice[s];
+        t = s - L.waveSlice[w];
+    };
 
     for (int s = 0; s < L.nSlices; ++s) {
         L.ifBase[s] = (int32_t)L.ifRowFace.size() - 1;
+        const int wE = L.ellOff[s + 1] - L.ellOff[s];
         for (int l = 0; l < kSlice; ++l) {
             int p = s * kSlice + l;
             int c = L.cellOfPos[p];
-            int wE = L.ellOff[s + 1] - L.ellOff[s];
-            int wL = L.lOff[s + 1] - L.lOff[s], wU = L.uOff[s + 1] - L.uOff[s];
             // padding entries of the SpMV point at the row itself with value 0
             for (int k = 0; k < wE; ++k) L.ellCol[(size_t)(L.ellOff[s] + k) * kSlice + l] = p;
             if (c < 0) continue;
@@ -205,25 +425,68 @@ SystemLayout buildLayout(const CoupledSystem &sys)
                 L.ellCol[e] = L.posOfCell[D.up_nb[q]];
                 L.ellVal[e] = R.upper[D.up_face[q]];
             }
-            (void)wE;
-            // DILU lower deps: ascending face order
+
+            int w, t, lane;
+            waveStep(p, w, t, lane);
+            const int T = L.waveSteps[w];
+            if (L.waveBox[w]) {
+                // fixed slots by axis of the neighbour: slot 2 - axis
+                const RegionSchedule &rs = L.schedules[r];
+                const int o = L.regionOffset[r];
+                const int sxy = rs.n[0] * rs.n[1];
+                auto slotOf = [&](int d) { return d == 1 ? 2 : d == rs.n[0] ? 1 : d == sxy ? 0 : -1; };
+                for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q) {
+                    int sl = slotOf((c - o) - (D.lo_nb[q] - o));
+                    if (sl < 0) throw std::runtime_error("box region with an unexpected lower neighbour");
+                    int f = D.lo_face[q];
+                    size_t e = ((size_t)s * 3 + sl) * kSlice + l;
+                    L.boxF[e] = lo[f];
+                    L.boxP[e] = R.upper[f] * lo[f];
+                }
+                for (int q = D.up_off[c]; q < D.up_off[c + 1]; ++q) {
+                    int sl = slotOf((D.up_nb[q] - o) - (c - o));
+                    if (sl < 0) throw std::runtime_error("box region with an unexpected upper neighbour");
+                    L.boxB[((size_t)s * 3 + sl) * kSlice + l] = R.upper[D.up_face[q]];
+                }
+            }
+            // forward sweep: lower neighbours, ascending face order
             k = 0;
             for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q, ++k) {
-                size_t e = (size_t)(L.lOff[s] + k) * kSlice + l;
                 int f = D.lo_face[q];
-                L.lCol[e] = L.posOfCell[D.lo_nb[q]];
-                L.lVal[e] = lo[f];
-                L.lProd[e] = R.upper[f] * lo[f];
+                int pd = L.posOfCell[D.lo_nb[q]];
+                int wd, td, ld;
+                waveStep(pd, wd, td, ld);
+                size_t e = ((size_t)s * K + k) * kSlice + l;
+                L.fVal[e] = lo[f];
+                L.fProd[e] = R.upper[f] * lo[f];
+                if (wd == w && t - td >= 1 && t - td <= kRing - 1) {
+                    L.fCode[e] = -((td % kRing) * kSlice + ld + 1);
+                } else {
+                    if (wd == w ? td >= t : wd > w)
+                        throw std::runtime_error("DILU schedule violates a forward dependency");
+                    if (L.fHalo[pd] < 0) L.fHalo[pd] = L.nHaloF++;
+                    L.fCode[e] = L.fHalo[pd];
+                }
             }
-            (void)wL;
-            // DILU upper deps: reverse face order
+            // backward sweep: upper neighbours, reverse face order; processing
+            // index i = T-1-t, so the ring slot of a dependency uses its own i
             k = 0;
             for (int q = D.up_off[c + 1] - 1; q >= D.up_off[c]; --q, ++k) {
-                size_t e = (size_t)(L.uOff[s] + k) * kSlice + l;
-                L.uCol[e] = L.posOfCell[D.up_nb[q]];
-                L.uVal[e] = R.upper[D.up_face[q]];
+                int pd = L.posOfCell[D.up_nb[q]];
+                int wd, td, ld;
+                waveStep(pd, wd, td, ld);
+                size_t e = ((size_t)s * K + k) * kSlice + l;
+                L.bVal[e] = R.upper[D.up_face[q]];
+                if (wd == w && td - t >= 1 && td - t <= kRing - 1) {
+                    int id = T - 1 - td;
+                    L.bCode[e] = -((id % kRing) * kSlice + ld + 1);
+                } else {
+                    if (wd == w ? td <= t : wd < w)
+                        throw std::runtime_error("DILU schedule violates a backward dependency");
+                    if (L.bHalo[pd] < 0) L.bHalo[pd] = L.nHaloB++;
+                    L.bCode[e] = L.bHalo[pd];
+                }
             }
-            (void)wU;
             // interface faces of this row
             if (!rowFaces[c].empty()) {
                 L.ifMask[s] |= 1ULL << l;

@@ -26,10 +26,10 @@ def small_cases():
     }
 
 
-def _ctx(m, case):
+def _ctx(m, case, schedule="auto"):
     A = m.CoupledFvMatrix(0)
     A.add_case(case)
-    A.commit()
+    A.commit(schedule)
     return A
 
 
@@ -49,11 +49,12 @@ def test_spmv_bitexact(gpu, name):
     assert np.array_equal(y, y_ref), np.abs(y - y_ref).max()
 
 
+@pytest.mark.parametrize("schedule", ["auto", "levels"])
 @pytest.mark.parametrize("name", list(small_cases()))
-def test_dilu_bitexact(gpu, name):
+def test_dilu_bitexact(gpu, name, schedule):
     case = small_cases()[name]
     O = ob.OracleSystem(case)
-    A = _ctx(gpu, case)
+    A = _ctx(gpu, case, schedule)
     rD_ref = O.dilu_factor()
     rD = A.dilu_factor()
     assert np.array_equal(rD, np.concatenate(rD_ref))
@@ -64,12 +65,13 @@ def test_dilu_bitexact(gpu, name):
     assert np.array_equal(w, w_ref), np.abs(w - w_ref).max()
 
 
+@pytest.mark.parametrize("schedule", ["auto", "levels"])
 @pytest.mark.parametrize("name", list(small_cases()))
-def test_bicgstab_fp64_matches_oracle(gpu, name):
+def test_bicgstab_fp64_matches_oracle(gpu, name, schedule):
     case = small_cases()[name]
     O = ob.OracleSystem(case)
     psi_ref, perf_ref = O.solve(tol=1e-10, maxIter=1000)
-    A = _ctx(gpu, case)
+    A = _ctx(gpu, case, schedule)
     perf = A.solve(solver="BiCGStab", tolerance=1e-10, maxIter=1000)[0]
     assert perf.converged and perf_ref["converged"]
     assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
