@@ -172,6 +172,10 @@ int mrfg_spmv(mrfg_ctx *ctx, int sys, const double *x, double *y);
 int mrfg_dilu_factor(mrfg_ctx *ctx, int sys, double *rD);
 /* w = M^-1 r with the DILU preconditioner (coupled Cholesky precondition). */
 int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w);
+/* The same in FP32: the single-precision records of the mixed-precision
+   solver, with rD rounded from the FP64 factor (no counterpart in the
+   reference, which is FP64 only). */
+int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w);
 /* Average device time (ms) of one launch over `reps` launches, from HIP
    events on the context's stream: which 0 = SpMV, 1 = DILU forward sweep,
    2 = DILU backward sweep, 3 = DILU factorisation. */

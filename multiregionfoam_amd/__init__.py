@@ -53,6 +53,7 @@ _lib = None
 
 I32P = C.POINTER(C.c_int32)
 F64P = C.POINTER(C.c_double)
+F32P = C.POINTER(C.c_float)
 
 
 def lib():
@@ -97,6 +98,7 @@ def lib():
         "mrfg_spmv": ([vp, C.c_int, F64P, F64P], C.c_int),
         "mrfg_dilu_factor": ([vp, C.c_int, F64P], C.c_int),
         "mrfg_precondition": ([vp, C.c_int, F64P, F64P], C.c_int),
+        "mrfg_precondition_fp32": ([vp, C.c_int, F32P, F32P], C.c_int),
         "mrfg_time_kernel": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_layout_info": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "mrfg_trace_dilu": ([vp, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -115,7 +117,7 @@ EXPORTED = (
     "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes").split()
+    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes").split()
 
 
 def _p(a, t):
@@ -369,6 +371,13 @@ class CoupledFvMatrix:
         r = np.ascontiguousarray(np.concatenate(r) if isinstance(r, (list, tuple)) else r, dtype=np.float64)
         w = np.zeros_like(r)
         self._check(lib().mrfg_precondition(self._h, sys, _p(r, C.c_double), _p(w, C.c_double)))
+        return w
+
+    def precondition_fp32(self, r, sys=0):
+        """w = M^-1 r with the FP32 records of the mixed-precision solver."""
+        r = np.ascontiguousarray(np.concatenate(r) if isinstance(r, (list, tuple)) else r, dtype=np.float32)
+        w = np.zeros_like(r)
+        self._check(lib().mrfg_precondition_fp32(self._h, sys, _p(r, C.c_float), _p(w, C.c_float)))
         return w
 
     def time_kernel(self, which, reps=20):

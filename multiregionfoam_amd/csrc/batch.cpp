@@ -138,7 +138,10 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     std::vector<unsigned long long> ifMask;
     std::vector<int32_t> cellOfPos, fCode, bCode, fHalo, bHalo, waveSlice, waveSteps, waveSys;
     std::vector<double> diag, source, sumA, psi0, fProd;
-    std::vector<std::array<int, 3>> waveKey;   // (key, system, local index)
+    std::vector<std::array<int, 3>> waveKey;   // (key, system, global wave index)
+    std::vector<uint8_t> waveBox;
+    std::vector<int32_t> boxMeta;
+    int boxIn = 0;
     hEllVal_.clear(); hDiag_.clear(); hIfCoeff_.clear(); hIfPairW_.clear(); hFVal_.clear(); hBVal_.clear();
     std::vector<BlockEntry> blocks;
     hInfo_.assign(nS, SysInfo{});
@@ -197,6 +200,21 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
             waveSlice.push_back(L.waveSlice[w] + slice);
             waveSteps.push_back(L.waveSteps[w]);
             waveSys.push_back(q);
+            waveBox.push_back(L.waveBox[w]);
+            // box tile geometry with batch-wide wave, slice and inbox indices
+            const int32_t *lm = &L.boxMeta[(size_t)w * kBoxMeta];
+            std::array<int32_t, kBoxMeta> gm;
+            for (int k = 0; k < kBoxMeta; ++k) gm[k] = lm[k];
+            if (L.waveBox[w]) {
+                for (int k : {kBmPrevB, kBmPrevC, kBmNextB, kBmNextC})
+                    gm[k] = lm[k] >= 0 ? lm[k] + I.waveBegin : -1;
+                gm[kBmSlice] = L.waveSlice[w] + slice;
+                gm[kBmSteps] = L.waveSteps[w];
+                gm[kBmInbox] = boxIn;
+                gm[kBmSys] = q;
+                boxIn += L.waveSteps[w] * (lm[kBmTB] + kSlice / lm[kBmTB]);
+            }
+            boxMeta.insert(boxMeta.end(), gm.begin(), gm.end());
         }
         I.nWaves = L.nWaves;
         I.blkBegin = (int)blocks.size();
@@ -316,10 +334,18 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     nWaves_ = (int)waveSlice.size();
 
     // forward dispatch order: by key (tile anti-diagonal / level), systems
-    // interleaved; every wave's producers precede it
+    // interleaved; every wave's producers precede it.  Box tile waves and
+    // general waves never depend on each other (DILU dependencies stay inside
+    // a region) and run in kernels of their own.
     std::sort(waveKey.begin(), waveKey.end());
-    std::vector<int32_t> order(nWaves_);
-    for (int i = 0; i < nWaves_; ++i) order[i] = waveKey[i][2];
+    std::vector<int32_t> order, boxOrder;
+    for (int i = 0; i < nWaves_; ++i) (waveBox[waveKey[i][2]] ? boxOrder : order).push_back(waveKey[i][2]);
+    nGenWaves_ = (int)order.size();
+    nBoxWaves_ = (int)boxOrder.size();
+    nBoxIn_ = boxIn;
+    if (nGenWaves_ == 0) nHaloF_ = nHaloB_ = 0;
+    if (order.empty()) order.push_back(0);
+    if (boxOrder.empty()) boxOrder.push_back(0);
 
     hipStream_t st = stream_;
     ellOff_.upload(ellOff, st);
@@ -333,12 +359,15 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     blocks_.upload(blocks, st);
     info_.upload(hInfo_, st);
     order_.upload(order, st);
+    boxOrder_.upload(boxOrder, st);
+    boxMeta_.upload(boxMeta, st);
     waveSlice_.upload(waveSlice, st);
     waveSteps_.upload(waveSteps, st);
     waveSys_.upload(waveSys, st);
-    haloF64_.alloc(std::max(1, nHaloF_));
-    haloB64_.alloc(std::max(1, nHaloB_));
-    haloC64_.alloc(std::max(1, nHaloF_));
+    // each inbox buffer: general part, then the box part
+    haloF64_.alloc(std::max(1, nHaloF_ + nBoxIn_));
+    haloB64_.alloc(std::max(1, nHaloB_ + nBoxIn_));
+    haloC64_.alloc(std::max(1, nHaloF_ + nBoxIn_));
     HIP_OK(fillHalo64(haloF64_.p, (int)haloF64_.n, st));
     HIP_OK(fillHalo64(haloB64_.p, (int)haloB64_.n, st));
     HIP_OK(fillHalo64(haloC64_.p, (int)haloC64_.n, st));
@@ -360,7 +389,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     }
     st64_.alloc(nS);
     st32_.alloc(nS);
-    partials_.alloc((size_t)std::max(1, nBlk_) * 4);
+    partials_.alloc((size_t)std::max(1, nBlk_) * 8);   // 4 double-double sums per block
     factorised_ = false;
     haveFloat_ = false;
     for (auto &v : v32_) v.release();
@@ -393,8 +422,8 @@ void Batch::ensureFloat()
         v.alloc(nPos_);
         v.zero(st);
     }
-    haloF32_.alloc(std::max(1, nHaloF_));
-    haloB32_.alloc(std::max(1, nHaloB_));
+    haloF32_.alloc(std::max(1, nHaloF_ + nBoxIn_));
+    haloB32_.alloc(std::max(1, nHaloB_ + nBoxIn_));
     HIP_OK(fillHalo32(haloF32_.p, (int)haloF32_.n, st));
     HIP_OK(fillHalo32(haloB32_.p, (int)haloB32_.n, st));
     haveFloat_ = true;
@@ -428,8 +457,12 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.waveSlice = waveSlice_.p;
     D.waveSteps = waveSteps_.p;
     D.waveSys = waveSys_.p;
-    D.nWaves = nWaves_;
+    D.nWaves = nGenWaves_;
     D.K = K_;
+    D.boxOrder = boxOrder_.p;
+    D.nBoxWaves = nBoxWaves_;
+    D.boxMeta = boxMeta_.p;
+    D.nBoxIn = nBoxIn_;
     D.fCodes = fCodes_.p;
     D.bCodes = bCodes_.p;
     D.fPack = m.fPack.p;
@@ -444,6 +477,9 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.haloC = haloC64_.p;
     D.nHaloF = nHaloF_;
     D.nHaloB = nHaloB_;
+    D.boxInF = (char *)hF + (size_t)nHaloF_ * sizeof(T);
+    D.boxInB = (char *)hB + (size_t)nHaloB_ * sizeof(T);
+    D.boxInC = (char *)haloC64_.p + (size_t)nHaloF_ * sizeof(double);
     D.err = err_.p;
     return D;
 }
@@ -717,6 +753,29 @@ void Batch::precondition(const double *rHost, double *wHost, int sys)
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
     HIP_OK(hipMemcpy(w.data(), dw.p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
+    for (long long c = 0; c < L.nCells; ++c) wHost[c] = w[posBegin(sys) + L.posOfCell[c]];
+}
+
+void Batch::precondition32(const float *rHost, float *wHost, int sys)
+{
+    ensureFloat();
+    factorise();
+    const SystemLayout &L = *lay_.at(sys);
+    std::vector<float> r(nPos_, 0.0f), w(nPos_);
+    for (long long c = 0; c < L.nCells; ++c) r[posBegin(sys) + L.posOfCell[c]] = rHost[c];
+    markAllRunning();
+    DevBuf<float> dr, dw;
+    dr.upload(r, stream_);
+    dw.alloc(nPos_);
+    dw.zero(stream_);
+    Krylov<float> K;
+    bindKrylov(K);
+    K.state = st64_.p;   // running flags of the systems (set above)
+    DiluView<float> D = diluView(m32_, haloF32_.p, haloB32_.p);
+    HIP_OK(launchPrecondition<float>(D, K, dr.p, dw.p, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    checkDeviceError();
+    HIP_OK(hipMemcpy(w.data(), dw.p, nPos_ * sizeof(float), hipMemcpyDeviceToHost));
     for (long long c = 0; c < L.nCells; ++c) wHost[c] = w[posBegin(sys) + L.posOfCell[c]];
 }
 

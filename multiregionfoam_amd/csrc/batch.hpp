@@ -95,6 +95,7 @@ public:
     void spmv(const double *xHost, double *yHost, int sys);           // y = A x
     void diluFactor(int sys, double *rDHost);                        // rD per cell
     void precondition(const double *rHost, double *wHost, int sys);   // w = M^-1 r
+    void precondition32(const float *rHost, float *wHost, int sys);   // FP32 records, rD from the FP64 factor
 
     // Time `reps` repetitions of one kernel on resident data; returns the
     // average device time per launch (ms), measured with HIP events on the
@@ -143,6 +144,7 @@ private:
     std::vector<const CoupledSystem *> sys_;
     std::vector<int> sliceBegin_;
     int nSlices_ = 0, nPos_ = 0, nBlk_ = 0, nWaves_ = 0, K_ = 3, nHaloF_ = 0, nHaloB_ = 0;
+    int nGenWaves_ = 0, nBoxWaves_ = 0, nBoxIn_ = 0;
     bool factorised_ = false, haveFloat_ = false;
 
     // structure (shared by both precisions)
@@ -152,7 +154,7 @@ private:
     DevBuf<BlockEntry> blocks_;
     DevBuf<SysInfo> info_;
     // DILU schedule
-    DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_;
+    DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_, boxOrder_, boxMeta_;
     DevBuf<int4> fCodes_, bCodes_;
     DevBuf<double2> prodPack_;
     int haloMaskF_ = 0, haloMaskB_ = 0;

@@ -450,6 +450,15 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w)
     });
 }
 
+int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w)
+{
+    if (!ctx || !r || !w) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).precondition32(r, w, sys);
+    });
+}
+
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms)
 {
     if (!ctx || !ms || reps < 1 || which < 0 || which > 3) return MRFG_ERR_ARG;

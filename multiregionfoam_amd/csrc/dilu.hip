@@ -18,8 +18,8 @@
 //
 // A step is latency-bound (one LDS round trip plus a short FMA chain), so
 // the kernel keeps memory out of the critical path: each row's record is a
-// few 16-byte chunks, records are prefetched kPrefetch steps ahead and halo
-// values kHalf steps ahead, every prefetch load is unconditional (the
+// few 16-byte chunks, records and inbox values are prefetched kPrefetch
+// steps ahead, every prefetch load is unconditional (the
 // compiler then counts outstanding loads exactly instead of draining them),
 // and the fast path has no divergent branch.
 #include <hip/hip_runtime.h>
@@ -31,7 +31,6 @@ namespace mrfg {
 namespace {
 
 constexpr int kPrefetch = 8;
-constexpr int kHalf = kPrefetch / 2;
 constexpr int kPollLimit = 1 << 20;
 
 template <typename T>
@@ -249,6 +248,245 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
         for (int q = blockIdx.x * 64 + lane; q < nClear; q += gridDim.x * 64) inboxClear[q] = kSent;
 }
 
+// ---------------------------------------------------------------------------
+// Structured-box tile waves.
+//
+// Lane l = bl + TB*cl of a tile wave owns grid line (bl, cl) of the tile's
+// TB x TC lines along the sequential axis; at step t it computes cell
+// ia = t - bl - cl of its line.  Each of the cell's three dependencies was
+// computed one step earlier (in processing order): along the line by the
+// lane itself (kept in a register), across the line by lane l -/+ 1 and
+// lane l -/+ TB (exchanged through 64 LDS values), and across the tile's
+// edge by the neighbouring tile's wave, which delivers it to this wave's
+// inbox: (TB + TC) values per step, [cl] for the b edge and [TC + bl] for
+// the c edge.  Records hold the three coefficients in fixed slots (slot
+// 2 - axis of the neighbour, which is also the reference's face order), so a
+// step loads its record, its input and two inbox values, all addressed
+// without another load.
+// ---------------------------------------------------------------------------
+
+// Prefetch distances.  A step is bound by its own instruction issue and LDS
+// round trip, not by memory latency, and a deeper inbox prefetch makes every
+// consumer tile lag its producer by that many more steps.
+constexpr int kBoxAhead = 4;     // steps between a record load and its use
+constexpr int kBoxInAhead = 3;   // steps between an inbox load and its use (<= kBoxAhead)
+
+// KT: the batch's record width K when it is 3 (records of two FP64 / one
+// FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
+// rD in a chunk of its own).
+template <typename T, int KT, int MODE, int AX>
+__device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m, const T *__restrict__ in, T *out,
+                                         typename Bits<T>::U *inbox, T *xch, unsigned long long &nSlow)
+{
+    using U = typename Bits<T>::U;
+    using R = typename RecordOf<T, MODE>::V;
+    constexpr int N = RecordOf<T, MODE>::N;
+    constexpr int G2 = 2 / N + 1;                // chunks holding slots 0..2
+    constexpr bool kExtra = KT == 0;             // rD (diag) in a chunk of its own
+    constexpr int NL = G2 + (kExtra ? 1 : 0);
+    constexpr U kSent = Bits<T>::kSent;
+    constexpr bool kFwd = MODE != kDiluBwd;
+    // record slot of the dependency along the sequential axis and along b, c
+    constexpr int kAxB = AX == 0 ? 1 : 0;
+    constexpr int kSlotA = 2 - AX, kSlotB = 2 - kAxB;   // the c neighbour holds the third slot
+    const int K = KT ? KT : V.K;
+    const int G = packChunks(K + 1, N);          // chunks per row of the record array
+    const int gLast = K / N, eLast = K % N;      // chunk / element of rD (diag)
+    const R *recs = MODE == kDiluFactor ? (const R *)V.prodPack : (const R *)(MODE == kDiluBwd ? V.bPack : V.fPack);
+
+    // per-lane constants; every address below is a wave-uniform base plus
+    // one of these 32-bit lane offsets
+    const unsigned lane = threadIdx.x;
+    const int TB = m[kBmTB], lg = __builtin_ctz(TB), TC = 64 >> lg, TBC = TB + TC;
+    const int nA = m[kBmNA], actB = m[kBmActB], actC = m[kBmActC];
+    const int s0 = m[kBmSlice], nSteps = m[kBmSteps], base = m[kBmInbox];
+    const int prevB = m[kBmPrevB], prevC = m[kBmPrevC], nextB = m[kBmNextB], nextC = m[kBmNextC];
+    const int bl = lane & (TB - 1), cl = lane >> lg;
+    const bool inTile = bl < actB && cl < actC;
+    // where each dependency comes from: the lane itself (sequential axis),
+    // a neighbouring lane, or the inbox on the tile's edge
+    const bool edgeB = kFwd ? bl == 0 : bl == TB - 1;
+    const bool edgeC = kFwd ? cl == 0 : cl == TC - 1;
+    const bool hasB = kFwd ? (bl > 0 || prevB >= 0) : (bl + 1 < actB || (edgeB && nextB >= 0));
+    const bool hasC = kFwd ? (cl > 0 || prevC >= 0) : (cl + 1 < actC || (edgeC && nextC >= 0));
+    const bool useB = edgeB && hasB, useC = edgeC && hasC;
+    const unsigned srcB = (kFwd ? lane - 1 : lane + 1) & 63;
+    const unsigned srcC = (kFwd ? lane - TB : lane + TB) & 63;
+    const unsigned inB = base + cl, inC = base + TC + bl;   // at step t: inbox + t * TBC
+    // where this lane's values go: the consumer's step is t -/+ (TB - 1)
+    // (resp. TC - 1), into the consumer's inbox at [cl] (resp. [TC + bl])
+    const int toB = kFwd ? nextB : prevB, toC = kFwd ? nextC : prevC;
+    const bool sendB = (kFwd ? bl == TB - 1 : bl == 0) && toB >= 0;
+    const bool sendC = (kFwd ? cl == TC - 1 : cl == 0) && toC >= 0;
+    const int dB = kFwd ? 1 - TB : TB - 1, dC = kFwd ? 1 - TC : TC - 1;
+    const unsigned outB = (toB >= 0 ? V.boxMeta[(size_t)toB * kBoxMeta + kBmInbox] : 0) + cl;
+    const unsigned outC = (toC >= 0 ? V.boxMeta[(size_t)toC * kBoxMeta + kBmInbox] : 0) + TC + bl;
+
+    struct Step {
+        R v[NL];
+        T a;
+    };
+    auto stepOf = [&](int i) {
+        i = i < nSteps ? i : nSteps - 1;
+        return kFwd ? i : nSteps - 1 - i;
+    };
+    auto load = [&](int i, Step &S) {
+        const int s = s0 + stepOf(i);
+        const R *row = recs + (size_t)s * G * 64;
+#pragma unroll
+        for (int g = 0; g < G2; ++g) S.v[g] = row[g * 64 + lane];
+        if (kExtra) S.v[NL - 1] = row[gLast * 64 + lane];
+        if (MODE != kDiluFactor) S.a = (in + (size_t)s * 64)[lane];
+    };
+    auto loadIn = [&](int i, U &hb, U &hc) {
+        const U *ib = inbox + (size_t)stepOf(i) * TBC;
+        hb = haloLoad(ib, inB);
+        hc = haloLoad(ib, inC);
+    };
+
+    // the exchange values start finite: a missing neighbour's zero
+    // coefficient multiplies whatever its lane holds
+    xch[lane] = T(0);
+    __builtin_amdgcn_wave_barrier();
+    Step buf[kBoxAhead];
+    U hbuf[kBoxAhead], cbuf[kBoxAhead];
+#pragma unroll
+    for (int u = 0; u < kBoxInAhead; ++u) loadIn(u, hbuf[u], cbuf[u]);
+#pragma unroll
+    for (int u = 0; u < kBoxAhead; ++u) load(u, buf[u]);
+    T yprev = T(0);
+    // the loop runs whole groups of steps: a step past the end is invalid for
+    // every lane, reads clamped addresses and stores nothing, so the only
+    // conditional code is the wave-uniform slow path and the stores (the body
+    // is unrolled over 4 rounds of the prefetch ring: the compiler then keeps
+    // the prefetches in flight across the loop's back edge)
+    for (int i0 = 0; i0 < nSteps; i0 += 4 * kBoxAhead) {
+#pragma unroll
+        for (int uu = 0; uu < 4 * kBoxAhead; ++uu) {
+            const int u = uu % kBoxAhead;
+            const int i = i0 + uu;
+            const bool inRange = i < nSteps;
+            Step &S = buf[u];
+            U &hb = hbuf[u];
+            U &hc = cbuf[u];
+            const int t = kFwd ? i : nSteps - 1 - i;
+            const int ia = t - bl - cl;
+            const bool valid = inTile && (unsigned)ia < (unsigned)nA;
+            const bool hasA = kFwd ? ia >= 1 : ia + 1 < nA;
+            const bool needB = valid & useB, needC = valid & useC;
+            const bool miss = (needB & (hb == kSent)) | (needC & (hc == kSent));
+            if (__builtin_expect(__any(miss), 0)) {
+                // rare slow path (wave-uniform): wait for the late values and
+                // reload the inbox values already requested for later steps
+                ++nSlow;
+                const U *ib = inbox + (size_t)t * TBC;
+                if (needB && hb == kSent) hb = haloWait(ib, (int)inB, kSent, V.err);
+                if (needC && hc == kSent) hc = haloWait(ib, (int)inC, kSent, V.err);
+#pragma unroll
+                for (int d = 1; d < kBoxInAhead; ++d) {
+                    const U *fb = inbox + (size_t)stepOf(i + d) * TBC;
+                    U &vb = hbuf[(u + d) % kBoxAhead];
+                    U &vc = cbuf[(u + d) % kBoxAhead];
+                    if (vb == kSent) vb = haloLoad(fb, inB);
+                    if (vc == kSent) vc = haloLoad(fb, inC);
+                }
+                __builtin_amdgcn_s_waitcnt(0);
+            }
+            T xb = xch[srcB], xc = xch[srcC];
+            // both exchange reads are issued by every lane (not sunk into the
+            // select), so the step has no divergent code
+            asm volatile("" : "+v"(xb), "+v"(xc));
+            const T vA = yprev;
+            const T vB = useB ? Bits<T>::from(hb) : xb;
+            const T vC = useC ? Bits<T>::from(hc) : xc;
+            auto slotVal = [&](int q) -> T { return (T)elem(S.v[q / N], q % N); };
+            T last;
+            if (kExtra) last = (T)elem(S.v[NL - 1], eLast);
+            else last = (T)elem(S.v[3 / N], 3 % N);
+            T y;
+            if (MODE == kDiluFwd) y = S.a * last;
+            else if (MODE == kDiluBwd) y = S.a;
+            else y = last;
+            // the three terms in slot order, which is the face order.  In the
+            // sweeps a missing neighbour has a zero coefficient and a finite
+            // value (exchanged values of invalid rows are zero), so its term
+            // is an exact zero; the factor divides and selects instead.
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const T v = q == kSlotA ? vA : q == kSlotB ? vB : vC;
+                const T c = slotVal(q);
+                if (MODE == kDiluFactor) {
+                    const bool has = q == kSlotA ? hasA : q == kSlotB ? hasB : hasC;
+                    y = has ? y - c / v : y;
+                } else {
+                    y = y - last * c * v;
+                }
+            }
+            const T yz = valid ? y : T(0);
+            // all lanes have read the previous step's values (LDS operations
+            // of a wavefront execute in order) before any overwrites them
+            xch[lane] = yz;
+            __builtin_amdgcn_wave_barrier();
+            yprev = yz;
+            if (inRange) {
+                const int s = s0 + t;
+                if (MODE == kDiluFactor) {
+                    const double rd = valid ? 1.0 / (double)y : 0.0;
+                    (V.rDout + (size_t)s * 64)[lane] = rd;
+                    constexpr int NP = Pack<double>::N;
+                    const int GP = packChunks(K + 1, NP);
+                    ((double *)&V.fPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                    ((double *)&V.bPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                } else {
+                    (out + (size_t)s * 64)[lane] = yz;
+                }
+            }
+            if (valid && sendB)
+                __hip_atomic_store(&(inbox + (ptrdiff_t)(t + dB) * TBC)[outB], Bits<T>::to(y), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (valid && sendC)
+                __hip_atomic_store(&(inbox + (ptrdiff_t)(t + dC) * TBC)[outC], Bits<T>::to(y), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            // the inbox values of a later step first, so that waiting for them
+            // leaves the record loads issued after them in flight
+            loadIn(i + kBoxInAhead, hbuf[(u + kBoxInAhead) % kBoxAhead], cbuf[(u + kBoxInAhead) % kBoxAhead]);
+            load(i + kBoxAhead, S);
+            // keep each step's prefetches where they are (not sunk towards
+            // their use)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+template <typename T, int KT, int MODE>
+__global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *st, const T *__restrict__ in, T *out,
+                                                 typename Bits<T>::U *inbox, typename Bits<T>::U *inboxClear,
+                                                 int nClear)
+{
+    __shared__ T xch[64];
+    const int w = V.boxOrder[MODE == kDiluBwd ? V.nBoxWaves - 1 - blockIdx.x : blockIdx.x];
+    const int32_t *m = V.boxMeta + (size_t)w * kBoxMeta;
+    const bool active = MODE == kDiluFactor || st[m[kBmSys]].status == kRunning;
+    const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
+    unsigned long long nSlow = 0;
+    if (active && m[kBmSteps] > 0) {
+        switch (m[kBmAxis]) {
+        case 0: boxSweep<T, KT, MODE, 0>(V, m, in, out, inbox, xch, nSlow); break;
+        case 1: boxSweep<T, KT, MODE, 1>(V, m, in, out, inbox, xch, nSlow); break;
+        default: boxSweep<T, KT, MODE, 2>(V, m, in, out, inbox, xch, nSlow); break;
+        }
+    }
+    if (V.trace && threadIdx.x == 0) {
+        unsigned long long *tr = V.trace + 4 * ((size_t)V.nWaves + blockIdx.x);
+        tr[0] = tStart;
+        tr[1] = wall_clock64();
+        tr[2] = nSlow;
+        tr[3] = (unsigned long long)m[kBmSteps];
+    }
+    if (inboxClear)
+        for (int q = blockIdx.x * 64 + threadIdx.x; q < nClear; q += gridDim.x * 64) inboxClear[q] = Bits<T>::kSent;
+}
+
 // rD of the FP32 records from the FP64 factor (one thread per row)
 template <int K>
 __global__ void k_convert_rd(DiluView<float> V, const double *rD64, int nPos)
@@ -271,6 +509,7 @@ template <typename T, int MODE>
 static hipError_t launchK(const DiluView<T> &V, const SysState *st, const T *in, T *out,
                           typename Bits<T>::U *use, typename Bits<T>::U *clr, int nClr, hipStream_t s)
 {
+    if (V.nWaves <= 0) return hipSuccess;
     const int mask = effectiveHaloMask(V.K, MODE == kDiluBwd ? V.haloMaskB : V.haloMaskF);
     switch (V.K) {
     case 3:
@@ -284,18 +523,33 @@ static hipError_t launchK(const DiluView<T> &V, const SysState *st, const T *in,
     return hipGetLastError();
 }
 
+template <typename T, int MODE>
+static hipError_t launchBox(const DiluView<T> &V, const SysState *st, const T *in, T *out,
+                            typename Bits<T>::U *use, typename Bits<T>::U *clr, int nClr, hipStream_t s)
+{
+    if (V.nBoxWaves <= 0) return hipSuccess;
+    const int n = V.nBoxWaves;
+    if (V.K == 3) k_dilu_box<T, 3, MODE><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+    else k_dilu_box<T, 0, MODE><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t diluForward(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s)
 {
     using U = typename Bits<T>::U;
-    return launchK<T, kDiluFwd>(V, st, in, out, (U *)V.haloF, (U *)V.haloB, V.nHaloB, s);
+    hipError_t e = launchK<T, kDiluFwd>(V, st, in, out, (U *)V.haloF, (U *)V.haloB, V.nHaloB, s);
+    if (e != hipSuccess) return e;
+    return launchBox<T, kDiluFwd>(V, st, in, out, (U *)V.boxInF, (U *)V.boxInB, V.nBoxIn, s);
 }
 
 template <typename T>
 hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipStream_t s)
 {
     using U = typename Bits<T>::U;
-    return launchK<T, kDiluBwd>(V, st, inout, inout, (U *)V.haloB, (U *)V.haloF, V.nHaloF, s);
+    hipError_t e = launchK<T, kDiluBwd>(V, st, inout, inout, (U *)V.haloB, (U *)V.haloF, V.nHaloF, s);
+    if (e != hipSuccess) return e;
+    return launchBox<T, kDiluBwd>(V, st, inout, inout, (U *)V.boxInB, (U *)V.boxInF, V.nBoxIn, s);
 }
 
 __global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v)
@@ -310,9 +564,14 @@ __global__ void k_fill_u32(unsigned int *a, int n, unsigned int v)
 
 hipError_t diluFactor(const DiluView<double> &V, hipStream_t s)
 {
-    // the factor publishes d through its own halo buffer, reset first
-    k_fill_u64<<<256, 256, 0, s>>>((unsigned long long *)V.haloC, V.nHaloF > 0 ? V.nHaloF : 1, kHaloSentinel64);
-    return launchK<double, kDiluFactor>(V, nullptr, nullptr, nullptr, (unsigned long long *)V.haloC, nullptr, 0, s);
+    // the factor publishes d through its own inbox buffer, reset first
+    const int n = V.nHaloF + V.nBoxIn;
+    k_fill_u64<<<256, 256, 0, s>>>((unsigned long long *)V.haloC, n > 0 ? n : 1, kHaloSentinel64);
+    hipError_t e = launchK<double, kDiluFactor>(V, nullptr, nullptr, nullptr, (unsigned long long *)V.haloC,
+                                                nullptr, 0, s);
+    if (e != hipSuccess) return e;
+    return launchBox<double, kDiluFactor>(V, nullptr, nullptr, nullptr, (unsigned long long *)V.boxInC, nullptr, 0,
+                                          s);
 }
 
 hipError_t diluConvertRD(const DiluView<float> &V32, const double *rD64, int nSlices, hipStream_t s)

@@ -53,7 +53,7 @@ inline int effectiveHaloMask(int K, int mask) { return (K == 3 && (mask & ~3) ==
 
 template <typename T>
 struct DiluView {
-    const int32_t *order = nullptr;      // waves in forward dispatch order
+    const int32_t *order = nullptr;      // general (code-driven) waves in forward dispatch order
     const int32_t *waveSlice = nullptr, *waveSteps = nullptr, *waveSys = nullptr;
     int nWaves = 0, K = 3;
     const int4 *fCodes = nullptr, *bCodes = nullptr;
@@ -65,8 +65,17 @@ struct DiluView {
     void *haloF = nullptr, *haloB = nullptr, *haloC = nullptr;   // inboxes
     int nHaloF = 0, nHaloB = 0;          // inbox sizes (values)
     int *err = nullptr;                  // bit 2: a halo wait expired
-    // optional trace: per wave (dispatch index) start / end realtime clock
-    // ticks, slow-path hits and steps (4 entries each)
+    // structured-box tile waves (dilu.hip, k_dilu_box): their own dispatch
+    // order, kBoxMeta ints of tile geometry per wave, and compact inboxes of
+    // (TB + TC) values per step per wave, one per direction
+    const int32_t *boxOrder = nullptr;
+    int nBoxWaves = 0;
+    const int32_t *boxMeta = nullptr;
+    void *boxInF = nullptr, *boxInB = nullptr, *boxInC = nullptr;
+    int nBoxIn = 0;
+    // optional trace: per wave start / end realtime clock ticks, slow-path
+    // hits and steps (4 entries each); general waves by dispatch index, then
+    // box waves by dispatch index
     unsigned long long *trace = nullptr;
 };
 

@@ -20,31 +20,57 @@ constexpr int kBlock = 256;
 constexpr double kGreat = 1e20;   // lduMatrix::great_
 constexpr double kSmall = 1e-20;  // lduMatrix::small_
 
-__device__ inline double waveSum(double v)
+// Reductions carry double-double sums (hi + lo, built from the error-free
+// TwoSum), so a reduced value is the sum of its terms to about 2^-100
+// relative and its rounding to double does not depend on the order of
+// summation: every system reduces to the same scalars alone or in a batch,
+// and the same as a sequential compensated sum over its cells.
+struct DD {
+    double hi, lo;
+};
+
+__device__ inline DD twoSum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+
+__device__ inline DD ddAdd(DD x, DD y)
+{
+    const DD s = twoSum(x.hi, y.hi);
+    const double e = s.lo + (x.lo + y.lo);
+    const double hi = s.hi + e;
+    return DD{hi, e - (hi - s.hi)};
+}
+
+__device__ inline DD waveSum(DD v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = 32; o > 0; o >>= 1) v = ddAdd(v, DD{__shfl_xor(v.hi, o, 64), __shfl_xor(v.lo, o, 64)});
     return v;
 }
 
-// Block-reduce NV values and store the block's partial sums (slot j of
-// partials[blockIdx.x * 4 + j]).  No device-scope fence is needed: the
+// Block-reduce NV values and store the block's partial sums (slot j as the
+// double-double pair partials[(blockIdx.x * 4 + j) * 2 + {0, 1}]).  No device-scope fence is needed: the
 // partials are read by the finalise kernel launched after this one.
 template <int NV>
 __device__ void blockPartial(double (&v)[NV], double *partials)
 {
-    __shared__ double sh[kBlock / 64][NV];
+    __shared__ DD sh[kBlock / 64][NV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    DD d[NV];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) v[j] = waveSum(v[j]);
+    for (int j = 0; j < NV; ++j) d[j] = waveSum(DD{v[j], 0.0});
     if (lane == 0)
-        for (int j = 0; j < NV; ++j) sh[wid][j] = v[j];
+        for (int j = 0; j < NV; ++j) sh[wid][j] = d[j];
     __syncthreads();
     if (threadIdx.x == 0)
         for (int j = 0; j < NV; ++j) {
-            double s = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) s += sh[w][j];
-            partials[(size_t)blockIdx.x * 4 + j] = s;
+            DD t = sh[0][j];
+            for (int w = 1; w < kBlock / 64; ++w) t = ddAdd(t, sh[w][j]);
+            partials[((size_t)blockIdx.x * 4 + j) * 2] = t.hi;
+            partials[((size_t)blockIdx.x * 4 + j) * 2 + 1] = t.lo;
         }
 }
 
@@ -135,11 +161,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int stage, const SysInfo *i
     const int q = blockIdx.x;
     if (st[q].status != kRunning) return;
     const int b0 = info[q].blkBegin, b1 = b0 + info[q].nBlk;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    DD acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     for (int b = b0 + threadIdx.x; b < b1; b += kBlock)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] += partials[(size_t)b * 4 + j];
-    __shared__ double sh[kBlock / 64][4];
+        for (int j = 0; j < 4; ++j)
+            acc[j] = ddAdd(acc[j], DD{partials[((size_t)b * 4 + j) * 2], partials[((size_t)b * 4 + j) * 2 + 1]});
+    __shared__ DD sh[kBlock / 64][4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = waveSum(acc[j]);
@@ -149,9 +176,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int stage, const SysInfo *i
     if (threadIdx.x == 0) {
         double out[4];
         for (int j = 0; j < 4; ++j) {
-            double s = 0.0;
-            for (int w = 0; w < kBlock / 64; ++w) s += sh[w][j];
-            out[j] = s;
+            DD t = sh[0][j];
+            for (int w = 1; w < kBlock / 64; ++w) t = ddAdd(t, sh[w][j]);
+            out[j] = t.hi + t.lo;
         }
         SysState S = st[q];
         applyStage(stage, S, out);

@@ -306,7 +306,7 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
             m[kBmPrevC] = tw(w.tb, w.tc - 1);
             m[kBmNextB] = tw(w.tb + 1, w.tc);
             m[kBmNextC] = tw(w.tb, w.tc + 1);
-            m[kBmPerm] = (2 - rs.seqAxis) | ((2 - b) << 2) | ((2 - cc) << 4);
+            m[kBmAxis] = rs.seqAxis;
             m[kBmActB] = w.actB;
             m[kBmActC] = w.actC;
         }
@@ -391,9 +391,6 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
     L.bVal.assign(nDep, 0.0);
     L.fHalo.assign(L.nPos, -1);
     L.bHalo.assign(L.nPos, -1);
-    L.boxF.assign((size_t)L.nSlices * 3 * kSlice, 0.0);
-    L.boxB.assign((size_t)L.nSlices * 3 * kSlice, 0.0);
-    L.boxP.assign((size_t)L.nSlices * 3 * kSlice, 0.0);
 
     auto waveStep = [&](int p, int &w, int &t, int &lane) {
         int s = p / kSlice;
@@ -430,61 +427,63 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
             waveStep(p, w, t, lane);
             const int T = L.waveSteps[w];
             if (L.waveBox[w]) {
-                // fixed slots by axis of the neighbour: slot 2 - axis
+                // box tile rows: fixed slots by the axis of the neighbour
+                // (slot 2 - axis), zero where there is none; the dependencies
+                // themselves follow from the tile geometry
                 const RegionSchedule &rs = L.schedules[r];
-                const int o = L.regionOffset[r];
                 const int sxy = rs.n[0] * rs.n[1];
                 auto slotOf = [&](int d) { return d == 1 ? 2 : d == rs.n[0] ? 1 : d == sxy ? 0 : -1; };
                 for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q) {
-                    int sl = slotOf((c - o) - (D.lo_nb[q] - o));
+                    int sl = slotOf(c - D.lo_nb[q]);
                     if (sl < 0) throw std::runtime_error("box region with an unexpected lower neighbour");
                     int f = D.lo_face[q];
-                    size_t e = ((size_t)s * 3 + sl) * kSlice + l;
-                    L.boxF[e] = lo[f];
-                    L.boxP[e] = R.upper[f] * lo[f];
+                    size_t e = ((size_t)s * K + sl) * kSlice + l;
+                    L.fVal[e] = lo[f];
+                    L.fProd[e] = R.upper[f] * lo[f];
                 }
                 for (int q = D.up_off[c]; q < D.up_off[c + 1]; ++q) {
-                    int sl = slotOf((D.up_nb[q] - o) - (c - o));
+                    int sl = slotOf(D.up_nb[q] - c);
                     if (sl < 0) throw std::runtime_error("box region with an unexpected upper neighbour");
-                    L.boxB[((size_t)s * 3 + sl) * kSlice + l] = R.upper[D.up_face[q]];
+                    L.bVal[((size_t)s * K + sl) * kSlice + l] = R.upper[D.up_face[q]];
                 }
-            }
-            // forward sweep: lower neighbours, ascending face order
-            k = 0;
-            for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q, ++k) {
-                int f = D.lo_face[q];
-                int pd = L.posOfCell[D.lo_nb[q]];
-                int wd, td, ld;
-                waveStep(pd, wd, td, ld);
-                size_t e = ((size_t)s * K + k) * kSlice + l;
-                L.fVal[e] = lo[f];
-                L.fProd[e] = R.upper[f] * lo[f];
-                if (wd == w && t - td >= 1 && t - td <= kRing - 1) {
-                    L.fCode[e] = -((td % kRing) * kSlice + ld + 1);
-                } else {
-                    if (wd == w ? td >= t : wd > w)
-                        throw std::runtime_error("DILU schedule violates a forward dependency");
-                    if (L.fHalo[pd] < 0) L.fHalo[pd] = L.nHaloF++;
-                    L.fCode[e] = L.fHalo[pd];
+            } else {
+                // forward sweep: lower neighbours, ascending face order
+                k = 0;
+                for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q, ++k) {
+                    int f = D.lo_face[q];
+                    int pd = L.posOfCell[D.lo_nb[q]];
+                    int wd, td, ld;
+                    waveStep(pd, wd, td, ld);
+                    size_t e = ((size_t)s * K + k) * kSlice + l;
+                    L.fVal[e] = lo[f];
+                    L.fProd[e] = R.upper[f] * lo[f];
+                    if (wd == w && t - td >= 1 && t - td <= kRing - 1) {
+                        L.fCode[e] = -((td % kRing) * kSlice + ld + 1);
+                    } else {
+                        if (wd == w ? td >= t : wd > w)
+                            throw std::runtime_error("DILU schedule violates a forward dependency");
+                        if (L.fHalo[pd] < 0) L.fHalo[pd] = L.nHaloF++;
+                        L.fCode[e] = L.fHalo[pd];
+                    }
                 }
-            }
-            // backward sweep: upper neighbours, reverse face order; processing
-            // index i = T-1-t, so the ring slot of a dependency uses its own i
-            k = 0;
-            for (int q = D.up_off[c + 1] - 1; q >= D.up_off[c]; --q, ++k) {
-                int pd = L.posOfCell[D.up_nb[q]];
-                int wd, td, ld;
-                waveStep(pd, wd, td, ld);
-                size_t e = ((size_t)s * K + k) * kSlice + l;
-                L.bVal[e] = R.upper[D.up_face[q]];
-                if (wd == w && td - t >= 1 && td - t <= kRing - 1) {
-                    int id = T - 1 - td;
-                    L.bCode[e] = -((id % kRing) * kSlice + ld + 1);
-                } else {
-                    if (wd == w ? td <= t : wd < w)
-                        throw std::runtime_error("DILU schedule violates a backward dependency");
-                    if (L.bHalo[pd] < 0) L.bHalo[pd] = L.nHaloB++;
-                    L.bCode[e] = L.bHalo[pd];
+                // backward sweep: upper neighbours, reverse face order; processing
+                // index i = T-1-t, so the ring slot of a dependency uses its own i
+                k = 0;
+                for (int q = D.up_off[c + 1] - 1; q >= D.up_off[c]; --q, ++k) {
+                    int pd = L.posOfCell[D.up_nb[q]];
+                    int wd, td, ld;
+                    waveStep(pd, wd, td, ld);
+                    size_t e = ((size_t)s * K + k) * kSlice + l;
+                    L.bVal[e] = R.upper[D.up_face[q]];
+                    if (wd == w && td - t >= 1 && td - t <= kRing - 1) {
+                        int id = T - 1 - td;
+                        L.bCode[e] = -((id % kRing) * kSlice + ld + 1);
+                    } else {
+                        if (wd == w ? td <= t : wd < w)
+                            throw std::runtime_error("DILU schedule violates a backward dependency");
+                        if (L.bHalo[pd] < 0) L.bHalo[pd] = L.nHaloB++;
+                        L.bCode[e] = L.bHalo[pd];
+                    }
                 }
             }
             // interface faces of this row

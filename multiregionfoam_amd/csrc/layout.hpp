@@ -54,7 +54,7 @@ enum {
     kBmPrevC,
     kBmNextB,
     kBmNextC,
-    kBmPerm,        // record slots of the sequential / b / c neighbour: bits 0-1, 2-3, 4-5
+    kBmAxis,        // sequential axis (0 x, 1 y, 2 z); b, c are the other two, ascending
     kBmActB,        // active lanes along b / c in this tile
     kBmActC,
     kBmSlice,       // first slice (filled by the batch)
@@ -98,12 +98,13 @@ struct SystemLayout {
     std::vector<double> fVal, fProd, bVal;
     std::vector<int32_t> fHalo, bHalo;       // per position: halo slot published or -1
     int nHaloF = 0, nHaloB = 0;
-    // structured-box waves: fixed-slot records [slice][slot][lane], slot =
-    // 2 - axis of the neighbour (forward lower coefficients, backward upper
-    // coefficients, factor upper*lower products; zero where none)
+    // structured-box tile waves: their rows keep the DILU values in fixed
+    // slots (slot = 2 - axis of the neighbour: z, y, x, which is both the
+    // face order of the lower neighbours and the reverse face order of the
+    // upper ones; zero where a neighbour is missing) and have no codes: the
+    // dependencies follow from the tile geometry in boxMeta
     std::vector<uint8_t> waveBox;            // per wave: 1 if a box tile
     std::vector<int32_t> boxMeta;            // kBoxMeta per wave
-    std::vector<double> boxF, boxB, boxP;
 };
 
 // Build the layout.  `preferLevels` forces the level schedule for every

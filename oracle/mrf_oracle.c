@@ -602,26 +602,46 @@ void orc_sumA(const orc_system *sys, double **sumA)
     }
 }
 
+/* Compensated summation (TwoSum error terms accumulated beside the sum).
+   A global sum (gSum / gSumProd / gSumMag) is then the sum of its terms to
+   about 2^-100 relative, so its rounding to double does not depend on the
+   order in which the terms are added: a sequential restatement and a
+   parallel reduction of the same terms agree.  Plain sequential summation
+   differs from either by rounding only. */
+typedef struct {
+    double hi, lo;
+} csum;
+
+static inline void csum_add(csum *a, double x)
+{
+    double s = a->hi + x;
+    double bb = s - a->hi;
+    a->lo += (a->hi - (s - bb)) + (x - bb);
+    a->hi = s;
+}
+
+static inline double csum_val(csum a) { return a.hi + a.lo; }
+
 /* lduMatrix::solver::normFactor: xRef = gAverage(x); tmp = sumA*xRef;
    sum(|Ax - tmp| + |b - tmp|) + small (1e-20). */
 double orc_norm_factor(const orc_system *sys, double *const *x, double *const *b,
                        double *const *Ax, double **tmp)
 {
-    double sx = 0.0;
+    csum sx = {0.0, 0.0};
     long n = 0;
     for (int r = 0; r < sys->nRegions; ++r) {
-        for (int c = 0; c < sys->regions[r].nCells; ++c) sx += x[r][c];
+        for (int c = 0; c < sys->regions[r].nCells; ++c) csum_add(&sx, x[r][c]);
         n += sys->regions[r].nCells;
     }
-    double xRef = sx / (double)n;
+    double xRef = csum_val(sx) / (double)n;
     orc_sumA(sys, tmp);
-    double nf = 0.0;
+    csum nf = {0.0, 0.0};
     for (int r = 0; r < sys->nRegions; ++r)
         for (int c = 0; c < sys->regions[r].nCells; ++c) {
             double t = tmp[r][c] * xRef;
-            nf += fabs(Ax[r][c] - t) + fabs(b[r][c] - t);
+            csum_add(&nf, fabs(Ax[r][c] - t) + fabs(b[r][c] - t));
         }
-    return nf + 1e-20;
+    return csum_val(nf) + 1e-20;
 }
 
 /* coupled Cholesky preconditioner, calcPreconDiag: per region, in face
@@ -742,18 +762,18 @@ static void ff_free(ffield *f)
 
 static double ff_sumprod(const ffield *a, const ffield *b)
 {
-    double s = 0.0;
+    csum s = {0.0, 0.0};
     for (int r = 0; r < a->nR; ++r)
-        for (int c = 0; c < a->n[r]; ++c) s += a->v[r][c] * b->v[r][c];
-    return s;
+        for (int c = 0; c < a->n[r]; ++c) csum_add(&s, a->v[r][c] * b->v[r][c]);
+    return csum_val(s);
 }
 
 static double ff_summag(const ffield *a)
 {
-    double s = 0.0;
+    csum s = {0.0, 0.0};
     for (int r = 0; r < a->nR; ++r)
-        for (int c = 0; c < a->n[r]; ++c) s += fabs(a->v[r][c]);
-    return s;
+        for (int c = 0; c < a->n[r]; ++c) csum_add(&s, fabs(a->v[r][c]));
+    return csum_val(s);
 }
 
 /* lduSolver::stop + SolverPerformance::checkConvergence */

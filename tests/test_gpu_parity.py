@@ -1,9 +1,12 @@
 """GPU parity of the HIP path against the CPU oracle, through the C ABI.
 
 Kernel-level checks are bit-exact (FP contraction is off in both builds and
-each row is evaluated in the face-loop order of the oracle).  Solves are
-compared on converged fields: relative L2 <= 1e-10 to the oracle's field
-with the same iteration count +-2 (FP64), <= 1e-8 for mixed precision.
+each row is evaluated in the face-loop order of the oracle).  FP64 solves
+are bit-identical to the oracle as well: global sums are compensated on both
+sides, so their rounding does not depend on the summation order, and every
+other operation is evaluated in the oracle's order.  The acceptance bar for
+FP64 (relative L2 <= 1e-10, iterations +-2) is therefore met with equality;
+mixed precision is held to relative L2 <= 1e-8 against the oracle's field.
 """
 import numpy as np
 import pytest
@@ -23,6 +26,10 @@ def small_cases():
         "duct_ggi": cases.duct3d(nf=(24, 12, 12), ns=(30, 15, 5)),
         "duct_conformal": cases.duct3d(nf=(20, 10, 8), ns=(20, 10, 4), conformal=True),
         "exchanger3": cases.exchanger3(n=(20, 8, 6)),
+        # several tiles per region, partial edge tiles, all three sequential
+        # axes (fluid along x / z, solid along x / y)
+        "duct_tiles": cases.duct3d(nf=(37, 23, 29), ns=(41, 27, 7)),
+        "duct_zlong": cases.duct3d(nf=(12, 14, 40), ns=(15, 18, 5)),
     }
 
 
@@ -75,9 +82,13 @@ def test_bicgstab_fp64_matches_oracle(gpu, name, schedule):
     perf = A.solve(solver="BiCGStab", tolerance=1e-10, maxIter=1000)[0]
     assert perf.converged and perf_ref["converged"]
     assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
-    assert perf.initial_residual == pytest.approx(perf_ref["initialResidual"], rel=1e-12)
     err = rel(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
     assert err <= FIELD_TOL, err
+    # and in fact identical
+    assert perf.n_iterations == perf_ref["nIterations"]
+    assert perf.initial_residual == perf_ref["initialResidual"]
+    assert perf.final_residual == perf_ref["finalResidual"]
+    assert np.array_equal(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
 
 
 def test_pcg_symmetric(gpu):
@@ -94,9 +105,30 @@ def test_pcg_symmetric(gpu):
     assert perf.converged and perf_ref["converged"]
     assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
     assert rel(np.concatenate(A.psi(0)), np.concatenate(psi_ref)) <= FIELD_TOL
+    assert perf.n_iterations == perf_ref["nIterations"]
+    assert np.array_equal(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
 
 
-@pytest.mark.parametrize("name", ["plate2d", "duct_ggi", "exchanger3"])
+@pytest.mark.parametrize("name", ["duct_tiles", "duct_zlong", "exchanger3"])
+def test_dilu_fp32_records(gpu, name):
+    """The FP32 preconditioner (records rounded from FP64, rD from the FP64
+    factor) stays within FP32 rounding of the FP64 oracle, also when run
+    again after other kernels have used the device."""
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    A = _ctx(gpu, case)
+    rD_ref = O.dilu_factor()
+    rng = np.random.default_rng(3)
+    for rep in range(2):
+        r = [rng.standard_normal(n["nCells"]) for n in O.regions]
+        w_ref = np.concatenate(O.dilu_precondition(rD_ref, r))
+        w32 = A.precondition_fp32(np.concatenate(r).astype(np.float32)).astype(np.float64)
+        assert np.all(np.isfinite(w32))
+        assert rel(w32, w_ref) < 1e-5, rel(w32, w_ref)
+        A.solve(tolerance=1e-8, maxIter=200)
+
+
+@pytest.mark.parametrize("name", ["plate2d", "duct_ggi", "exchanger3", "duct_tiles", "duct_zlong"])
 def test_mixed_precision_refinement(gpu, name):
     case = small_cases()[name]
     O = ob.OracleSystem(case)
@@ -153,19 +185,20 @@ def test_fvsolution_dictionary_selects_solver(gpu):
     assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
 
 
-def test_full_size_duct_residual(gpu):
-    """1.4M-cell GGI duct: SpMV bit-exact at full size and the GPU's
-    converged field has a true (oracle-evaluated) residual below tolerance."""
+def test_full_size_duct_matches_oracle(gpu):
+    """1.4M-cell GGI duct (the benchmark workload): SpMV bit-exact at full
+    size, and the FP64 solve identical to the oracle's (iterations, residuals
+    and field)."""
     case = cases.duct3d()
     O = ob.OracleSystem(case)
     A = _ctx(gpu, case)
     x = [r["psi"] + np.linspace(0.0, 1.0, r["nCells"]) for r in O.regions]
     assert np.array_equal(A.spmv(x), np.concatenate(O.amul(x)))
+    psi_ref, perf_ref = O.solve(tol=1e-10, maxIter=1000)
     perf = A.solve(tolerance=1e-10, maxIter=1000)[0]
-    assert perf.converged
-    psi = A.psi(0)
-    b = [r["source"] for r in O.regions]
-    Ax = O.amul(psi)
-    nf = O.norm_factor([r["psi"] for r in O.regions], b, O.amul([r["psi"] for r in O.regions]))
-    res = sum(np.abs(bb - aa).sum() for bb, aa in zip(b, Ax)) / nf
-    assert res < 1e-10 * 1.5, res
+    assert perf.converged and perf_ref["converged"]
+    assert perf.n_iterations == perf_ref["nIterations"]
+    assert perf.final_residual == perf_ref["finalResidual"]
+    psi = np.concatenate(A.psi(0))
+    assert rel(psi, np.concatenate(psi_ref)) <= FIELD_TOL
+    assert np.array_equal(psi, np.concatenate(psi_ref))
