@@ -51,6 +51,17 @@ def run_steps(A, ctrl, warmup, steps):
     return total, lat, perf
 
 
+def kernel_probe(A, reps):
+    """Replay each hot kernel `reps` times, then copy 256 MiB device to device
+    `reps` times (reads 256 MiB and writes 256 MiB per dispatch)."""
+    for which, name in ((0, "spmv"), (1, "dilu_fwd"), (2, "dilu_bwd"), (3, "dilu_factor")):
+        ms = A.time_kernel(which, reps)
+        print("probe %s %.4f ms" % (name, ms), file=sys.stderr, flush=True)
+    ms = A.copy_probe(256 * 2**20, reps)
+    print("probe copy 268435456 bytes read + written per dispatch: %.4f ms (%.0f GB/s)"
+          % (ms, 2 * 256 * 2**20 / (ms * 1e-3) / 1e9), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +70,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
+                         "device copy of known size (for rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE "
+                         "per dispatch, the copy giving the counter's scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus %d is not run "
@@ -73,6 +88,9 @@ def main():
     A = m.CoupledFvMatrix(0)
     A.add_case(case)
     A.commit()
+    if args.kernel_probe:
+        kernel_probe(A, args.kernel_reps)
+        return
     info = A.layout_info(0)
     bm64 = A.bytes_model(0, 8)
     bm32 = A.bytes_model(0, 4)
@@ -106,8 +124,19 @@ def main():
     launch_ms = {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd}[dominant]
     launch_bytes = bm64["spmv"] if dominant == "spmv" else 0.5 * bm64["dilu"]
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    kernel_names = {"spmv": "k_spmv_plain<double>", "dilu_fwd": "k_dilu_wave<double, 3, 0>",
-                    "dilu_bwd": "k_dilu_wave<double, 3, 1>"}
+    kernel_names = {"spmv": "k_spmv_plain<double>", "dilu_fwd": "k_dilu_box<double, 3, 0>",
+                    "dilu_bwd": "k_dilu_box<double, 3, 1>"}
+
+    # HBM bytes per launch of that kernel, from the committed rocprofv3 PMC
+    # summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain
+    # copy kernel); None when no summary covers the kernel
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "duct3d_hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tk = json.load(f).get("kernels", {})
+        if kernel_names[dominant] in tk:
+            traffic = tk[kernel_names[dominant]]["total_bytes"]
 
     # ---- CPU baseline: the oracle (C restatement), one full FP64 solve ----
     cpu = None
@@ -157,7 +186,7 @@ def main():
         "kernels_ms": launch_ms and {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd},
         "roofline": {"bound": "hbm", "kernel": kernel_names[dominant], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": launch_bytes},
+                     "traffic": traffic, "bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

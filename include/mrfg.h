@@ -190,6 +190,11 @@ int mrfg_trace_dilu(mrfg_ctx *ctx, long long *out, int max_waves, int *n_waves);
 int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info);
 /* Canonical algorithmic bytes at value size 8 (fp64) or 4 (fp32). */
 int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b_spmv, double *b_dilu);
+/* Average time (ms) of a plain device-to-device copy of `bytes` bytes over
+   `reps` launches (each reads and writes `bytes`): the reference kernel for
+   scaling HBM traffic counters (profiling facility; no counterpart in the
+   reference). */
+int mrfg_copy_probe(mrfg_ctx *ctx, long long bytes, int reps, double *ms);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ def lib():
         "mrfg_layout_info": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "mrfg_trace_dilu": ([vp, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)], C.c_int),
         "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
+        "mrfg_copy_probe": ([vp, C.c_longlong, C.c_int, F64P], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -117,7 +118,7 @@ EXPORTED = (
     "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes").split()
+    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe").split()
 
 
 def _p(a, t):
@@ -379,6 +380,12 @@ class CoupledFvMatrix:
         w = np.zeros_like(r)
         self._check(lib().mrfg_precondition_fp32(self._h, sys, _p(r, C.c_float), _p(w, C.c_float)))
         return w
+
+    def copy_probe(self, nbytes, reps=10):
+        """Average ms of a plain device copy of nbytes (reads + writes nbytes)."""
+        ms = C.c_double()
+        self._check(lib().mrfg_copy_probe(self._h, int(nbytes), reps, C.byref(ms)))
+        return ms.value
 
     def time_kernel(self, which, reps=20):
         ms = C.c_double()

@@ -489,6 +489,15 @@ int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info)
     });
 }
 
+int mrfg_copy_probe(mrfg_ctx *ctx, long long bytes, int reps, double *ms)
+{
+    if (!ctx || !ms || bytes < 16 || reps < 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        hipError_t e = mrfg::copyProbe(ctx->device, (size_t)bytes, reps, ms);
+        if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e));
+    });
+}
+
 int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b_spmv, double *b_dilu)
 {
     if (!ctx || (value_size != 4 && value_size != 8)) return MRFG_ERR_ARG;

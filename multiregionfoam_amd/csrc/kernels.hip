@@ -586,6 +586,44 @@ hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K3
     MRFG_CHECK_LAUNCH();
 }
 
+// Plain device copy (16 bytes per thread per iteration, grid-stride): the
+// reference point of the HBM traffic counters.
+__global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t copyProbe(int device, size_t bytes, int reps, double *ms)
+{
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    const size_t n = bytes / sizeof(uint4);
+    uint4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((e = hipMalloc(&a, n * sizeof(uint4))) != hipSuccess) return e;
+    if ((e = hipMalloc(&b, n * sizeof(uint4))) != hipSuccess) {
+        (void)hipFree(a);
+        return e;
+    }
+    (void)hipMemset(a, 0, n * sizeof(uint4));
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    k_copy_probe<<<2048, 256>>>(a, b, n);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) k_copy_probe<<<2048, 256>>>(a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    *ms = (double)t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return e;
+}
+
 #define MRFG_INSTANTIATE(T)                                                                                          \
     template hipError_t launchSpmvPlain<T>(const MatView<T> &, const BlockEntry *, int, const T *, T *, hipStream_t); \
     template hipError_t launchSetup<T>(const MatView<T> &, const Krylov<T> &, bool, hipStream_t);                    \

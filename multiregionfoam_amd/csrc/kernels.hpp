@@ -36,6 +36,10 @@ template <typename T>
 hipError_t launchPcgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st);
 
 
+// Average time (ms) of a plain device-to-device copy of `bytes` (reads and
+// writes `bytes` per launch), for calibrating HBM traffic counters.
+hipError_t copyProbe(int device, size_t bytes, int reps, double *ms);
+
 hipError_t launchRefinePrepare(int nSys, const SysState *outer, SysState *inner, double innerRelTol,
                                int innerMaxIter, hipStream_t st);
 hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &K64, hipStream_t st);
