@@ -36,19 +36,21 @@ def pct(v, q):
     return v[f] + (v[c] - v[f]) * (k - f)
 
 
-def run_steps(A, ctrl, warmup, steps):
+def run_steps(A, ctrl, warmup, steps, all_perf=False, verbose=False):
     for _ in range(warmup):
         A.reset_psi()
         A.solve(ctrl)
     lat, perf = [], None
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         s0 = time.perf_counter()
         A.reset_psi()
-        perf = A.solve(ctrl)[0]
+        perf = A.solve(ctrl)
         lat.append(time.perf_counter() - s0)
+        if verbose:
+            print("  step %d/%d: %.1f ms" % (k + 1, steps, 1e3 * lat[-1]), file=sys.stderr, flush=True)
     total = time.perf_counter() - t0
-    return total, lat, perf
+    return total, lat, (perf if all_perf else perf[0])
 
 
 def kernel_probe(A, reps):
@@ -62,6 +64,135 @@ def kernel_probe(A, reps):
           % (ms, 2 * 256 * 2**20 / (ms * 1e-3) / 1e9), file=sys.stderr, flush=True)
 
 
+def progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_exchanger(m, cases, args):
+    """Three-region counter-flow exchanger, 3 x 2.4M cells, two GGI interfaces:
+    mixed FP32/FP64 refinement against FP64, with the device FP64 heat-flux
+    conservation check of both interfaces after each solve."""
+    import numpy as np
+    progress("exchanger3: building 3 x 200x120x100 cells")
+    A = m.CoupledFvMatrix(0)
+    A.add_case(cases.exchanger3())
+    A.commit()
+    info = A.layout_info(0)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
+                          innerRelTol=1e-4)
+        total, lat, perf = run_steps(A, ctrl, args.warmup, args.steps, verbose=True)
+        flux = [A.interface_flux(i) for i in range(2)]
+        res[prec] = dict(solves_s=args.steps / total, ms_per_step=1e3 * total / args.steps,
+                         iterations=perf.n_iterations, outer=perf.n_outer, converged=perf.converged,
+                         final_residual=perf.final_residual, p50_ms=1e3 * pct(lat, 0.5),
+                         p99_ms=1e3 * pct(lat, 0.99),
+                         flux_imbalance=[abs(qa + qb) / abs(qa) for qa, qb in flux],
+                         flux_W=[qa for qa, _ in flux])
+        res[prec + "_psi"] = np.concatenate(A.psi(0))
+        progress("exchanger3 %s: %.1f ms/solve" % (prec, res[prec]["ms_per_step"]))
+    err = float(np.linalg.norm(res["mixed_psi"] - res["fp64_psi"]) / np.linalg.norm(res["fp64_psi"]))
+    out = {
+        "metric": "coupled solves/s (three-region exchanger, 7.2M cells); mixed vs FP64; interface-flux check",
+        "value": res["mixed"]["solves_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["mixed"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 (mixed refinement)",
+        "data": "synthetic (seeded case generator)",
+        "config": {"workload": "exchanger3: fluid A / wall / fluid B, 3 x 200x120x100, two GGI interfaces",
+                   "cells": info["cells"], "internal_faces": info["faces"], "ggi_pairs": info["pairs"]},
+        "mixed": res["mixed"], "fp64": res["fp64"], "mixed_speedup_vs_fp64":
+            res["mixed"]["solves_s"] / res["fp64"]["solves_s"],
+        "mixed_rel_l2_vs_fp64": err,
+    }
+    print(json.dumps(out))
+
+
+def run_batch(m, cases, args):
+    """64 independent two-region systems of seeded uneven size (cells per
+    region log-uniform in [10k, 500k], conductivity ratio log-uniform in
+    [1, 1000]), all solved by the same launches."""
+    import numpy as np
+    progress("batch64: building 64 systems")
+    cs = cases.batch(64)
+    A = m.CoupledFvMatrix(0)
+    for c in cs:
+        A.add_case(c)
+    A.commit()
+    cells = sum(A.layout_info(q)["cells"] for q in range(len(cs)))
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
+                          innerRelTol=1e-4)
+        total, lat, perfs = run_steps(A, ctrl, args.warmup, args.steps, all_perf=True, verbose=True)
+        res[prec] = dict(systems_s=len(cs) * args.steps / total, ms_per_batch=1e3 * total / args.steps,
+                         converged=sum(p.converged for p in perfs), max_iterations=max(p.n_iterations for p in perfs),
+                         max_outer=max(p.n_outer for p in perfs),
+                         p50_ms=1e3 * pct(lat, 0.5), p99_ms=1e3 * pct(lat, 0.99))
+        progress("batch64 %s: %.1f ms per batch" % (prec, res[prec]["ms_per_batch"]))
+    out = {
+        "metric": "coupled solves/s, 64 uneven systems per launch; mixed vs FP64",
+        "value": res["mixed"]["systems_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["mixed"]["ms_per_batch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 (mixed refinement)",
+        "data": "synthetic (seeded case generator)",
+        "config": {"workload": "batch64: 64 two-region box systems, 10k-500k cells per region", "systems": len(cs),
+                   "cells": cells},
+        "mixed": res["mixed"], "fp64": res["fp64"],
+        "mixed_speedup_vs_fp64": res["mixed"]["systems_s"] / res["fp64"]["systems_s"],
+    }
+    print(json.dumps(out))
+
+
+def run_serving(m, cases, args):
+    """Poisson arrivals at 50 req/s of two-region 3-D solves (20k-2M cells,
+    half GGI) submitted on their arrival times to the serving engine; latency
+    = completion - arrival, throughput = requests / (last completion - first
+    arrival); FP64 and mixed runs of the same stream."""
+    import numpy as np
+    stream = cases.serving_stream(args.requests)
+    texts = [(t, cases.to_dict_text(c)) for t, c in stream]
+    cells = [sum(int(np.prod(r["nCells"])) for r in c["regions"]) for _, c in stream]
+    res = {}
+    for prec in ("fp64", "mixed"):
+        srv = m.Server(device=0, hbm_budget=200 << 30, workers=8, max_batch=8, tolerance=1e-10,
+                       maxIter=3000, precision=prec, innerRelTol=1e-4)
+        # warm-up: one request through every worker
+        for _ in range(8):
+            srv.submit(texts[0][1])
+        srv.wait_all()
+        base = 8
+        t0 = time.perf_counter()
+        for t_arr, txt in texts:
+            dt = t0 + t_arr - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            srv.submit(txt)
+        srv.wait_all()
+        rs = [srv.result(base + i) for i in range(len(texts))]
+        lat = [r.latency for r in rs]
+        first = min(r.t_submit for r in rs)
+        last = max(r.t_done for r in rs)
+        res[prec] = dict(requests_s=len(rs) / (last - first), p50_ms=1e3 * pct(lat, 0.5),
+                         p99_ms=1e3 * pct(lat, 0.99), mean_batch=float(np.mean([r.batch_size for r in rs])),
+                         converged=sum(r.perf.converged for r in rs),
+                         mean_queue_ms=1e3 * float(np.mean([r.t_start - r.t_submit for r in rs])))
+        progress("serving %s: %.1f req/s, p50 %.0f ms, p99 %.0f ms" % (prec, res[prec]["requests_s"],
+                                                                       res[prec]["p50_ms"], res[prec]["p99_ms"]))
+        del srv
+    out = {
+        "metric": "serving: solves/s and p50/p99 latency at 50 req/s Poisson arrivals; mixed vs FP64",
+        "value": res["fp64"]["requests_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 8, "ms_per_step": 1e3 / res["fp64"]["requests_s"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded case generator)",
+        "config": {"workload": "serving: %d requests, Poisson 50 req/s, 20k-2M cells, 50%% GGI" % args.requests,
+                   "total_cells": int(sum(cells)), "arrival_span_s": texts[-1][0],
+                   "workers": 8, "max_batch": 8, "hbm_budget_GiB": 200},
+        "fp64": res["fp64"], "mixed": res["mixed"],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +201,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--workload", default="duct3d", choices=["duct3d", "exchanger3", "batch64", "serving"],
+                    help="duct3d (default, the headline line): the 1.404M-cell two-region GGI solve; "
+                         "exchanger3: three regions of 2.4M cells, two GGI interfaces, mixed precision "
+                         "with the FP64 interface-flux check; batch64: 64 independent uneven systems "
+                         "solved together; serving: Poisson arrivals at 50 req/s of 20k-2M-cell solves "
+                         "through the continuous-batching server")
+    ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
                          "device copy of known size (for rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE "
@@ -83,6 +221,13 @@ def main():
     import numpy as np
     import multiregionfoam_amd as m
     from multiregionfoam_amd import cases
+
+    if args.workload == "exchanger3":
+        return run_exchanger(m, cases, args)
+    if args.workload == "batch64":
+        return run_batch(m, cases, args)
+    if args.workload == "serving":
+        return run_serving(m, cases, args)
 
     case = cases.duct3d()
     A = m.CoupledFvMatrix(0)

@@ -48,7 +48,8 @@ typedef struct mrfg_case mrfg_case;
 typedef struct {
     double initial_residual;   /* sum|b-Ax0| / normFactor */
     double final_residual;     /* sum|b-Ax| / normFactor (FP64 in every mode) */
-    int n_iterations;          /* Krylov iterations (mixed: total inner iterations) */
+    int n_iterations;          /* Krylov iterations (mixed: FP32 inner iterations plus the FP64
+                                  continuation of a system whose refinement stalled) */
     int converged;
     int singular;              /* breakdown (rho or (t,t) or (w,p) zero) */
     int n_outer;               /* mixed: FP64 refinement steps; fp64: 0 */
@@ -140,6 +141,14 @@ int mrfg_reset_psi(mrfg_ctx *ctx);
 int mrfg_get_psi(mrfg_ctx *ctx, int sys, int region, double *psi);
 int mrfg_set_psi(mrfg_ctx *ctx, int sys, int region, const double *psi);
 
+/* FP64 interface heat-flux conservation check on the device: the heat flow
+   into each side of interface `iface` from the system's current field,
+   Q = sum_f coeff_f * (T_neighbour,f - T_own,f), with T_neighbour the GGI
+   interpolate of the other side (monolithicCouplingFvPatchField.C:381-464
+   coefficients; the interface flux of monolithicTemperature...C:204-224).
+   A conserving coupled solution gives q_a + q_b = 0. */
+int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b);
+
 /* ---- case builder ------------------------------------------------------ */
 
 /* Assemble a case dictionary (regions of transportTemperature /
@@ -195,6 +204,33 @@ int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b
    scaling HBM traffic counters (profiling facility; no counterpart in the
    reference). */
 int mrfg_copy_probe(mrfg_ctx *ctx, long long bytes, int reps, double *ms);
+
+/* ---- serving (continuous batching; no counterpart in the reference) ------
+
+   A server admits coupled solves as they arrive: worker threads, each with
+   its own device-resident batch and HIP stream, take the oldest request,
+   assemble it, and start it once its device footprint fits the HBM budget
+   left by the requests in flight; requests already waiting join the same
+   batch (up to max_batch systems).  Different workers' batches run
+   concurrently on the device.  A result does not depend on the batch or the
+   worker that produced it.  Every request is solved with the controls of
+   solver entry `entry` of `fv_solution`; keep_fields = 1 keeps each
+   result's field for mrfg_server_get_psi. */
+typedef struct mrfg_server mrfg_server;
+int mrfg_server_create(mrfg_server **srv, int device, long long hbm_budget_bytes, int n_workers, int max_batch,
+                       const char *fv_solution, const char *entry, int keep_fields);
+void mrfg_server_destroy(mrfg_server *srv);
+/* Queue one request (a case dictionary, as mrfg_case_build reads). */
+int mrfg_server_submit(mrfg_server *srv, const char *case_dict, int *request_id);
+/* Block until every submitted request has finished or failed. */
+int mrfg_server_wait_all(mrfg_server *srv);
+/* Result of a finished request: perf, times[3] = submit, start, done (s
+   since the server started), and the number of systems of its batch.
+   MRFG_ERR_STATE while it runs; MRFG_ERR_DEVICE if it failed
+   (mrfg_server_last_error). */
+int mrfg_server_result(mrfg_server *srv, int request_id, mrfg_perf *perf, double *times, int *batch_size);
+int mrfg_server_get_psi(mrfg_server *srv, int request_id, int region, double *psi);
+const char *mrfg_server_last_error(const mrfg_server *srv);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,15 @@ def lib():
         "mrfg_trace_dilu": ([vp, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)], C.c_int),
         "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
         "mrfg_copy_probe": ([vp, C.c_longlong, C.c_int, F64P], C.c_int),
+        "mrfg_interface_flux": ([vp, C.c_int, C.c_int, F64P, F64P], C.c_int),
+        "mrfg_server_create": ([C.POINTER(vp), C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
+                                C.c_int], C.c_int),
+        "mrfg_server_destroy": ([vp], None),
+        "mrfg_server_submit": ([vp, C.c_char_p, C.POINTER(C.c_int)], C.c_int),
+        "mrfg_server_wait_all": ([vp], C.c_int),
+        "mrfg_server_result": ([vp, C.c_int, C.c_void_p, F64P, C.POINTER(C.c_int)], C.c_int),
+        "mrfg_server_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
+        "mrfg_server_last_error": ([vp], C.c_char_p),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -118,7 +127,9 @@ EXPORTED = (
     "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe").split()
+    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
+    "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
+    "mrfg_server_get_psi mrfg_server_last_error").split()
 
 
 def _p(a, t):
@@ -146,15 +157,19 @@ def parse_controls(fv_solution: str, entry: str) -> Controls:
     return c
 
 
-def controls(solver="BiCGStab", tolerance=1e-10, relTol=0.0, minIter=0, maxIter=1000,
-             precision="fp64", innerRelTol=1e-4, innerMaxIter=1000, maxOuter=50, checkInterval=8):
-    """Build a Controls record from keyword values (same keys as fvSolution)."""
-    text = ("solvers { X { solver %s; preconditioner { preconditioner Cholesky; } tolerance %r; "
+def _controls_text(solver="BiCGStab", tolerance=1e-10, relTol=0.0, minIter=0, maxIter=1000,
+                   precision="fp64", innerRelTol=1e-4, innerMaxIter=1000, maxOuter=50, checkInterval=8):
+    """fvSolution text with one solver entry "X" holding the given keys."""
+    return ("solvers { X { solver %s; preconditioner { preconditioner Cholesky; } tolerance %r; "
             "relTol %r; minIter %d; maxIter %d; precision %s; checkInterval %d; "
             "refinement { innerRelTol %r; innerMaxIter %d; maxOuter %d; } } }"
             % (solver, float(tolerance), float(relTol), minIter, maxIter, precision, checkInterval,
                float(innerRelTol), innerMaxIter, maxOuter))
-    return parse_controls(text, "X")
+
+
+def controls(**kw):
+    """Build a Controls record from keyword values (same keys as fvSolution)."""
+    return parse_controls(_controls_text(**kw), "X")
 
 
 @dataclass
@@ -381,6 +396,13 @@ class CoupledFvMatrix:
         self._check(lib().mrfg_precondition_fp32(self._h, sys, _p(r, C.c_float), _p(w, C.c_float)))
         return w
 
+    def interface_flux(self, iface=0, sys=0):
+        """(Q_a, Q_b): heat flow into each side of a monolithic interface from
+        the current FP64 field, computed on the device."""
+        qa, qb = C.c_double(), C.c_double()
+        self._check(lib().mrfg_interface_flux(self._h, sys, iface, C.byref(qa), C.byref(qb)))
+        return qa.value, qb.value
+
     def copy_probe(self, nbytes, reps=10):
         """Average ms of a plain device copy of nbytes (reads + writes nbytes)."""
         ms = C.c_double()
@@ -412,3 +434,87 @@ class CoupledFvMatrix:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         self._check(lib().mrfg_bytes(self._h, sys, value_size, C.byref(a), C.byref(b), C.byref(c)))
         return dict(iter=a.value, spmv=b.value, dilu=c.value)
+
+
+# ---------------------------------------------------------------------------
+# Serving (continuous batching over a request stream)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class ServeResult:
+    perf: SolverPerformance
+    t_submit: float
+    t_start: float
+    t_done: float
+    batch_size: int
+
+    @property
+    def latency(self):
+        return self.t_done - self.t_submit
+
+
+class Server:
+    """Request loop over coupled solves: HBM-budget admission, worker threads
+    with their own device-resident batches, batching of waiting requests."""
+
+    def __init__(self, device=0, hbm_budget=64 << 30, workers=4, max_batch=8, fv_solution=None,
+                 entry="Tcoupled", keep_fields=False, **controls_kw):
+        if fv_solution is None:
+            kw = dict(solver="BiCGStab", tolerance=1e-10, maxIter=1000)
+            kw.update(controls_kw)
+            fv_solution = _controls_text(**kw)
+            entry = "X"
+        h = C.c_void_p()
+        rc = lib().mrfg_server_create(C.byref(h), device, int(hbm_budget), workers, max_batch,
+                                      _text(fv_solution), _text(entry), 1 if keep_fields else 0)
+        if rc != MRFG_OK:
+            raise MrfgError(rc, "cannot create the server")
+        self._h = h
+        self._sizes = []
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().mrfg_server_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def submit(self, case):
+        text = case if isinstance(case, str) else cases.to_dict_text(case)
+        rid = C.c_int()
+        rc = lib().mrfg_server_submit(self._h, _text(text), C.byref(rid))
+        if rc != MRFG_OK:
+            raise MrfgError(rc, "submit failed")
+        if not isinstance(case, str):
+            self._sizes.append([int(np.prod(r["nCells"])) for r in case["regions"]])
+        else:
+            self._sizes.append(None)
+        return rid.value
+
+    def wait_all(self):
+        lib().mrfg_server_wait_all(self._h)
+
+    def result(self, rid):
+        perf = Perf()
+        t = np.zeros(3)
+        bs = C.c_int()
+        rc = lib().mrfg_server_result(self._h, rid, C.byref(perf), _p(t, C.c_double), C.byref(bs))
+        if rc != MRFG_OK:
+            raise MrfgError(rc, lib().mrfg_server_last_error(self._h).decode())
+        sp = SolverPerformance(perf.initial_residual, perf.final_residual, perf.n_iterations, bool(perf.converged),
+                               bool(perf.singular), perf.n_outer, perf.seconds, perf.bytes_moved)
+        return ServeResult(sp, float(t[0]), float(t[1]), float(t[2]), bs.value)
+
+    def psi(self, rid):
+        sizes = self._sizes[rid]
+        if sizes is None:
+            raise ValueError("fields of requests submitted as text: sizes unknown")
+        out = []
+        for r, n in enumerate(sizes):
+            a = np.zeros(n)
+            rc = lib().mrfg_server_get_psi(self._h, rid, r, _p(a, C.c_double))
+            if rc != MRFG_OK:
+                raise MrfgError(rc, "no field kept for request %d" % rid)
+            out.append(a)
+        return out

@@ -16,6 +16,8 @@ The named workloads:
   0.37 fluid cells: a non-conformal GGI interface, 1.404M cells total.
 * ``batch``   -- many independent two-region systems of seeded uneven size.
 * ``exchanger3`` -- three regions (fluid / wall / fluid), two GGI interfaces.
+* ``serving_stream`` -- Poisson arrivals of two-region 3-D box solves of
+  log-uniform size, half of them with a GGI interface.
 """
 from __future__ import annotations
 
@@ -257,3 +259,32 @@ def scaled(c, factor):
     for r in c["regions"]:
         r["nCells"] = tuple(max(1, int(round(n * factor))) if n > 1 else 1 for n in r["nCells"])
     return c
+
+
+def serving_case(total_cells, ggi, seed):
+    """A two-region 3-D duct-over-slab solve of about `total_cells` cells
+    (fluid 73 %, solid 27 %, as in duct3d), GGI (1.25x refined, offset solid
+    face grid) or conformal, seeded velocity noise and solid conductivity
+    log-uniform in [1, 200] per 10-cell block."""
+    m = max(3, int(round((0.73 * total_cells / 2.0) ** (1.0 / 3.0))))
+    nf = (2 * m, m, m)
+    if ggi:
+        nsx, nsy = int(round(2 * m * 1.25)), int(round(m * 1.25))
+    else:
+        nsx, nsy = 2 * m, m
+    nsz = max(2, int(round(0.27 * total_cells / float(nsx * nsy))))
+    return duct3d(nf=nf, ns=(nsx, nsy, nsz), offset=0.37, seed=seed, conformal=not ggi)
+
+
+def serving_stream(n_requests=100, rate=50.0, lo=20_000, hi=2_000_000, ggi_fraction=0.5, seed=20260104):
+    """[(arrival time in s, case)] for Poisson arrivals at `rate` per second,
+    total cells log-uniform in [lo, hi]."""
+    rng = _rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n_requests):
+        t += rng.expovariate(rate)
+        cells = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        ggi = rng.random() < ggi_fraction
+        out.append((t, serving_case(cells, ggi, seed + 31 * i)))
+    return out

@@ -24,12 +24,19 @@ namespace mrfg {
             throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
     } while (0)
 
+// Buffers keep their capacity: a batch re-filled with systems of at most
+// the same size allocates (and frees, which synchronises the device) nothing.
 template <typename T>
 void DevBuf<T>::alloc(size_t count)
 {
+    if (p && count <= cap) {
+        n = count;
+        return;
+    }
     release();
     n = count;
-    HIP_OK(hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)));
+    cap = std::max<size_t>(count, 1);
+    HIP_OK(hipMalloc((void **)&p, cap * sizeof(T)));
 }
 
 template <typename T>
@@ -128,6 +135,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     HIP_OK(hipStreamSynchronize(stream_));
     sys_ = systems;
     lay_.clear();
+    ifDev_.clear();
     for (auto *s : systems) lay_.emplace_back(new SystemLayout(buildLayout(*s, preferLevels_)));
     const int nS = (int)systems.size();
     sliceBegin_.assign(nS, 0);
@@ -391,10 +399,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     st32_.alloc(nS);
     partials_.alloc((size_t)std::max(1, nBlk_) * 8);   // 4 double-double sums per block
     factorised_ = false;
-    haveFloat_ = false;
-    for (auto &v : v32_) v.release();
-    haloF32_.release();
-    haloB32_.release();
+    haveFloat_ = false;   // FP32 copies are rebuilt (in the buffers kept) on demand
     HIP_OK(hipStreamSynchronize(st));
 }
 
@@ -602,7 +607,7 @@ void Batch::runFp64(const SolverControls &c)
     }
 }
 
-void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters)
+void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std::vector<int> &fp64Iters)
 {
     Krylov<double> K64;
     bindKrylov(K64);
@@ -614,9 +619,15 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters)
     const bool pcg = c.solver == SolverControls::PCG;
     const int nS = (int)sys_.size();
     innerIters.assign(nS, 0);
+    fp64Iters.assign(nS, 0);
     HIP_OK(launchSetup<double>(A64, K64, false, stream_));
     std::vector<SysState> hs, hi(nS);
     readState(hs);
+    // a system whose outer residual stops falling (the FP32 correction no
+    // longer resolves its error) continues in FP64 from its current field
+    std::vector<double> prevRes(nS);
+    std::vector<char> fallback(nS, 0);
+    for (int q = 0; q < nS; ++q) prevRes[q] = hs[q].finalRes;
     int outer = 0;
     while (anyRunning(hs) && outer <= c.maxOuter) {
         HIP_OK(launchRefinePrepare(nS, st64_.p, st32_.p, c.innerRelTol, c.innerMaxIter, stream_));
@@ -640,7 +651,55 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters)
         HIP_OK(launchRefineResidual(A64, K64, stream_));
         readState(hs);
         ++outer;
+        bool changed = false;
+        for (int q = 0; q < nS; ++q) {
+            if (hs[q].status != kRunning) continue;
+            if (!(hs[q].finalRes < kStallFactor * prevRes[q])) {
+                fallback[q] = 1;
+                hs[q].status = kIdle;
+                changed = true;
+            }
+            prevRes[q] = hs[q].finalRes;
+        }
+        if (changed) {
+            HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+        }
     }
+    for (int q = 0; q < nS; ++q)
+        if (hs[q].status == kRunning) fallback[q] = 1;   // outer steps exhausted
+    bool any = false;
+    for (int q = 0; q < nS; ++q) any |= fallback[q] != 0;
+    if (!any) return;
+    // FP64 continuation of the fallback systems only
+    std::vector<SysState> mixedState = hs, h(nS);
+    for (int q = 0; q < nS; ++q) {
+        SysState s = {};
+        s.tol = c.tolerance;
+        s.relTol = c.relTol;
+        s.minIter = 0;
+        s.maxIter = c.maxIter;
+        s.nCells = hInfo_[q].nCells;
+        s.status = fallback[q] ? kRunning : kIdle;
+        h[q] = s;
+    }
+    HIP_OK(hipMemcpyAsync(st64_.p, h.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    SolverControls c64 = c;
+    c64.precision = SolverControls::FP64;
+    runFp64(c64);
+    readState(h);
+    for (int q = 0; q < nS; ++q) {
+        if (fallback[q]) {
+            fp64Iters[q] = h[q].nIter;
+            SysState m = h[q];
+            m.initRes = mixedState[q].initRes;
+            m.nIter = mixedState[q].nIter;
+            mixedState[q] = m;
+        }
+    }
+    HIP_OK(hipMemcpyAsync(st64_.p, mixedState.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
 }
 
 void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
@@ -654,8 +713,8 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
     factorised_ = false;
     factorise();
     initState(c);
-    std::vector<int> inner;
-    if (c.precision == SolverControls::Mixed) runMixed(c, inner);
+    std::vector<int> inner, fp64More;
+    if (c.precision == SolverControls::Mixed) runMixed(c, inner, fp64More);
     else runFp64(c);
     HIP_OK(hipStreamSynchronize(stream_));
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -671,7 +730,7 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
         p.singular = hs[q].status == kSingular;
         p.seconds = secs;
         if (c.precision == SolverControls::Mixed) {
-            p.nIterations = inner[q];
+            p.nIterations = inner[q] + fp64More[q];
             p.nOuter = hs[q].nIter;
             p.bytes = p.nIterations * bytesIter((int)q, 4) + p.nOuter * bytesSpmv((int)q, 8);
         } else {
@@ -754,6 +813,46 @@ void Batch::precondition(const double *rHost, double *wHost, int sys)
     checkDeviceError();
     HIP_OK(hipMemcpy(w.data(), dw.p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
     for (long long c = 0; c < L.nCells; ++c) wHost[c] = w[posBegin(sys) + L.posOfCell[c]];
+}
+
+void Batch::interfaceFlux(int sys, int iface, double *qa, double *qb)
+{
+    HIP_OK(hipSetDevice(device_));
+    const CoupledSystem &S = *sys_.at(sys);
+    const SystemLayout &L = *lay_.at(sys);
+    const CoupledInterface &I = S.interfaces.at(iface);
+    if (ifDev_.size() != sys_.size()) ifDev_.resize(sys_.size());
+    if (ifDev_[sys].size() != S.interfaces.size()) ifDev_[sys].resize(S.interfaces.size());
+    const InterfaceSide *sides[2] = {&I.a, &I.b};
+    double out[2] = {0.0, 0.0};
+    DevBuf<double> dOut;
+    dOut.alloc(2);
+    for (int q = 0; q < 2; ++q) {
+        auto &slot = ifDev_[sys][iface][q];
+        const InterfaceSide &O = *sides[q], &Nb = *sides[1 - q];
+        if (!slot) {
+            slot.reset(new IfaceDev);
+            const int p0 = posBegin(sys);
+            const int oO = L.regionOffset[O.region], oN = L.regionOffset[Nb.region];
+            std::vector<int32_t> fcPos(O.nFaces()), pairPos(O.idx.size());
+            for (int f = 0; f < O.nFaces(); ++f) fcPos[f] = p0 + L.posOfCell[oO + O.fc[f]];
+            for (size_t k = 0; k < O.idx.size(); ++k) pairPos[k] = p0 + L.posOfCell[oN + Nb.fc[O.idx[k]]];
+            slot->nFaces = O.nFaces();
+            slot->fcPos.upload(fcPos, stream_);
+            slot->pairPos.upload(pairPos.empty() ? std::vector<int32_t>{0} : pairPos, stream_);
+            slot->off.upload(O.off, stream_);
+            slot->coeff.upload(O.coeff.empty() ? std::vector<double>{0.0} : O.coeff, stream_);
+            slot->w.upload(O.w.empty() ? std::vector<double>{0.0} : O.w, stream_);
+        }
+        const int nBlk = (slot->nFaces + 255) / 256;
+        if ((size_t)nBlk * 8 > partials_.n) throw std::runtime_error("interface larger than the reduction buffer");
+        HIP_OK(launchInterfaceFlux(slot->nFaces, slot->fcPos.p, slot->coeff.p, slot->off.p, slot->pairPos.p,
+                                   slot->w.p, x64_.p, partials_.p, dOut.p + q, stream_));
+    }
+    HIP_OK(hipMemcpyAsync(out, dOut.p, sizeof(out), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    *qa = out[0];
+    *qb = out[1];
 }
 
 void Batch::precondition32(const float *rHost, float *wHost, int sys)

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <memory>
 #include <string>
 #include <vector>
@@ -50,7 +51,8 @@ struct SolvePerf {
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
-    size_t n = 0;
+    size_t n = 0;      // elements in use
+    size_t cap = 0;    // elements allocated
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -60,6 +62,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        cap = 0;
     }
     void alloc(size_t count);
     void upload(const std::vector<T> &h, hipStream_t st);
@@ -96,6 +99,10 @@ public:
     void diluFactor(int sys, double *rDHost);                        // rD per cell
     void precondition(const double *rHost, double *wHost, int sys);   // w = M^-1 r
     void precondition32(const float *rHost, float *wHost, int sys);   // FP32 records, rD from the FP64 factor
+    // Heat flow into each side of interface `iface` of system `sys` from the
+    // current FP64 field, on the device: Q = sum coeff*(T_neighbour - T_own)
+    // per side (a conserving interface gives qa + qb = 0).
+    void interfaceFlux(int sys, int iface, double *qa, double *qb);
 
     // Time `reps` repetitions of one kernel on resident data; returns the
     // average device time per launch (ms), measured with HIP events on the
@@ -126,7 +133,10 @@ private:
     void initState(const SolverControls &c);
     void readState(std::vector<SysState> &out);
     void runFp64(const SolverControls &c);
-    void runMixed(const SolverControls &c, std::vector<int> &innerIters);
+    // Mixed FP32/FP64 iterative refinement; a system whose outer residual
+    // falls by less than kStallFactor in an outer step continues in FP64.
+    static constexpr double kStallFactor = 0.5;
+    void runMixed(const SolverControls &c, std::vector<int> &innerIters, std::vector<int> &fp64Iters);
     void checkDeviceError();
     void markAllRunning();
     template <typename T>
@@ -170,6 +180,13 @@ private:
     DevBuf<double> v64_[12];
     DevBuf<float> v32_[14];
     DevBuf<SysState> st64_, st32_;
+    // per system, per interface, per side: the face lists in batch positions
+    struct IfaceDev {
+        int nFaces = 0;
+        DevBuf<int32_t> fcPos, off, pairPos;
+        DevBuf<double> coeff, w;
+    };
+    std::vector<std::vector<std::array<std::unique_ptr<IfaceDev>, 2>>> ifDev_;
     DevBuf<double> partials_;
     std::vector<SysInfo> hInfo_;
 };

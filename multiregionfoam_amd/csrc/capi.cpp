@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "serve.hpp"
 #include "coupled_system.hpp"
 #include "dictionary.hpp"
 
@@ -450,6 +451,18 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w)
     });
 }
 
+int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b)
+{
+    if (!ctx || !q_a || !q_b) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        mrfg::Batch &B = committedBatch(ctx);
+        if (iface < 0 || iface >= (int)ctx->systems[sys]->interfaces.size())
+            throw std::invalid_argument("interface index out of range");
+        B.interfaceFlux(sys, iface, q_a, q_b);
+    });
+}
+
 int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w)
 {
     if (!ctx || !r || !w) return MRFG_ERR_ARG;
@@ -509,5 +522,87 @@ int mrfg_bytes(mrfg_ctx *ctx, int sys, int value_size, double *b_iter, double *b
         if (b_dilu) *b_dilu = B.bytesDilu(sys, value_size);
     });
 }
+
+/* ---- serving ----------------------------------------------------------- */
+
+struct mrfg_server {
+    std::unique_ptr<mrfg::Server> srv;
+    std::string err;
+};
+
+int mrfg_server_create(mrfg_server **out, int device, long long hbm_budget_bytes, int n_workers, int max_batch,
+                       const char *fv_solution, const char *entry, int keep_fields)
+{
+    if (!out || !fv_solution || !entry || hbm_budget_bytes <= 0 || n_workers < 1 || max_batch < 1)
+        return MRFG_ERR_ARG;
+    *out = nullptr;
+    try {
+        mrfg::Dict d = mrfg::parseDict(fv_solution);
+        mrfg::SolverControls c = mrfg::parseControls(d.subDict("solvers").subDict(entry));
+        auto *s = new mrfg_server;
+        s->srv.reset(new mrfg::Server(device, (size_t)hbm_budget_bytes, n_workers, max_batch, c, keep_fields != 0));
+        *out = s;
+        return MRFG_OK;
+    } catch (const mrfg::DictError &) {
+        return MRFG_ERR_DICT;
+    } catch (const std::exception &) {
+        return MRFG_ERR_DEVICE;
+    }
+}
+
+void mrfg_server_destroy(mrfg_server *s) { delete s; }
+
+int mrfg_server_submit(mrfg_server *s, const char *case_dict, int *request_id)
+{
+    if (!s || !case_dict || !request_id) return MRFG_ERR_ARG;
+    *request_id = s->srv->submit(case_dict);
+    return MRFG_OK;
+}
+
+int mrfg_server_wait_all(mrfg_server *s)
+{
+    if (!s) return MRFG_ERR_ARG;
+    s->srv->waitAll();
+    return MRFG_OK;
+}
+
+int mrfg_server_result(mrfg_server *s, int id, mrfg_perf *perf, double *times, int *batch_size)
+{
+    if (!s || id < 0 || id >= s->srv->nRequests()) return MRFG_ERR_ARG;
+    const mrfg::ServeRequest &R = s->srv->request(id);
+    if (R.status == -1) {
+        s->err = R.error;
+        return MRFG_ERR_DEVICE;
+    }
+    if (R.status != 2) return MRFG_ERR_STATE;
+    if (perf) {
+        perf->initial_residual = R.perf.initialResidual;
+        perf->final_residual = R.perf.finalResidual;
+        perf->n_iterations = R.perf.nIterations;
+        perf->converged = R.perf.converged;
+        perf->singular = R.perf.singular;
+        perf->n_outer = R.perf.nOuter;
+        perf->seconds = R.perf.seconds;
+        perf->bytes_moved = R.perf.bytes;
+    }
+    if (times) {
+        times[0] = R.tSubmit;
+        times[1] = R.tAdmit;
+        times[2] = R.tDone;
+    }
+    if (batch_size) *batch_size = R.batchSize;
+    return MRFG_OK;
+}
+
+int mrfg_server_get_psi(mrfg_server *s, int id, int region, double *psi)
+{
+    if (!s || !psi || id < 0 || id >= s->srv->nRequests()) return MRFG_ERR_ARG;
+    const mrfg::ServeRequest &R = s->srv->request(id);
+    if (R.status != 2 || region < 0 || region >= (int)R.psi.size()) return MRFG_ERR_STATE;
+    std::copy(R.psi[region].begin(), R.psi[region].end(), psi);
+    return MRFG_OK;
+}
+
+const char *mrfg_server_last_error(const mrfg_server *s) { return s ? s->err.c_str() : ""; }
 
 }  // extern "C"

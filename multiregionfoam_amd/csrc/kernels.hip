@@ -431,7 +431,11 @@ __global__ void k_refine_prepare(int nSys, const SysState *outer, SysState *inne
     if (q >= nSys) return;
     SysState s = {};
     s.tol = 0.0;
-    s.relTol = innerRelTol;
+    // reduce the correction's residual by innerRelTol, or only by as much as
+    // the outer tolerance still needs (with a margin of 4): the last outer
+    // step does not overshoot
+    const double need = outer[q].finalRes > 0.0 ? 0.25 * outer[q].tol / outer[q].finalRes : 1.0;
+    s.relTol = fmin(fmax(innerRelTol, need), 0.5);
     s.minIter = 0;
     s.maxIter = innerMaxIter;
     s.nCells = outer[q].nCells;
@@ -583,6 +587,51 @@ hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, 
 hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st)
 {
     k_refine_update<<<K64.nBlk, kBlock, 0, st>>>(K64.blocks, K64.state, K64.x, K32.x);
+    MRFG_CHECK_LAUNCH();
+}
+
+// Interface heat flow of one side of a monolithic interface,
+// Q = sum_f coeff_f * (Tn_f - To_f) with Tn_f = sum_p w_p x[pos_p] (the
+// patch-neighbour field through the GGI weights), as a double-double sum:
+// one term per face, block partials, then one block adds the partials.
+__global__ __launch_bounds__(kBlock) void k_iface_flux(int nFaces, const int32_t *fcPos, const double *coeff,
+                                                     const int32_t *off, const int32_t *pairPos,
+                                                     const double *w, const double *x, double *partials)
+{
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    double v[1] = {0.0};
+    if (f < nFaces) {
+        double tn = 0.0;
+        for (int q = off[f]; q < off[f + 1]; ++q) tn += w[q] * x[pairPos[q]];
+        v[0] = coeff[f] * (tn - x[fcPos[f]]);
+    }
+    blockPartial<1>(v, partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_iface_flux_sum(int nBlk, const double *partials, double *out)
+{
+    DD acc{0.0, 0.0};
+    for (int b = threadIdx.x; b < nBlk; b += kBlock)
+        acc = ddAdd(acc, DD{partials[(size_t)b * 8], partials[(size_t)b * 8 + 1]});
+    __shared__ DD sh[kBlock / 64];
+    acc = waveSum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t = sh[0];
+        for (int q = 1; q < kBlock / 64; ++q) t = ddAdd(t, sh[q]);
+        out[0] = t.hi + t.lo;
+    }
+}
+
+hipError_t launchInterfaceFlux(int nFaces, const int32_t *fcPos, const double *coeff, const int32_t *off,
+                               const int32_t *pairPos, const double *w, const double *x, double *partials,
+                               double *out, hipStream_t st)
+{
+    const int nBlk = (nFaces + kBlock - 1) / kBlock;
+    if (nBlk > 0)
+        k_iface_flux<<<nBlk, kBlock, 0, st>>>(nFaces, fcPos, coeff, off, pairPos, w, x, partials);
+    k_iface_flux_sum<<<1, kBlock, 0, st>>>(nBlk, partials, out);
     MRFG_CHECK_LAUNCH();
 }
 

@@ -36,6 +36,12 @@ template <typename T>
 hipError_t launchPcgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st);
 
 
+// Heat flow through one side of a monolithic interface (FP64, double-double
+// sum) into out[0]; partials needs 8 doubles per 256 faces.
+hipError_t launchInterfaceFlux(int nFaces, const int32_t *fcPos, const double *coeff, const int32_t *off,
+                               const int32_t *pairPos, const double *w, const double *x, double *partials,
+                               double *out, hipStream_t st);
+
 // Average time (ms) of a plain device-to-device copy of `bytes` (reads and
 // writes `bytes` per launch), for calibrating HBM traffic counters.
 hipError_t copyProbe(int device, size_t bytes, int reps, double *ms);

@@ -707,7 +707,8 @@ void orc_interface_flux(const orc_system *sys, int i, double *const *psi,
         }
         double dOwn = 0.5 * LO->cellSize[a], dNei = 0.5 * LN->cellSize[a];
         double magDelta = dOwn + dNei, wP = dNei / magDelta;
-        double Q = 0.0, S = 0.0;
+        csum Q = {0.0, 0.0};
+        double S = 0.0;
         for (int f = 0; f < n; ++f) {
             double Tn = 0.0, kn = 0.0;
             for (int p = off[f]; p < off[f + 1]; ++p) {
@@ -715,7 +716,7 @@ void orc_interface_flux(const orc_system *sys, int i, double *const *psi,
                 kn += w[p] * LN->k[fcN[idx[p]]];
             }
             double To = xO[fc[f]];
-            Q += coeff[f] * (Tn - To);
+            csum_add(&Q, coeff[f] * (Tn - To));
             /* monolithicTemperatureFvPatchScalarField.C:204-224 source with
                Tw from monolithicThermalDiffusivity...C:381 */
             double kOwn = LO->k[fc[f]] / (1.0 - wP) / magDelta;
@@ -726,7 +727,7 @@ void orc_interface_flux(const orc_system *sys, int i, double *const *psi,
                 S += (kOwn * (Tw - To) - kd * (Tn - To)) * magSf[f];
             }
         }
-        if (side) { *QB = Q; *srcB = S; } else { *QA = Q; *srcA = S; }
+        if (side) { *QB = csum_val(Q); *srcB = S; } else { *QA = csum_val(Q); *srcA = S; }
     }
 }
 

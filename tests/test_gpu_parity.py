@@ -141,6 +141,26 @@ def test_mixed_precision_refinement(gpu, name):
     assert err <= MIXED_TOL, err
 
 
+@pytest.mark.parametrize("name,balance", [("plate2d", 1e-6), ("duct_conformal", 1e-6),
+                                          ("duct_ggi", 2e-2), ("exchanger3", 2e-2)])
+def test_interface_flux_check_matches_oracle(gpu, name, balance):
+    """The device's FP64 interface heat-flux check equals the oracle's on the
+    converged field.  Conformal interfaces conserve heat to the solver
+    tolerance; GGI interfaces with independently rescaled weights on each
+    side (and partly uncovered edge faces) balance only approximately, as
+    the reference's rescaled GGI does."""
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    A = _ctx(gpu, case)
+    A.solve(tolerance=1e-12, maxIter=2000)
+    psi = A.psi(0)
+    for i in range(len(O.interfaces)):
+        qa, qb = A.interface_flux(i)
+        QA, QB, _, _ = O.interface_flux(i, psi)
+        assert qa == QA and qb == QB
+        assert abs(qa + qb) <= balance * abs(qa), (qa, qb)
+
+
 def test_batch_equals_alone(gpu):
     """Each system's result is bit-identical alone and inside a batch."""
     cs = [cases.box_pair(4000, 3000, 20.0, 11), cases.box_pair(9000, 2000, 300.0, 12),
@@ -159,6 +179,46 @@ def test_batch_equals_alone(gpu):
         assert perfs[q].n_iterations == p0.n_iterations
         assert perfs[q].final_residual == p0.final_residual
         assert np.array_equal(np.concatenate(B.psi(q)), x0)
+
+
+def test_serving_results_equal_alone(gpu):
+    """Requests served concurrently (several workers, batches formed from
+    waiting requests) give the same results as each solved alone."""
+    reqs = [cases.serving_case(n, ggi, 100 + i) for i, (n, ggi) in
+            enumerate([(3000, True), (8000, False), (2000, False), (12000, True), (5000, True),
+                       (4000, False), (9000, True), (2500, False)])]
+    srv = gpu.Server(device=0, hbm_budget=8 << 30, workers=3, max_batch=4, keep_fields=True,
+                     tolerance=1e-10, maxIter=1000)
+    ids = [srv.submit(c) for c in reqs]
+    srv.wait_all()
+    batched = 0
+    for rid, c in zip(ids, reqs):
+        res = srv.result(rid)
+        batched += res.batch_size > 1
+        assert res.perf.converged and res.t_done >= res.t_start >= res.t_submit
+        A = _ctx(gpu, c)
+        p = A.solve(tolerance=1e-10, maxIter=1000)[0]
+        assert res.perf.n_iterations == p.n_iterations
+        assert res.perf.final_residual == p.final_residual
+        for a, b in zip(srv.psi(rid), A.psi(0)):
+            assert np.array_equal(a, b)
+    assert batched > 0          # at least one batch held several requests
+
+
+def test_serving_admission_by_budget(gpu):
+    """Under a small HBM budget the requests that fit are served and one
+    larger than the whole budget fails cleanly without blocking the rest."""
+    small = [cases.serving_case(3000, False, 200 + i) for i in range(3)]
+    big = cases.serving_case(400000, False, 299)
+    srv = gpu.Server(device=0, hbm_budget=150 << 20, workers=2, max_batch=4, tolerance=1e-8, maxIter=500)
+    ids = [srv.submit(c) for c in small]
+    big_id = srv.submit(big)
+    srv.wait_all()
+    for rid in ids:
+        res = srv.result(rid)
+        assert res.perf.converged
+    with pytest.raises(gpu.MrfgError, match="budget"):
+        srv.result(big_id)
 
 
 def test_fvsolution_dictionary_selects_solver(gpu):
