@@ -13,7 +13,7 @@ struct SysState {
     double rho, rhoOld, alpha, omega, beta;
     double normFactor, xSum, initRes, finalRes;
     double tol, relTol;
-    double red[4];          // last finalised reduction values (diagnostics)
+    double red0, red1, red2, red3;   // last finalised reduction values (diagnostics)
     int nIter, minIter, maxIter, status;
     int nCells, pad0, pad1, pad2;
 };

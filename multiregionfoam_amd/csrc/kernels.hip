@@ -115,13 +115,13 @@ __device__ inline void applyStage(int stage, SysState &S, const double *out)
         }
         break;
     case kFinAlpha:
-        S.red[0] = out[0];
+        S.red0 = out[0];
         if (S.rho == 0.0 || out[0] == 0.0) S.status = kSingular;
         else S.alpha = S.rho / out[0];
         break;
     case kFinOmega:
-        S.red[0] = out[0];
-        S.red[1] = out[1];
+        S.red0 = out[0];
+        S.red1 = out[1];
         if (out[1] == 0.0) S.status = kSingular;
         else S.omega = out[0] / out[1];
         break;
@@ -139,7 +139,7 @@ __device__ inline void applyStage(int stage, SysState &S, const double *out)
         S.beta = S.rho / S.rhoOld;
         break;
     case kFinPcgAlpha:
-        S.red[0] = out[0];
+        S.red0 = out[0];
         if (out[0] == 0.0 || fabs(out[0]) / S.normFactor < 1e-300) S.status = kSingular;
         else S.alpha = S.rho / out[0];
         break;
@@ -152,37 +152,70 @@ __device__ inline void applyStage(int stage, SysState &S, const double *out)
     }
 }
 
-// One block per system: reduce the block partials of the system in a fixed
-// order (deterministic: the result does not depend on the other systems of
-// the batch) and apply the stage's scalar update.
-__global__ __launch_bounds__(kBlock) void k_finalize(int stage, const SysInfo *info, SysState *st,
-                                                     const double *partials)
+// One block per system: reduce the system's block partials and apply the
+// stage's scalar update.  The kernel sits between two vector kernels, so it
+// is written for latency: every partial load, and the state load, is issued
+// before the first use (one memory round trip for systems of up to
+// kFinBlock * kFinUnroll blocks), and the per-wave sums are combined by a
+// wave-level tree instead of one thread.  The order of every sum is fixed, so
+// the result does not depend on the other systems of the batch.
+constexpr int kFinBlock = 1024;
+constexpr int kFinUnroll = 8;
+
+template <int NV>
+__global__ __launch_bounds__(kFinBlock) void k_finalize(int stage, const SysInfo *info, SysState *st,
+                                                        const double *partials)
 {
     const int q = blockIdx.x;
-    if (st[q].status != kRunning) return;
-    const int b0 = info[q].blkBegin, b1 = b0 + info[q].nBlk;
-    DD acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-    for (int b = b0 + threadIdx.x; b < b1; b += kBlock)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            acc[j] = ddAdd(acc[j], DD{partials[((size_t)b * 4 + j) * 2], partials[((size_t)b * 4 + j) * 2 + 1]});
-    __shared__ DD sh[kBlock / 64][4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const SysInfo I = info[q];
+    SysState S = st[q];   // uniform: scalar loads, in flight with the partials
+    DD acc[NV];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[j] = waveSum(acc[j]);
-    if (lane == 0)
-        for (int j = 0; j < 4; ++j) sh[wid][j] = acc[j];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double out[4];
-        for (int j = 0; j < 4; ++j) {
-            DD t = sh[0][j];
-            for (int w = 1; w < kBlock / 64; ++w) t = ddAdd(t, sh[w][j]);
-            out[j] = t.hi + t.lo;
+    for (int j = 0; j < NV; ++j) acc[j] = DD{0.0, 0.0};
+    for (int k0 = threadIdx.x; k0 < I.nBlk; k0 += kFinBlock * kFinUnroll) {
+        double2 h[kFinUnroll][NV];
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; ++u) {
+            const int k = k0 + u * kFinBlock;
+            const double2 *rec = reinterpret_cast<const double2 *>(partials + (size_t)(I.blkBegin + min(k, I.nBlk - 1)) * 8);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) h[u][j] = rec[j];
         }
-        SysState S = st[q];
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; ++u)
+            if (k0 + u * kFinBlock < I.nBlk)
+#pragma unroll
+                for (int j = 0; j < NV; ++j) acc[j] = ddAdd(acc[j], DD{h[u][j].x, h[u][j].y});
+    }
+    __shared__ DD sh[kFinBlock / 64][NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = waveSum(acc[j]);
+    if (lane == 0)
+        for (int j = 0; j < NV; ++j) sh[wid][j] = acc[j];
+    __syncthreads();
+    if (wid != 0) return;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = waveSum(lane < kFinBlock / 64 ? sh[lane][j] : DD{0.0, 0.0});
+    if (lane == 0 && S.status == kRunning) {
+        double out[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < NV; ++j) out[j] = acc[j].hi + acc[j].lo;
         applyStage(stage, S, out);
-        st[q] = S;
+        // field by field: a whole-struct copy would keep S in memory
+        SysState &G = st[q];
+        G.rho = S.rho;
+        G.rhoOld = S.rhoOld;
+        G.alpha = S.alpha;
+        G.omega = S.omega;
+        G.beta = S.beta;
+        G.normFactor = S.normFactor;
+        G.xSum = S.xSum;
+        G.initRes = S.initRes;
+        G.finalRes = S.finalRes;
+        G.red0 = S.red0;
+        G.red1 = S.red1;
+        G.nIter = S.nIter;
+        G.status = S.status;
     }
 }
 
@@ -488,10 +521,16 @@ __global__ __launch_bounds__(kBlock) void k_refine_update(const BlockEntry *bloc
 
 #define MRFG_CHECK_LAUNCH() return hipGetLastError()
 
-template <typename T>
-static void fin(int stage, const Krylov<T> &K, hipStream_t st)
+// number of double-double sums each stage reduces
+constexpr int finSums(int stage)
 {
-    k_finalize<<<K.nSys, kBlock, 0, st>>>(stage, K.info, K.state, K.partials);
+    return stage == kFinInitBicg || stage == kFinInitPcg ? 3 : stage == kFinOmega || stage == kFinResBicg ? 2 : 1;
+}
+
+template <int Stage, typename T>
+static void fin(const Krylov<T> &K, hipStream_t st)
+{
+    k_finalize<finSums(Stage)><<<K.nSys, kFinBlock, 0, st>>>(Stage, K.info, K.state, K.partials);
 }
 
 template <typename T>
@@ -506,11 +545,12 @@ hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStr
 {
     k_setup_residual<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.x, K.b,
                                                    K.ax, K.r);
-    fin(kFinXSum, K, st);
+    fin<kFinXSum>(K, st);
     k_setup_norm<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.sumA, K.ax, K.b,
                                                K.r, pcg ? nullptr : K.rw, pcg ? K.pA : K.pv, pcg ? nullptr : K.v,
                                                pcg ? 1 : 0);
-    fin(pcg ? kFinInitPcg : kFinInitBicg, K, st);
+    if (pcg) fin<kFinInitPcg>(K, st);
+    else fin<kFinInitBicg>(K, st);
     MRFG_CHECK_LAUNCH();
 }
 
@@ -532,17 +572,17 @@ hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const 
         if (e != hipSuccess) return e;
     }
     k_spmv_alpha<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.ph, K.rw, K.v);
-    fin(kFinAlpha, K, st);
+    fin<kFinAlpha>(K, st);
     k_bicg_s<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.r, K.v, K.sv);
     {
         hipError_t e = launchPrecondition(D, K, K.sv, K.sh, st);
         if (e != hipSuccess) return e;
     }
     k_spmv_omega<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.sh, K.sv, K.t);
-    fin(kFinOmega, K, st);
+    fin<kFinOmega>(K, st);
     k_bicg_xr<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.x, K.r, K.ph, K.sh,
                                             K.sv, K.t, K.rw);
-    fin(kFinResBicg, K, st);
+    fin<kFinResBicg>(K, st);
     MRFG_CHECK_LAUNCH();
 }
 
@@ -554,12 +594,12 @@ hipError_t launchPcgIteration(const MatView<T> &A, const DiluView<T> &D, const K
         if (e != hipSuccess) return e;
     }
     k_pcg_rho<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.w, K.r);
-    fin(kFinPcgRho, K, st);
+    fin<kFinPcgRho>(K, st);
     k_pcg_p<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.w, K.pA);
     k_spmv_pcg<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.pA, K.w);
-    fin(kFinPcgAlpha, K, st);
+    fin<kFinPcgAlpha>(K, st);
     k_pcg_xr<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.x, K.r, K.pA, K.w);
-    fin(kFinResPcg, K, st);
+    fin<kFinResPcg>(K, st);
     MRFG_CHECK_LAUNCH();
 }
 
@@ -574,7 +614,7 @@ hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &
 {
     k_refine_residual<<<K64.nBlk, kBlock, 0, st>>>(A, K64.blocks, K64.info, K64.state, K64.partials,
                                                    K64.x, K64.b, K64.r);
-    fin(kFinRefine, K64, st);
+    fin<kFinRefine>(K64, st);
     MRFG_CHECK_LAUNCH();
 }
 
