@@ -56,7 +56,8 @@ def run_steps(A, ctrl, warmup, steps, all_perf=False, verbose=False):
 def kernel_probe(A, reps):
     """Replay each hot kernel `reps` times, then copy 256 MiB device to device
     `reps` times (reads 256 MiB and writes 256 MiB per dispatch)."""
-    for which, name in ((0, "spmv"), (1, "dilu_fwd"), (2, "dilu_bwd"), (3, "dilu_factor")):
+    for which, name in ((0, "spmv"), (1, "dilu_fwd"), (2, "dilu_bwd"), (3, "dilu_factor"),
+                        (4, "dilu_fwd_in_solve"), (5, "dilu_bwd_in_solve"), (6, "spmv_alpha_in_solve")):
         ms = A.time_kernel(which, reps)
         print("probe %s %.4f ms" % (name, ms), file=sys.stderr, flush=True)
     ms = A.copy_probe(256 * 2**20, reps)
@@ -260,16 +261,23 @@ def main():
                      p50_ms=1e3 * pct(mlat, 0.5), p99_ms=1e3 * pct(mlat, 0.99))
 
     # ---- per-kernel timing (HIP events on the library stream) ----
+    # back to back (the kernel alone, its data hot in the caches), and inside
+    # BiCGStab iterations (one launch per iteration timed: the caches and
+    # clocks of a solve); the roofline uses the latter
     A.reset_psi()
     k_spmv = A.time_kernel(0, args.kernel_reps)
     k_fwd = A.time_kernel(1, args.kernel_reps)
     k_bwd = A.time_kernel(2, args.kernel_reps)
-    per_iter = {"spmv": 2 * k_spmv, "dilu_fwd": 2 * k_fwd, "dilu_bwd": 2 * k_bwd}
+    s_fwd = A.time_kernel(4, args.kernel_reps)
+    s_bwd = A.time_kernel(5, args.kernel_reps)
+    s_spmv = A.time_kernel(6, args.kernel_reps)
+    per_iter = {"spmv": 2 * s_spmv, "dilu_fwd": 2 * s_fwd, "dilu_bwd": 2 * s_bwd}
     dominant = max(per_iter, key=per_iter.get)
-    launch_ms = {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd}[dominant]
-    launch_bytes = bm64["spmv"] if dominant == "spmv" else 0.5 * bm64["dilu"]
+    launch_ms = {"spmv": s_spmv, "dilu_fwd": s_fwd, "dilu_bwd": s_bwd}[dominant]
+    # the in-solve SpMV also reads the shadow residual for its dot product
+    launch_bytes = (bm64["spmv"] + 8 * info["cells"]) if dominant == "spmv" else 0.5 * bm64["dilu"]
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    kernel_names = {"spmv": "k_spmv_plain<double>", "dilu_fwd": "k_dilu_box<double, 3, 0>",
+    kernel_names = {"spmv": "k_spmv_alpha<double>", "dilu_fwd": "k_dilu_box<double, 3, 0>",
                     "dilu_bwd": "k_dilu_box<double, 3, 1>"}
 
     # HBM bytes per launch of that kernel, from the committed rocprofv3 PMC
@@ -328,7 +336,8 @@ def main():
         "p99_ms": 1e3 * pct(lat, 0.99),
         "rel_l2_vs_fp64_oracle": err64,
         "mixed": mixed,
-        "kernels_ms": launch_ms and {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd},
+        "kernels_ms": {"back_to_back": {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd},
+                       "in_solve": {"spmv_alpha": s_spmv, "dilu_fwd": s_fwd, "dilu_bwd": s_bwd}},
         "roofline": {"bound": "hbm", "kernel": kernel_names[dominant], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": launch_bytes},

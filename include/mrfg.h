@@ -187,7 +187,10 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w);
 int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w);
 /* Average device time (ms) of one launch over `reps` launches, from HIP
    events on the context's stream: which 0 = SpMV, 1 = DILU forward sweep,
-   2 = DILU backward sweep, 3 = DILU factorisation. */
+   2 = DILU backward sweep, 3 = DILU factorisation, each launched back to
+   back; 4 = the first forward sweep, 5 = the first backward sweep and 6 =
+   the SpMV with the alpha reduction of each of `reps` BiCGStab iterations
+   (run from the initial guess with tolerance 0; psi is restored after). */
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms);
 /* Trace one DILU forward/backward pair (tracing facility; no counterpart
    in the reference): out receives, per wave in dispatch order, [start ns,

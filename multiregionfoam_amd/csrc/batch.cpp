@@ -911,8 +911,25 @@ double Batch::timeKernel(int which, int reps)
             if (which == 2) HIP_OK(hipEventRecord(b, stream_));
         }
     };
-    one(reps);   // warm-up into the spare event pair
-    for (int i = 0; i < reps; ++i) one(i);
+    if (which >= 4) {
+        // inside BiCGStab iterations of the solve (tolerance 0, from the
+        // initial guess): the kernel sees the caches and clocks of a solve
+        SolverControls c;
+        c.tolerance = 0.0;
+        c.relTol = 0.0;
+        c.maxIter = reps + 2;
+        initState(c);
+        resetPsi();
+        HIP_OK(launchSetup<double>(A, K, false, stream_));
+        for (int i = 0; i <= reps; ++i) {
+            const IterProbe probe{which - 3, ev[2 * (i == 0 ? reps : i - 1)], ev[2 * (i == 0 ? reps : i - 1) + 1]};
+            HIP_OK(launchBicgIteration<double>(A, D, K, stream_, &probe));
+        }
+        resetPsi();
+    } else {
+        one(reps);   // warm-up into the spare event pair
+        for (int i = 0; i < reps; ++i) one(i);
+    }
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
     double tot = 0;

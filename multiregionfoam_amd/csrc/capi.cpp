@@ -474,7 +474,7 @@ int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w)
 
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms)
 {
-    if (!ctx || !ms || reps < 1 || which < 0 || which > 3) return MRFG_ERR_ARG;
+    if (!ctx || !ms || reps < 1 || which < 0 || which > 6) return MRFG_ERR_ARG;
     return guard(ctx, [&] { *ms = committedBatch(ctx).timeKernel(which, reps); });
 }
 

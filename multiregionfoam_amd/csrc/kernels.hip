@@ -564,14 +564,26 @@ hipError_t launchPrecondition(const DiluView<T> &D, const Krylov<T> &K, const T 
 
 
 template <typename T>
-hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st)
+hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st,
+                               const IterProbe *probe)
 {
+    auto mark = [&](int which, bool begin) {
+        if (probe && probe->which == which) (void)hipEventRecord(begin ? probe->begin : probe->end, st);
+    };
     k_bicg_p<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
     {
-        hipError_t e = launchPrecondition(D, K, K.pv, K.ph, st);
+        mark(1, true);
+        hipError_t e = diluForward<T>(D, K.state, K.pv, K.ph, st);
         if (e != hipSuccess) return e;
+        mark(1, false);
+        mark(2, true);
+        e = diluBackward<T>(D, K.state, K.ph, st);
+        if (e != hipSuccess) return e;
+        mark(2, false);
     }
+    mark(3, true);
     k_spmv_alpha<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.ph, K.rw, K.v);
+    mark(3, false);
     fin<kFinAlpha>(K, st);
     k_bicg_s<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.r, K.v, K.sv);
     {
@@ -718,7 +730,7 @@ hipError_t copyProbe(int device, size_t bytes, int reps, double *ms)
     template hipError_t launchSetup<T>(const MatView<T> &, const Krylov<T> &, bool, hipStream_t);                    \
     template hipError_t launchPrecondition<T>(const DiluView<T> &, const Krylov<T> &, const T *, T *, hipStream_t);   \
     template hipError_t launchBicgIteration<T>(const MatView<T> &, const DiluView<T> &, const Krylov<T> &,           \
-                                               hipStream_t);                                                         \
+                                               hipStream_t, const IterProbe *);                                      \
     template hipError_t launchPcgIteration<T>(const MatView<T> &, const DiluView<T> &, const Krylov<T> &, hipStream_t);
 
 MRFG_INSTANTIATE(double)

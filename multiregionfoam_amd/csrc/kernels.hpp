@@ -30,8 +30,15 @@ template <typename T>
 hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st);
 template <typename T>
 hipError_t launchPrecondition(const DiluView<T> &D, const Krylov<T> &K, const T *in, T *out, hipStream_t st);
+// Events recorded around one launch of a BiCGStab iteration (timing only).
+struct IterProbe {
+    int which;              // 1 = first forward sweep, 2 = first backward sweep, 3 = SpMV with the alpha sum
+    hipEvent_t begin, end;
+};
+
 template <typename T>
-hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st);
+hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st,
+                               const IterProbe *probe = nullptr);
 template <typename T>
 hipError_t launchPcgIteration(const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K, hipStream_t st);
 

@@ -245,6 +245,22 @@ def test_fvsolution_dictionary_selects_solver(gpu):
     assert abs(perf.n_iterations - perf_ref["nIterations"]) <= 2
 
 
+def test_kernel_timing_inside_iterations(gpu):
+    """Timing one launch per BiCGStab iteration (modes 4-6) runs real
+    iterations from the initial guess and restores it: a solve afterwards
+    gives the same result as before."""
+    case = small_cases()["duct_tiles"]
+    A = _ctx(gpu, case)
+    p0 = A.solve(tolerance=1e-10, maxIter=500)[0]
+    psi0 = np.concatenate(A.psi(0))
+    for which in (1, 2, 4, 5, 6):
+        assert A.time_kernel(which, 5) > 0.0
+    A.reset_psi()
+    p1 = A.solve(tolerance=1e-10, maxIter=500)[0]
+    assert p1.n_iterations == p0.n_iterations and p1.final_residual == p0.final_residual
+    assert np.array_equal(np.concatenate(A.psi(0)), psi0)
+
+
 def test_full_size_duct_matches_oracle(gpu):
     """1.4M-cell GGI duct (the benchmark workload): SpMV bit-exact at full
     size, and the FP64 solve identical to the oracle's (iterations, residuals
