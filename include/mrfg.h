@@ -168,6 +168,21 @@ int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int 
 int mrfg_case_interface(const mrfg_case *c, int iface,
                         int32_t *fc_a, double *coeff_a, int32_t *off_a, int32_t *idx_a, double *w_a,
                         int32_t *fc_b, double *coeff_b, int32_t *off_b, int32_t *idx_b, double *w_b);
+/* Read an OpenFOAM / foam-extend multi-region case directory and assemble
+   it like mrfg_case_build: constant/multiRegionProperties (region types),
+   constant/<region>/polyMesh (ASCII points, faces, owner, neighbour,
+   boundary), 0/<region>/{T,k,U}, constant/<region>/transportProperties,
+   constant/regionInterfaceProperties (monolithicCoupledPatches),
+   system/controlDict (deltaT) and system/<region>/fvSchemes (ddt), the
+   layout of tutorials/conjugateHeatTransfer/flowOverHeatedPlate.  `options`
+   is dictionary text (may be empty): fieldDir, interfaceProperties, ddt,
+   deltaT.  The equations are those of setCoupledEqns
+   (conductTemperature.C:135-152, transportTemperature.C:129-150) with
+   phi = interpolate(U) & Sf (transportTemperature.C:86-94); a `backward`
+   ddt is taken at its first step (Euler).  Meshes must be orthogonal to
+   1 degree, coupled patches planar, axis-normal, of axis-aligned
+   rectangular faces. */
+int mrfg_case_read_foam(const char *case_dir, const char *options, mrfg_case **out, char *err, int err_len);
 /* Add a built case as a new system of the context (begin_system +
    set_region + set_interface). */
 int mrfg_add_case(mrfg_ctx *ctx, const mrfg_case *c, int *sys_id);

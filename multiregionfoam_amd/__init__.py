@@ -87,6 +87,7 @@ def lib():
         "mrfg_set_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_case_build": ([C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int], C.c_int),
         "mrfg_case_free": ([vp], None),
+        "mrfg_case_read_foam": ([C.c_char_p, C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int], C.c_int),
         "mrfg_case_sizes": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "mrfg_case_region_sizes": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
                                    C.c_int),
@@ -125,7 +126,7 @@ def lib():
 EXPORTED = (
     "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region "
     "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
-    "mrfg_set_psi mrfg_case_build mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
+    "mrfg_set_psi mrfg_case_build mrfg_case_read_foam mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
@@ -194,9 +195,13 @@ class SolverPerformance:
 # ---------------------------------------------------------------------------
 
 class Case:
-    """A case dictionary assembled by the native case builder."""
+    """A case dictionary assembled by the native case builder, or (through
+    ``Case.from_foam``) an OpenFOAM case directory read from disk."""
 
-    def __init__(self, case):
+    def __init__(self, case=None, _handle=None):
+        if _handle is not None:
+            self._h = _handle
+            return
         text = case if isinstance(case, str) else cases.to_dict_text(case)
         h = C.c_void_p()
         err = C.create_string_buffer(1024)
@@ -204,6 +209,23 @@ class Case:
         if rc != MRFG_OK:
             raise MrfgError(rc, err.value.decode())
         self._h = h
+
+    @classmethod
+    def from_foam(cls, case_dir, fieldDir=None, interfaceProperties=None, ddt=None, deltaT=None):
+        """Read and assemble a multi-region case directory (polyMesh, 0/,
+        constant/, system/), as multiRegionFoam sets up its coupled T
+        equation; keyword options override what the case files select."""
+        opts = []
+        for k, v in (("fieldDir", fieldDir), ("interfaceProperties", interfaceProperties), ("ddt", ddt),
+                     ("deltaT", deltaT)):
+            if v is not None:
+                opts.append("%s %s;" % (k, repr(float(v)) if k == "deltaT" else v))
+        h = C.c_void_p()
+        err = C.create_string_buffer(2048)
+        rc = lib().mrfg_case_read_foam(_text(os.fspath(case_dir)), _text(" ".join(opts)), C.byref(h), err, 2048)
+        if rc != MRFG_OK:
+            raise MrfgError(rc, err.value.decode())
+        return cls(_handle=h)
 
     def __del__(self):
         try:

@@ -346,6 +346,21 @@ int mrfg_case_build(const char *dict_text, mrfg_case **out, char *err, int err_l
     return MRFG_OK;
 }
 
+int mrfg_case_read_foam(const char *case_dir, const char *options, mrfg_case **out, char *err, int err_len)
+{
+    if (!case_dir || !out) return MRFG_ERR_ARG;
+    *out = nullptr;
+    mrfg_ctx tmp;
+    auto c = std::make_unique<mrfg_case>();
+    int rc = guard(&tmp, [&] { c->sys = mrfg::readFoamCase(case_dir, options ? options : ""); });
+    if (rc != MRFG_OK) {
+        copyErr(tmp.err, err, err_len);
+        return rc;
+    }
+    *out = c.release();
+    return MRFG_OK;
+}
+
 void mrfg_case_free(mrfg_case *c) { delete c; }
 
 int mrfg_case_sizes(const mrfg_case *c, int *n_regions, int *n_interfaces)

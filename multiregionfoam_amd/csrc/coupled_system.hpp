@@ -75,6 +75,18 @@ struct CaseExtras {
 
 CoupledSystem buildCase(const std::string &dictText, CaseExtras *extras = nullptr);
 
+// Read a multi-region case from disk (ASCII polyMesh, 0/ fields, constant/
+// and system/ dictionaries) and assemble its coupled system.  `options` is
+// dictionary text: fieldDir (sub-directory of 0/<region>/ holding T and k),
+// interfaceProperties (file under constant/), ddt (steadyState | Euler |
+// backward, overriding fvSchemes) and deltaT (overriding controlDict).
+struct FoamCaseInfo {
+    std::vector<std::vector<double>> volumes, k;   // per region, per cell
+    std::vector<double> maxNonOrthDeg, rDeltaT, rhoCp;   // per region
+};
+
+CoupledSystem readFoamCase(const std::string &caseDir, const std::string &options, FoamCaseInfo *info = nullptr);
+
 // splitmix64 hash -> uniform [0,1)
 double hashU01(uint64_t seed, uint64_t index);
 

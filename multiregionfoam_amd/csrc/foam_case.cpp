@@ -1,0 +1,1019 @@
+// Reading an OpenFOAM / foam-extend multi-region case from disk: ASCII
+// polyMesh, the 0/ fields and the constant/ and system/ dictionaries, then
+// the finite-volume assembly of the temperature equations on that general
+// (polyhedral, orthogonal) mesh into the coupled block system.
+//
+// Follows the reference's case layout and region types:
+//   constant/multiRegionProperties        regions and their region types
+//   constant/regionInterfaceProperties    monolithicCoupledPatches
+//       (tutorials/conjugateHeatTransfer/flowOverHeatedPlate/constant/*)
+//   conductTemperature::setCoupledEqns    (conductTemperature.C:135-152)
+//   transportTemperature::setCoupledEqns  (transportTemperature.C:129-150),
+//       phi = linearInterpolate(U) & Sf   (transportTemperature.C:86-94)
+//   monolithicThermalDiffusivity::calcThermalDiffusivity
+//                          (monolithicThermalDiffusivityFvPatchScalarField.C:167-277)
+//   directMap face matching (directMapInterfaceToInterfaceMapping.C:150-175)
+// Geometry, interpolation weights and delta coefficients follow foam-extend's
+// primitiveMesh / surfaceInterpolation definitions; the first time step of a
+// `backward` ddt scheme is the Euler step.
+//
+// Limits (each raises an error): internal faces must be in upper-triangular
+// order; non-orthogonality above kMaxNonOrthDeg (the explicit non-orthogonal
+// correction of `corrected` snGrad is not assembled); coupled patches must be
+// planar and normal to a coordinate axis with faces that are axis-aligned
+// rectangles (blockMesh hex meshes), which makes the GGI overlaps exact.
+#include "coupled_system.hpp"
+#include "dictionary.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cerrno>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace mrfg {
+
+namespace {
+
+constexpr double kMaxNonOrthDeg = 1.0;
+
+using Vec = std::array<double, 3>;
+
+inline Vec operator+(const Vec &a, const Vec &b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+inline Vec operator-(const Vec &a, const Vec &b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline Vec operator*(double s, const Vec &a) { return {s * a[0], s * a[1], s * a[2]}; }
+inline double dot(const Vec &a, const Vec &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline Vec cross(const Vec &a, const Vec &b)
+{
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+inline double mag(const Vec &a) { return std::sqrt(dot(a, a)); }
+
+struct FoamError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+std::string readFile(const std::string &path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw FoamError("cannot open '" + path + "'");
+    std::ostringstream s;
+    s << f.rdbuf();
+    return s.str();
+}
+
+bool fileExists(const std::string &path)
+{
+    std::ifstream f(path);
+    return (bool)f;
+}
+
+// ---------------------------------------------------------------------------
+// polyMesh list files: FoamFile header, then "N ( item ... )"
+// ---------------------------------------------------------------------------
+class ListReader {
+public:
+    ListReader(const std::string &path) : path_(path), text_(readFile(path))
+    {
+        p_ = text_.c_str();
+        e_ = p_ + text_.size();
+        skipHeader();
+    }
+    int count()
+    {
+        long n = integer();
+        if (n < 0) fail("negative list size");
+        expect('(');
+        return (int)n;
+    }
+    long integer()
+    {
+        skip();
+        char *end;
+        errno = 0;
+        long v = std::strtol(p_, &end, 10);
+        if (end == p_ || errno) fail("integer expected");
+        p_ = end;
+        return v;
+    }
+    double real()
+    {
+        skip();
+        char *end;
+        double v = std::strtod(p_, &end);
+        if (end == p_) fail("number expected");
+        p_ = end;
+        return v;
+    }
+    void expect(char c)
+    {
+        skip();
+        if (p_ >= e_ || *p_ != c) fail(std::string("'") + c + "' expected");
+        ++p_;
+    }
+
+private:
+    [[noreturn]] void fail(const std::string &what)
+    {
+        throw FoamError(path_ + ": " + what + " at byte " + std::to_string(p_ - text_.c_str()));
+    }
+    void skip()
+    {
+        for (;;) {
+            while (p_ < e_ && (*p_ == ' ' || *p_ == '\t' || *p_ == '\n' || *p_ == '\r')) ++p_;
+            if (p_ + 1 < e_ && p_[0] == '/' && p_[1] == '/') {
+                while (p_ < e_ && *p_ != '\n') ++p_;
+            } else if (p_ + 1 < e_ && p_[0] == '/' && p_[1] == '*') {
+                const char *q = std::strstr(p_ + 2, "*/");
+                p_ = q ? q + 2 : e_;
+            } else {
+                return;
+            }
+        }
+    }
+    void skipHeader()
+    {
+        skip();
+        if (e_ - p_ >= 8 && std::strncmp(p_, "FoamFile", 8) == 0) {
+            const char *q = std::strchr(p_, '}');
+            if (!q) fail("unterminated FoamFile header");
+            p_ = q + 1;
+        }
+    }
+    std::string path_, text_;
+    const char *p_, *e_;
+};
+
+// ---------------------------------------------------------------------------
+// Mesh: points, faces, owner / neighbour, patches, and the derived geometry
+// ---------------------------------------------------------------------------
+struct Patch {
+    std::string name, type;
+    int start = 0, size = 0;
+};
+
+struct PolyMesh {
+    std::string name;
+    std::vector<Vec> points;
+    std::vector<int32_t> faceOff, faceVerts;
+    std::vector<int32_t> owner, neighbour;   // neighbour: internal faces only
+    std::vector<Patch> patches;
+    int nCells = 0, nInternal = 0;
+    // geometry
+    std::vector<Vec> Cf, Sf, C;
+    std::vector<double> magSf, V;
+    std::vector<double> w, deltaCoeffs;      // internal faces
+    double maxNonOrthDeg = 0.0;
+
+    int nFaces() const { return (int)owner.size(); }
+    const Patch &patch(const std::string &n) const
+    {
+        for (auto &p : patches)
+            if (p.name == n) return p;
+        throw FoamError("region '" + name + "': no patch '" + n + "'");
+    }
+    // unit normal and owner-centre-to-face normal distance of boundary face f
+    double nDist(int f) const { return dot((1.0 / magSf[f]) * Sf[f], Cf[f] - C[owner[f]]); }
+};
+
+void readPolyMesh(const std::string &dir, PolyMesh &m)
+{
+    {
+        ListReader r(dir + "/points");
+        int n = r.count();
+        m.points.resize(n);
+        for (int i = 0; i < n; ++i) {
+            r.expect('(');
+            for (int a = 0; a < 3; ++a) m.points[i][a] = r.real();
+            r.expect(')');
+        }
+        r.expect(')');
+    }
+    {
+        ListReader r(dir + "/faces");
+        int n = r.count();
+        m.faceOff.assign(1, 0);
+        m.faceOff.reserve(n + 1);
+        m.faceVerts.reserve((size_t)n * 4);
+        for (int i = 0; i < n; ++i) {
+            long k = r.integer();
+            if (k < 3) throw FoamError(dir + "/faces: face " + std::to_string(i) + " has fewer than 3 points");
+            r.expect('(');
+            for (long j = 0; j < k; ++j) {
+                long v = r.integer();
+                if (v < 0 || v >= (long)m.points.size())
+                    throw FoamError(dir + "/faces: point index out of range in face " + std::to_string(i));
+                m.faceVerts.push_back((int32_t)v);
+            }
+            r.expect(')');
+            m.faceOff.push_back((int32_t)m.faceVerts.size());
+        }
+        r.expect(')');
+    }
+    auto labels = [&](const std::string &file, std::vector<int32_t> &out) {
+        ListReader r(dir + "/" + file);
+        int n = r.count();
+        out.resize(n);
+        for (int i = 0; i < n; ++i) out[i] = (int32_t)r.integer();
+        r.expect(')');
+    };
+    labels("owner", m.owner);
+    labels("neighbour", m.neighbour);
+    const int nF = (int)m.faceOff.size() - 1;
+    if ((int)m.owner.size() != nF) throw FoamError(dir + ": owner and faces differ in size");
+    m.nInternal = (int)m.neighbour.size();
+    m.nCells = 0;
+    for (int32_t c : m.owner) m.nCells = std::max(m.nCells, c + 1);
+    for (int32_t c : m.neighbour) m.nCells = std::max(m.nCells, c + 1);
+    for (int f = 0; f < m.nInternal; ++f) {
+        if (m.owner[f] >= m.neighbour[f])
+            throw FoamError(dir + ": internal face " + std::to_string(f) + " has owner >= neighbour");
+        if (f > 0 && (m.owner[f - 1] > m.owner[f] ||
+                      (m.owner[f - 1] == m.owner[f] && m.neighbour[f - 1] >= m.neighbour[f])))
+            throw FoamError(dir + ": internal faces are not in upper-triangular order");
+    }
+    // boundary: "N ( name { type ..; nFaces ..; startFace ..; } ... )"
+    std::string b = readFile(dir + "/boundary");
+    size_t h = b.find("FoamFile");
+    if (h != std::string::npos) {
+        size_t e = b.find('}', h);
+        if (e == std::string::npos) throw FoamError(dir + "/boundary: unterminated FoamFile header");
+        b = b.substr(e + 1);
+    }
+    Dict bd = parseDict("patches " + b + ";");
+    int next = m.nInternal;
+    for (const Value &v : bd.lookup("patches").tokens) {
+        if (!v.isList()) continue;
+        for (const Value &it : v.items) {
+            if (!it.isDict()) continue;
+            Patch p;
+            p.name = it.dict->name;
+            p.type = it.dict->word("type");
+            p.size = (int)it.dict->number("nFaces");
+            p.start = (int)it.dict->number("startFace");
+            if (p.start != next) throw FoamError(dir + "/boundary: patch '" + p.name + "' does not follow the previous one");
+            next += p.size;
+            m.patches.push_back(p);
+        }
+    }
+    if (next != nF) throw FoamError(dir + "/boundary: patches do not cover the boundary faces");
+}
+
+// primitiveMesh face centres / areas (triangle fan about the point average)
+// and cell centres / volumes (pyramids about the face-centre average).
+void computeGeometry(PolyMesh &m)
+{
+    const int nF = m.nFaces();
+    m.Cf.resize(nF);
+    m.Sf.resize(nF);
+    m.magSf.resize(nF);
+    for (int f = 0; f < nF; ++f) {
+        const int32_t *v = &m.faceVerts[m.faceOff[f]];
+        const int n = m.faceOff[f + 1] - m.faceOff[f];
+        if (n == 3) {
+            const Vec &a = m.points[v[0]], &b = m.points[v[1]], &c = m.points[v[2]];
+            m.Cf[f] = (1.0 / 3.0) * (a + b + c);
+            m.Sf[f] = 0.5 * cross(b - a, c - a);
+        } else {
+            Vec fc{0, 0, 0};
+            for (int i = 0; i < n; ++i) fc = fc + m.points[v[i]];
+            fc = (1.0 / n) * fc;
+            Vec sumN{0, 0, 0}, sumAc{0, 0, 0};
+            double sumA = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const Vec &p = m.points[v[i]], &q = m.points[v[(i + 1) % n]];
+                Vec c = p + q + fc;
+                Vec nt = cross(q - p, fc - p);
+                double a = mag(nt);
+                sumN = sumN + nt;
+                sumA += a;
+                sumAc = sumAc + a * c;
+            }
+            m.Cf[f] = sumA > 0.0 ? (1.0 / (3.0 * sumA)) * sumAc : fc;
+            m.Sf[f] = 0.5 * sumN;
+        }
+        m.magSf[f] = mag(m.Sf[f]);
+        if (!(m.magSf[f] > 0.0)) throw FoamError("region '" + m.name + "': face " + std::to_string(f) + " has zero area");
+    }
+    std::vector<Vec> cEst(m.nCells, Vec{0, 0, 0});
+    std::vector<int> nCf(m.nCells, 0);
+    for (int f = 0; f < nF; ++f) {
+        cEst[m.owner[f]] = cEst[m.owner[f]] + m.Cf[f];
+        ++nCf[m.owner[f]];
+        if (f < m.nInternal) {
+            cEst[m.neighbour[f]] = cEst[m.neighbour[f]] + m.Cf[f];
+            ++nCf[m.neighbour[f]];
+        }
+    }
+    for (int c = 0; c < m.nCells; ++c) {
+        if (nCf[c] < 4) throw FoamError("region '" + m.name + "': cell " + std::to_string(c) + " has fewer than 4 faces");
+        cEst[c] = (1.0 / nCf[c]) * cEst[c];
+    }
+    std::vector<Vec> ctr(m.nCells, Vec{0, 0, 0});
+    std::vector<double> vol(m.nCells, 0.0);
+    auto pyramid = [&](int c, int f, double sign) {
+        double pv = sign * dot(m.Sf[f], m.Cf[f] - cEst[c]);
+        Vec pc = 0.75 * m.Cf[f] + 0.25 * cEst[c];
+        ctr[c] = ctr[c] + pv * pc;
+        vol[c] += pv;
+    };
+    for (int f = 0; f < nF; ++f) {
+        pyramid(m.owner[f], f, 1.0);
+        if (f < m.nInternal) pyramid(m.neighbour[f], f, -1.0);
+    }
+    m.C.resize(m.nCells);
+    m.V.resize(m.nCells);
+    for (int c = 0; c < m.nCells; ++c) {
+        if (!(vol[c] > 0.0)) throw FoamError("region '" + m.name + "': cell " + std::to_string(c) + " has non-positive volume");
+        m.C[c] = (1.0 / vol[c]) * ctr[c];
+        m.V[c] = vol[c] / 3.0;
+    }
+    // surfaceInterpolation: linear weights and (non-orthogonal) delta coefficients
+    m.w.resize(m.nInternal);
+    m.deltaCoeffs.resize(m.nInternal);
+    double minCos = 1.0;
+    for (int f = 0; f < m.nInternal; ++f) {
+        const Vec &co = m.C[m.owner[f]], &cn = m.C[m.neighbour[f]];
+        double sfdOwn = std::fabs(dot(m.Sf[f], m.Cf[f] - co));
+        double sfdNei = std::fabs(dot(m.Sf[f], cn - m.Cf[f]));
+        m.w[f] = sfdNei / (sfdOwn + sfdNei);
+        Vec delta = cn - co;
+        Vec unit = (1.0 / m.magSf[f]) * m.Sf[f];
+        double nd = dot(unit, delta);
+        m.deltaCoeffs[f] = 1.0 / std::fmax(nd, 0.05 * mag(delta));
+        minCos = std::fmin(minCos, nd / mag(delta));
+    }
+    m.maxNonOrthDeg = std::acos(std::fmax(-1.0, std::fmin(1.0, minCos))) * 180.0 / M_PI;
+    if (m.maxNonOrthDeg > kMaxNonOrthDeg)
+        throw FoamError("region '" + m.name + "': mesh non-orthogonality " + std::to_string(m.maxNonOrthDeg) +
+                        " deg exceeds the supported " + std::to_string(kMaxNonOrthDeg) +
+                        " deg (no explicit non-orthogonal correction)");
+}
+
+// ---------------------------------------------------------------------------
+// Fields: internalField and per-patch boundary conditions
+// ---------------------------------------------------------------------------
+struct PatchBC {
+    std::string type;
+    std::vector<double> value;   // per face (scalar) or 3 per face (vector)
+};
+
+struct FieldFile {
+    std::vector<double> internal;              // 1 or 3 per cell
+    std::map<std::string, PatchBC> patches;
+};
+
+const Value *lastOf(const std::vector<Value> &t, Value::Kind k)
+{
+    for (size_t i = t.size(); i-- > 0;)
+        if (t[i].kind == k) return &t[i];
+    return nullptr;
+}
+
+// "uniform X" / "uniform (x y z)" / "nonuniform List<..> N (...)" / "$internalField"
+std::vector<double> fieldValues(const Dict &top, const std::vector<Value> &tok, int n, int nComp,
+                                const std::string &what)
+{
+    std::vector<double> out;
+    if (tok.size() == 1 && tok[0].isWord() && tok[0].asWord().size() > 1 && tok[0].asWord()[0] == '$')
+        return fieldValues(top, top.lookup(tok[0].asWord().substr(1)).tokens, n, nComp, what);
+    if (tok.empty() || !tok[0].isWord()) throw FoamError(what + ": value expected");
+    const std::string &kind = tok[0].asWord();
+    auto comp = [&](const Value &v, std::vector<double> &dst) {
+        if (nComp == 1) {
+            dst.push_back(v.asNumber());
+        } else {
+            if (!v.isList() || (int)v.items.size() != nComp) throw FoamError(what + ": vector value expected");
+            for (const Value &x : v.items) dst.push_back(x.asNumber());
+        }
+    };
+    if (kind == "uniform") {
+        if (tok.size() < 2) throw FoamError(what + ": uniform value missing");
+        std::vector<double> one;
+        comp(tok[1], one);
+        out.reserve((size_t)n * nComp);
+        for (int i = 0; i < n; ++i) out.insert(out.end(), one.begin(), one.end());
+    } else if (kind == "nonuniform") {
+        const Value *l = lastOf(tok, Value::List);
+        if (!l) throw FoamError(what + ": nonuniform list missing");
+        if ((int)l->items.size() != n) throw FoamError(what + ": nonuniform list has the wrong size");
+        for (const Value &v : l->items) comp(v, out);
+    } else {
+        throw FoamError(what + ": unsupported value form '" + kind + "'");
+    }
+    return out;
+}
+
+FieldFile readField(const std::string &path, const PolyMesh &m, int nComp)
+{
+    Dict d = parseDict(readFile(path));
+    FieldFile F;
+    F.internal = fieldValues(d, d.lookup("internalField").tokens, m.nCells, nComp, path + " internalField");
+    const Dict &bf = d.subDict("boundaryField");
+    for (const Patch &p : m.patches) {
+        PatchBC bc;
+        const Dict *pd = nullptr;
+        for (auto &e : bf.entries)
+            if (e.dict && e.key == p.name) pd = e.dict.get();
+        if (!pd) throw FoamError(path + ": no boundary condition for patch '" + p.name + "'");
+        bc.type = pd->word("type");
+        if (pd->found("value") && bc.type != "empty")
+            bc.value = fieldValues(d, pd->lookup("value").tokens, p.size, nComp, path + " patch " + p.name);
+        F.patches[p.name] = bc;
+    }
+    return F;
+}
+
+// the numeric value of a transportProperties entry "name [dims] value;" or
+// "name name [dims] value;"
+double property(const Dict &d, const std::string &key, const std::string &file)
+{
+    const Value *v = lastOf(d.lookup(key).tokens, Value::Number);
+    if (!v) throw FoamError(file + ": no value for '" + key + "'");
+    return v->number;
+}
+
+// ---------------------------------------------------------------------------
+// Regions and assembly
+// ---------------------------------------------------------------------------
+enum class BcT { FixedValue, ZeroGradient, Empty, Coupled };
+
+struct FoamRegion {
+    PolyMesh mesh;
+    bool transport = false;
+    double rhoCp = 0.0;                 // rho*cp (transport) / rho*cv (conduct)
+    std::vector<double> T, k;           // cell values
+    std::vector<BcT> bcT;               // per patch
+    std::vector<std::vector<double>> Tb, kb;   // per patch face values (fixedValue T; k)
+    std::vector<double> phi;            // internal faces
+    std::vector<std::vector<double>> phiB;     // per patch
+    // coupled patches: face conductivity and delta coefficients, and the
+    // boundary coefficients handed to the interface
+    std::map<int, std::vector<double>> coupledK, coupledDelta, coupledCoeff;
+};
+
+struct FvMatrixG {
+    std::vector<double> diag, upper, lower, source;
+    bool asymmetric = false;
+    std::vector<std::vector<double>> internalCoeffs, boundaryCoeffs;   // per patch
+    void init(const PolyMesh &m, bool asym)
+    {
+        asymmetric = asym;
+        diag.assign(m.nCells, 0.0);
+        upper.assign(m.nInternal, 0.0);
+        if (asym) lower.assign(m.nInternal, 0.0);
+        source.assign(m.nCells, 0.0);
+        internalCoeffs.resize(m.patches.size());
+        boundaryCoeffs.resize(m.patches.size());
+        for (size_t p = 0; p < m.patches.size(); ++p) {
+            internalCoeffs[p].assign(m.patches[p].size, 0.0);
+            boundaryCoeffs[p].assign(m.patches[p].size, 0.0);
+        }
+    }
+    void negSumDiag(const PolyMesh &m)
+    {
+        const std::vector<double> &lo = asymmetric ? lower : upper;
+        for (int f = 0; f < m.nInternal; ++f) {
+            diag[m.owner[f]] -= lo[f];
+            diag[m.neighbour[f]] -= upper[f];
+        }
+    }
+};
+
+FvMatrixG laplacian(const FoamRegion &R)
+{
+    const PolyMesh &m = R.mesh;
+    FvMatrixG M;
+    M.init(m, false);
+    for (int f = 0; f < m.nInternal; ++f) {
+        double kf = m.w[f] * R.k[m.owner[f]] + (1.0 - m.w[f]) * R.k[m.neighbour[f]];
+        M.upper[f] = m.deltaCoeffs[f] * (kf * m.magSf[f]);
+    }
+    M.negSumDiag(m);
+    for (size_t p = 0; p < m.patches.size(); ++p) {
+        const Patch &P = m.patches[p];
+        for (int i = 0; i < P.size; ++i) {
+            const int f = P.start + i;
+            switch (R.bcT[p]) {
+            case BcT::FixedValue: {
+                double deltab = 1.0 / m.nDist(f);
+                double gammaMagSf = R.kb[p][i] * m.magSf[f];
+                M.internalCoeffs[p][i] = gammaMagSf * (-deltab);
+                M.boundaryCoeffs[p][i] = -gammaMagSf * (deltab * R.Tb[p][i]);
+                break;
+            }
+            case BcT::Coupled: {
+                double gammaMagSf = R.coupledK.at((int)p)[i] * m.magSf[f];
+                double delta = R.coupledDelta.at((int)p)[i];
+                M.internalCoeffs[p][i] = gammaMagSf * (-delta);
+                M.boundaryCoeffs[p][i] = -gammaMagSf * delta;
+                break;
+            }
+            default:
+                break;
+            }
+        }
+    }
+    return M;
+}
+
+FvMatrixG divUpwind(const FoamRegion &R)
+{
+    const PolyMesh &m = R.mesh;
+    FvMatrixG M;
+    M.init(m, true);
+    for (int f = 0; f < m.nInternal; ++f) {
+        double w = R.phi[f] >= 0.0 ? 1.0 : 0.0;
+        M.lower[f] = -w * R.phi[f];
+        M.upper[f] = M.lower[f] + R.phi[f];
+    }
+    M.negSumDiag(m);
+    for (size_t p = 0; p < m.patches.size(); ++p) {
+        for (int i = 0; i < m.patches[p].size; ++i) {
+            double ph = R.phiB[p][i];
+            double w = ph >= 0.0 ? 1.0 : 0.0;
+            switch (R.bcT[p]) {
+            case BcT::FixedValue:
+                M.internalCoeffs[p][i] = ph * 0.0;
+                M.boundaryCoeffs[p][i] = -ph * R.Tb[p][i];
+                break;
+            case BcT::Coupled:
+                M.internalCoeffs[p][i] = ph * w;
+                M.boundaryCoeffs[p][i] = -ph * (1.0 - w);
+                break;
+            case BcT::ZeroGradient:
+                M.internalCoeffs[p][i] = ph * 1.0;
+                M.boundaryCoeffs[p][i] = -ph * 0.0;
+                break;
+            case BcT::Empty:
+                break;
+            }
+        }
+    }
+    return M;
+}
+
+// Euler ddt with coefficient rho (1 for fvm::ddt(T)): diag = rho V / dt,
+// source = rho V T_old / dt; steadyState adds nothing.
+void addDdt(const FoamRegion &R, double rDeltaT, double rho, FvMatrixG &M)
+{
+    const PolyMesh &m = R.mesh;
+    for (int c = 0; c < m.nCells; ++c) {
+        M.diag[c] += rDeltaT * rho * m.V[c];
+        M.source[c] += rDeltaT * rho * R.T[c] * m.V[c];
+    }
+}
+
+struct Rect {
+    double lo[2], hi[2];
+};
+
+// a planar patch normal to axis `axis`: each face's rectangle in the two
+// tangential coordinates (faces must be axis-aligned rectangles)
+std::vector<Rect> patchRects(const PolyMesh &m, const Patch &P, int &axis)
+{
+    Vec sum{0, 0, 0};
+    double area = 0.0;
+    for (int i = 0; i < P.size; ++i) {
+        sum = sum + m.Sf[P.start + i];
+        area += m.magSf[P.start + i];
+    }
+    axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (std::fabs(sum[a]) > std::fabs(sum[axis])) axis = a;
+    if (std::fabs(sum[axis]) < (1.0 - 1e-9) * area)
+        throw FoamError("patch '" + P.name + "' of region '" + m.name + "' is not planar and axis-normal");
+    const int b = axis == 0 ? 1 : 0, c = axis == 2 ? 1 : 2;
+    std::vector<Rect> r(P.size);
+    for (int i = 0; i < P.size; ++i) {
+        const int f = P.start + i;
+        Rect q{{1e300, 1e300}, {-1e300, -1e300}};
+        for (int j = m.faceOff[f]; j < m.faceOff[f + 1]; ++j) {
+            const Vec &p = m.points[m.faceVerts[j]];
+            q.lo[0] = std::fmin(q.lo[0], p[b]);
+            q.hi[0] = std::fmax(q.hi[0], p[b]);
+            q.lo[1] = std::fmin(q.lo[1], p[c]);
+            q.hi[1] = std::fmax(q.hi[1], p[c]);
+        }
+        double ra = (q.hi[0] - q.lo[0]) * (q.hi[1] - q.lo[1]);
+        if (std::fabs(ra - m.magSf[f]) > 1e-9 * m.magSf[f])
+            throw FoamError("patch '" + P.name + "' of region '" + m.name + "': face " + std::to_string(i) +
+                            " is not an axis-aligned rectangle");
+        r[i] = q;
+    }
+    return r;
+}
+
+// GGI weights of side A's faces gathering side B's faces: exact rectangle
+// overlap areas rescaled to sum 1 per covered face; overlaps below 1e-10 of
+// the smaller face are dropped.  A bucket grid over B keeps the search local.
+void ggiWeights(const std::vector<Rect> &A, const std::vector<Rect> &B, InterfaceSide &S)
+{
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300}, size = 0.0;
+    for (const Rect &r : B)
+        for (int d = 0; d < 2; ++d) {
+            lo[d] = std::fmin(lo[d], r.lo[d]);
+            hi[d] = std::fmax(hi[d], r.hi[d]);
+        }
+    for (const Rect &r : B) size += std::sqrt((r.hi[0] - r.lo[0]) * (r.hi[1] - r.lo[1]));
+    size = B.empty() ? 1.0 : 2.0 * size / (double)B.size();
+    int nb[2];
+    for (int d = 0; d < 2; ++d) nb[d] = std::max(1, std::min(4096, (int)std::ceil((hi[d] - lo[d]) / size)));
+    auto bucket = [&](double x, int d) {
+        int i = (int)std::floor((x - lo[d]) / (hi[d] - lo[d] + 1e-300) * nb[d]);
+        return std::max(0, std::min(nb[d] - 1, i));
+    };
+    std::vector<std::vector<int>> grid((size_t)nb[0] * nb[1]);
+    for (int j = 0; j < (int)B.size(); ++j)
+        for (int y = bucket(B[j].lo[1], 1); y <= bucket(B[j].hi[1], 1); ++y)
+            for (int x = bucket(B[j].lo[0], 0); x <= bucket(B[j].hi[0], 0); ++x) grid[(size_t)y * nb[0] + x].push_back(j);
+    S.off.assign(A.size() + 1, 0);
+    S.idx.clear();
+    S.w.clear();
+    std::vector<int> cand;
+    for (size_t i = 0; i < A.size(); ++i) {
+        const Rect &a = A[i];
+        S.off[i] = (int32_t)S.w.size();
+        cand.clear();
+        for (int y = bucket(a.lo[1], 1); y <= bucket(a.hi[1], 1); ++y)
+            for (int x = bucket(a.lo[0], 0); x <= bucket(a.hi[0], 0); ++x)
+                for (int j : grid[(size_t)y * nb[0] + x]) cand.push_back(j);
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        const double areaA = (a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]);
+        const size_t p0 = S.w.size();
+        double sum = 0.0;
+        for (int j : cand) {
+            const Rect &b = B[j];
+            double ox = std::fmin(a.hi[0], b.hi[0]) - std::fmax(a.lo[0], b.lo[0]);
+            double oy = std::fmin(a.hi[1], b.hi[1]) - std::fmax(a.lo[1], b.lo[1]);
+            if (ox <= 0.0 || oy <= 0.0) continue;
+            double ar = ox * oy;
+            double areaB = (b.hi[0] - b.lo[0]) * (b.hi[1] - b.lo[1]);
+            if (ar <= 1e-10 * std::fmin(areaA, areaB)) continue;
+            S.idx.push_back(j);
+            S.w.push_back(ar);
+            sum += ar;
+        }
+        if (sum > 1e-12 * areaA) {
+            for (size_t q = p0; q < S.w.size(); ++q) S.w[q] = S.w[q] / sum;
+        } else {
+            S.idx.resize(p0);
+            S.w.resize(p0);
+        }
+    }
+    S.off[A.size()] = (int32_t)S.w.size();
+}
+
+// calcThermalDiffusivity with radiation off, on general geometry: the owner
+// side's normal distance, the GGI-interpolated neighbour distance and k.
+void calcThermalDiffusivity(FoamRegion &own, int pOwn, const FoamRegion &nei, int pNei, InterfaceSide &S)
+{
+    const PolyMesh &mo = own.mesh, &mn = nei.mesh;
+    const Patch &Po = mo.patches[pOwn], &Pn = mn.patches[pNei];
+    std::vector<double> &kface = own.coupledK[pOwn];
+    std::vector<double> &delta = own.coupledDelta[pOwn];
+    kface.resize(Po.size);
+    delta.resize(Po.size);
+    S.fc.resize(Po.size);
+    S.magSf.resize(Po.size);
+    for (int i = 0; i < Po.size; ++i) {
+        const int f = Po.start + i;
+        const int cO = mo.owner[f];
+        S.fc[i] = cO;
+        S.magSf[i] = mo.magSf[f];
+        double dOwn = mo.nDist(f), dNei = 0.0, fNei = 0.0;
+        for (int p = S.off[i]; p < S.off[i + 1]; ++p) {
+            const int g = Pn.start + S.idx[p];
+            dNei += S.w[p] * mn.nDist(g);
+            fNei += S.w[p] * nei.k[mn.owner[g]];
+        }
+        if (S.off[i + 1] == S.off[i])
+            throw FoamError("coupled patch '" + Po.name + "' of region '" + mo.name + "': face " +
+                            std::to_string(i) + " has no partner face");
+        double fOwn = own.k[cO];
+        double magDelta = dOwn + dNei;       // magLongDelta
+        double wP = dNei / magDelta;         // patch weights
+        delta[i] = 1.0 / magDelta;           // patch deltaCoeffs
+        double kOwn = fOwn / (1.0 - wP) / magDelta;
+        double kNei = fNei / wP / magDelta;
+        double k = kOwn * kNei;
+        k /= delta[i] * (kOwn + kNei);
+        double kHarm = 1.0 / (wP / fOwn + (1.0 - wP) / fNei);
+        kface[i] = std::fmax(std::fmin(k, 100.0 * kHarm), 0.01 * kHarm);
+    }
+}
+
+struct Options {
+    std::string fieldDir;                    // under 0/<region>/ ("" = directly)
+    std::string interfaceFile;               // under constant/
+    std::string ddt;                         // "" = from fvSchemes
+    double deltaT = 0.0;                     // 0 = from controlDict
+};
+
+void readRegion(const std::string &dir, const std::string &name, const std::string &type, const Options &opt,
+                FoamRegion &R)
+{
+    PolyMesh &m = R.mesh;
+    m.name = name;
+    readPolyMesh(dir + "/constant/" + name + "/polyMesh", m);
+    computeGeometry(m);
+    R.transport = type == "transportTemperature";
+
+    std::string tp = dir + "/constant/" + name + "/transportProperties";
+    Dict props = parseDict(readFile(tp));
+    const double rho = property(props, "rho", tp);
+    R.rhoCp = rho * property(props, R.transport ? "cp" : "cv", tp);
+
+    std::string fdir = dir + "/0/" + name + "/";
+    if (!opt.fieldDir.empty()) fdir += opt.fieldDir + "/";
+    else if (!fileExists(fdir + "T") && fileExists(fdir + "orig/monolithic/T")) fdir += "orig/monolithic/";
+    FieldFile T = readField(fdir + "T", m, 1);
+    R.T = T.internal;
+    const size_t nP = m.patches.size();
+    R.bcT.resize(nP);
+    R.Tb.resize(nP);
+    R.kb.resize(nP);
+    for (size_t p = 0; p < nP; ++p) {
+        const Patch &P = m.patches[p];
+        const PatchBC &bc = T.patches.at(P.name);
+        if (bc.type == "fixedValue") {
+            R.bcT[p] = BcT::FixedValue;
+            if ((int)bc.value.size() != P.size) throw FoamError(fdir + "T: fixedValue patch '" + P.name + "' has no value");
+            R.Tb[p] = bc.value;
+        } else if (bc.type == "zeroGradient") {
+            R.bcT[p] = BcT::ZeroGradient;
+        } else if (bc.type == "empty") {
+            R.bcT[p] = BcT::Empty;
+        } else if (bc.type == "monolithicTemperature") {
+            R.bcT[p] = BcT::Coupled;
+        } else {
+            throw FoamError(fdir + "T: patch '" + P.name + "': unsupported type '" + bc.type + "'");
+        }
+        if ((R.bcT[p] == BcT::Empty) != (P.type == "empty"))
+            throw FoamError(fdir + "T: patch '" + P.name + "' is empty in only one of mesh and field");
+    }
+    // conductivity: 0/<region>/k, else transportProperties k (uniform)
+    if (fileExists(fdir + "k")) {
+        FieldFile K = readField(fdir + "k", m, 1);
+        R.k = K.internal;
+        for (size_t p = 0; p < nP; ++p) {
+            const Patch &P = m.patches[p];
+            const PatchBC &bc = K.patches.at(P.name);
+            R.kb[p].resize(P.size);
+            for (int i = 0; i < P.size; ++i)
+                R.kb[p][i] = bc.type == "fixedValue" ? bc.value.at(i) : R.k[m.owner[P.start + i]];
+        }
+    } else {
+        R.k.assign(m.nCells, property(props, "k", tp));
+        for (size_t p = 0; p < nP; ++p) {
+            const Patch &P = m.patches[p];
+            R.kb[p].resize(P.size);
+            for (int i = 0; i < P.size; ++i) R.kb[p][i] = R.k[m.owner[P.start + i]];
+        }
+    }
+    for (double x : R.k)
+        if (!(x > 0.0)) throw FoamError("region '" + name + "': conductivity must be positive");
+
+    // flux: phi = linearInterpolate(U) & Sf, boundary U_b & Sf
+    R.phiB.resize(nP);
+    if (R.transport) {
+        std::string up = dir + "/0/" + name + "/U";
+        FieldFile U = readField(up, m, 3);
+        auto Uc = [&](int c) { return Vec{U.internal[3 * c], U.internal[3 * c + 1], U.internal[3 * c + 2]}; };
+        R.phi.resize(m.nInternal);
+        for (int f = 0; f < m.nInternal; ++f) {
+            Vec uf = m.w[f] * Uc(m.owner[f]) + (1.0 - m.w[f]) * Uc(m.neighbour[f]);
+            R.phi[f] = dot(uf, m.Sf[f]);
+        }
+        for (size_t p = 0; p < nP; ++p) {
+            const Patch &P = m.patches[p];
+            const PatchBC &bc = U.patches.at(P.name);
+            R.phiB[p].assign(P.size, 0.0);
+            for (int i = 0; i < P.size; ++i) {
+                const int f = P.start + i;
+                Vec ub{0, 0, 0};
+                if (bc.type == "fixedValue") ub = Vec{bc.value.at(3 * i), bc.value.at(3 * i + 1), bc.value.at(3 * i + 2)};
+                else if (bc.type == "zeroGradient") ub = Uc(m.owner[f]);
+                else if (bc.type == "slip") {
+                    Vec n = (1.0 / m.magSf[f]) * m.Sf[f], u = Uc(m.owner[f]);
+                    ub = u - dot(u, n) * n;
+                } else if (bc.type != "noSlip" && bc.type != "empty")
+                    throw FoamError(up + ": patch '" + P.name + "': unsupported type '" + bc.type + "'");
+                R.phiB[p][i] = dot(ub, m.Sf[f]);
+            }
+        }
+    } else {
+        for (size_t p = 0; p < nP; ++p) R.phiB[p].assign(m.patches[p].size, 0.0);
+    }
+}
+
+// the region's TEqn, as setCoupledEqns builds it
+FvMatrixG setCoupledEqns(const FoamRegion &R, double rDeltaT)
+{
+    const PolyMesh &m = R.mesh;
+    FvMatrixG lap = laplacian(R);
+    FvMatrixG E;
+    E.init(m, R.transport);
+    if (R.transport) {
+        // rho*cp*(fvm::ddt(T) + fvm::div(phi,T)) == fvm::laplacian(k,T)
+        FvMatrixG A = divUpwind(R);
+        if (rDeltaT > 0.0) addDdt(R, rDeltaT, 1.0, A);
+        for (int c = 0; c < m.nCells; ++c) {
+            E.diag[c] = R.rhoCp * A.diag[c] - lap.diag[c];
+            E.source[c] = R.rhoCp * A.source[c] - lap.source[c];
+        }
+        for (int f = 0; f < m.nInternal; ++f) {
+            E.upper[f] = R.rhoCp * A.upper[f] - lap.upper[f];
+            E.lower[f] = R.rhoCp * A.lower[f] - lap.upper[f];
+        }
+        for (size_t p = 0; p < m.patches.size(); ++p)
+            for (int i = 0; i < m.patches[p].size; ++i) {
+                E.internalCoeffs[p][i] = R.rhoCp * A.internalCoeffs[p][i] - lap.internalCoeffs[p][i];
+                E.boundaryCoeffs[p][i] = R.rhoCp * A.boundaryCoeffs[p][i] - lap.boundaryCoeffs[p][i];
+            }
+    } else {
+        // fvm::ddt(rho*cv, T) == fvm::laplacian(k, T)
+        FvMatrixG D;
+        D.init(m, false);
+        if (rDeltaT > 0.0) addDdt(R, rDeltaT, R.rhoCp, D);
+        for (int c = 0; c < m.nCells; ++c) {
+            E.diag[c] = D.diag[c] - lap.diag[c];
+            E.source[c] = D.source[c] - lap.source[c];
+        }
+        for (int f = 0; f < m.nInternal; ++f) E.upper[f] = 0.0 - lap.upper[f];
+        for (size_t p = 0; p < m.patches.size(); ++p)
+            for (int i = 0; i < m.patches[p].size; ++i) {
+                E.internalCoeffs[p][i] = 0.0 - lap.internalCoeffs[p][i];
+                E.boundaryCoeffs[p][i] = 0.0 - lap.boundaryCoeffs[p][i];
+            }
+    }
+    return E;
+}
+
+// "regions ( (fluid (icoFluid transportTemperature)) (solid (conductTemperature)) );"
+std::vector<std::pair<std::string, std::string>> regionList(const std::string &dir)
+{
+    std::string f = dir + "/constant/multiRegionProperties";
+    Dict d = parseDict(readFile(f));
+    std::vector<std::pair<std::string, std::string>> out;
+    for (const Value &r : d.list("regions")) {
+        if (!r.isList() || r.items.size() < 2 || !r.items[0].isWord()) throw FoamError(f + ": bad regions entry");
+        std::string type;
+        const Value &types = r.items[1];
+        auto consider = [&](const Value &v) {
+            if (v.isWord() && (v.asWord() == "transportTemperature" || v.asWord() == "conductTemperature"))
+                type = v.asWord();
+        };
+        if (types.isList())
+            for (const Value &v : types.items) consider(v);
+        else
+            consider(types);
+        if (type.empty())
+            throw FoamError(f + ": region '" + r.items[0].asWord() +
+                            "' has no temperature region type (transportTemperature / conductTemperature)");
+        out.push_back({r.items[0].asWord(), type});
+    }
+    return out;
+}
+
+}  // namespace
+
+CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsText, FoamCaseInfo *info)
+{
+    Options opt;
+    {
+        Dict o = parseDict(optionsText);
+        opt.fieldDir = o.wordOrDefault("fieldDir", "");
+        opt.interfaceFile = o.wordOrDefault("interfaceProperties", "");
+        opt.ddt = o.wordOrDefault("ddt", "");
+        opt.deltaT = o.numberOrDefault("deltaT", 0.0);
+    }
+    auto regions = regionList(dir);
+    // time step: controlDict deltaT, ddt scheme of the first region's fvSchemes
+    double deltaT = opt.deltaT;
+    if (deltaT <= 0.0) deltaT = parseDict(readFile(dir + "/system/controlDict")).number("deltaT");
+    std::vector<FoamRegion> R(regions.size());
+    std::vector<double> rDeltaT(regions.size(), 0.0);
+    std::map<std::string, int> byName;
+    for (size_t r = 0; r < regions.size(); ++r) {
+        readRegion(dir, regions[r].first, regions[r].second, opt, R[r]);
+        byName[regions[r].first] = (int)r;
+        std::string ddt = opt.ddt;
+        if (ddt.empty()) {
+            std::string fs = dir + "/system/" + regions[r].first + "/fvSchemes";
+            if (!fileExists(fs)) fs = dir + "/system/fvSchemes";
+            ddt = parseDict(readFile(fs)).subDict("ddtSchemes").word("default");
+        }
+        if (ddt == "steadyState") rDeltaT[r] = 0.0;
+        else if (ddt == "Euler" || ddt == "backward") rDeltaT[r] = 1.0 / deltaT;   // backward: first step
+        else throw FoamError("ddt scheme '" + ddt + "' is not supported (steadyState, Euler, backward)");
+    }
+
+    // monolithic interfaces
+    std::string ifile = dir + "/constant/" + (opt.interfaceFile.empty() ? "regionInterfaceProperties" : opt.interfaceFile);
+    if (opt.interfaceFile.empty() && !fileExists(ifile)) ifile = dir + "/constant/regionInterfacePropertiesMonolithic";
+    CoupledSystem sys;
+    std::vector<std::array<int, 4>> sides;   // region A, patch A, region B, patch B
+    if (fileExists(ifile)) {
+        Dict d = parseDict(readFile(ifile));
+        if (d.found("monolithicCoupledPatches"))
+            for (const Value &it : d.list("monolithicCoupledPatches")) {
+                if (!it.isDict()) continue;
+                const auto &pp = it.dict->list("coupledPatchPair");
+                if (pp.size() != 2 || !pp[0].isList() || !pp[1].isList())
+                    throw FoamError(ifile + ": coupledPatchPair must be ((regionA patchA) (regionB patchB))");
+                int reg[2], pat[2];
+                for (int s = 0; s < 2; ++s) {
+                    const std::string &rn = pp[s].items.at(0).asWord(), &pn = pp[s].items.at(1).asWord();
+                    auto f = byName.find(rn);
+                    if (f == byName.end()) throw FoamError(ifile + ": unknown region '" + rn + "'");
+                    reg[s] = f->second;
+                    pat[s] = -1;
+                    const PolyMesh &m = R[reg[s]].mesh;
+                    for (size_t p = 0; p < m.patches.size(); ++p)
+                        if (m.patches[p].name == pn) pat[s] = (int)p;
+                    if (pat[s] < 0) throw FoamError(ifile + ": region '" + rn + "' has no patch '" + pn + "'");
+                    if (R[reg[s]].bcT[pat[s]] != BcT::Coupled)
+                        throw FoamError(ifile + ": patch '" + pn + "' of '" + rn + "' is not monolithicTemperature");
+                }
+                int axA, axB;
+                auto rA = patchRects(R[reg[0]].mesh, R[reg[0]].mesh.patches[pat[0]], axA);
+                auto rB = patchRects(R[reg[1]].mesh, R[reg[1]].mesh.patches[pat[1]], axB);
+                if (axA != axB) throw FoamError(ifile + ": coupled patches of '" + it.dict->name + "' are not parallel");
+                CoupledInterface I;
+                I.name = it.dict->name;
+                I.a.region = reg[0];
+                I.b.region = reg[1];
+                ggiWeights(rA, rB, I.a);
+                ggiWeights(rB, rA, I.b);
+                calcThermalDiffusivity(R[reg[0]], pat[0], R[reg[1]], pat[1], I.a);
+                calcThermalDiffusivity(R[reg[1]], pat[1], R[reg[0]], pat[0], I.b);
+                sys.interfaces.push_back(std::move(I));
+                sides.push_back({reg[0], pat[0], reg[1], pat[1]});
+            }
+    }
+    for (size_t r = 0; r < R.size(); ++r)
+        for (size_t p = 0; p < R[r].bcT.size(); ++p)
+            if (R[r].bcT[p] == BcT::Coupled && !R[r].coupledK.count((int)p) && R[r].mesh.patches[p].size > 0)
+                throw FoamError("coupled patch '" + R[r].mesh.patches[p].name + "' of region '" + R[r].mesh.name +
+                                "' has no monolithicCoupledPatches entry");
+
+    sys.regions.resize(R.size());
+    for (size_t r = 0; r < R.size(); ++r) {
+        const FoamRegion &rc = R[r];
+        const PolyMesh &m = rc.mesh;
+        FvMatrixG E = setCoupledEqns(rc, rDeltaT[r]);
+        LduRegion &L = sys.regions[r];
+        L.name = m.name;
+        L.nCells = m.nCells;
+        L.nFaces = m.nInternal;
+        L.lAddr.assign(m.owner.begin(), m.owner.begin() + m.nInternal);
+        L.uAddr = m.neighbour;
+        L.symmetric = !E.asymmetric;
+        L.diag = E.diag;
+        L.upper = E.upper;
+        L.lower = E.asymmetric ? E.lower : E.upper;
+        L.source = E.source;
+        L.psi = rc.T;
+        // addBoundaryDiag for every patch, addBoundarySource(couples=false)
+        // for the others; the coupled patches' boundaryCoeffs become the
+        // interface coefficients
+        for (size_t p = 0; p < m.patches.size(); ++p)
+            for (int i = 0; i < m.patches[p].size; ++i)
+                L.diag[m.owner[m.patches[p].start + i]] += E.internalCoeffs[p][i];
+        for (size_t p = 0; p < m.patches.size(); ++p) {
+            if (rc.bcT[p] == BcT::Coupled) continue;
+            for (int i = 0; i < m.patches[p].size; ++i)
+                L.source[m.owner[m.patches[p].start + i]] += E.boundaryCoeffs[p][i];
+        }
+        for (size_t i = 0; i < sides.size(); ++i)
+            for (int s = 0; s < 2; ++s) {
+                if (sides[i][2 * s] != (int)r) continue;
+                InterfaceSide &S = s ? sys.interfaces[i].b : sys.interfaces[i].a;
+                S.coeff = E.boundaryCoeffs[sides[i][2 * s + 1]];
+            }
+    }
+    if (info) {
+        info->volumes.clear();
+        info->maxNonOrthDeg.clear();
+        info->rDeltaT = rDeltaT;
+        info->rhoCp.clear();
+        info->k.clear();
+        for (auto &rc : R) {
+            info->volumes.push_back(rc.mesh.V);
+            info->maxNonOrthDeg.push_back(rc.mesh.maxNonOrthDeg);
+            info->rhoCp.push_back(rc.rhoCp);
+            info->k.push_back(rc.k);
+        }
+    }
+    sys.check();
+    return sys;
+}
+
+}  // namespace mrfg

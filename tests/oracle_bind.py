@@ -187,7 +187,54 @@ class OracleSystem:
                 offB=offB, idxB=_arr(I.idxB, int(offB[-1]), np.int32), wB=_arr(I.wB, int(offB[-1]), np.float64),
                 magSfA=_arr(I.magSfA, nA, np.float64), magSfB=_arr(I.magSfB, nB, np.float64)))
 
+    @classmethod
+    def from_arrays(cls, regions, interfaces):
+        """An oracle system over already assembled arrays (dicts with the
+        fields of Case.region / Case.interface), e.g. a case read from disk.
+        The arrays stay owned by Python."""
+        self = cls.__new__(cls)
+        self._keep = []
+
+        def ptr(a, ctype, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype).copy()
+            self._keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        s = System()
+        s.nRegions, s.nInterfaces = len(regions), len(interfaces)
+        L = (Ldu * max(1, len(regions)))()
+        for r, R in enumerate(regions):
+            L[r].nCells, L[r].nFaces = R["nCells"], R["nFaces"]
+            L[r].lAddr = ptr(R["lAddr"], C.c_int32, np.int32)
+            L[r].uAddr = ptr(R["uAddr"], C.c_int32, np.int32)
+            L[r].diag = ptr(R["diag"], C.c_double, np.float64)
+            L[r].lower = ptr(R["lower"], C.c_double, np.float64)
+            L[r].upper = ptr(R["upper"], C.c_double, np.float64)
+            L[r].source = ptr(R["source"], C.c_double, np.float64)
+            L[r].psi = ptr(R["psi"], C.c_double, np.float64)
+            L[r].symmetric = int(bool(R["symmetric"]))
+            L[r].k = ptr(np.ones(R["nCells"]), C.c_double, np.float64)
+            L[r].cellSize = ptr(np.ones(3), C.c_double, np.float64)
+        I = (Iface * max(1, len(interfaces)))()
+        for i, It in enumerate(interfaces):
+            I[i].regionA, I[i].regionB, I[i].nA, I[i].nB = It["regionA"], It["regionB"], It["nA"], It["nB"]
+            for side in "AB":
+                setattr(I[i], "fc" + side, ptr(It["fc" + side], C.c_int32, np.int32))
+                setattr(I[i], "coeff" + side, ptr(It["coeff" + side], C.c_double, np.float64))
+                setattr(I[i], "off" + side, ptr(It["off" + side], C.c_int32, np.int32))
+                setattr(I[i], "idx" + side, ptr(It["idx" + side], C.c_int32, np.int32))
+                setattr(I[i], "w" + side, ptr(It["w" + side], C.c_double, np.float64))
+                setattr(I[i], "magSf" + side, ptr(np.ones(It["n" + side]), C.c_double, np.float64))
+        self._keep += [L, I]
+        s.regions, s.interfaces = L, I
+        self.sys = s
+        self.names = [R.get("name", "region%d" % r) for r, R in enumerate(regions)]
+        self._load()
+        return self
+
     def __del__(self):
+        if getattr(self, "_keep", None) is not None:
+            return   # arrays owned by Python
         try:
             lib().orc_free(C.byref(self.sys))
         except Exception:

@@ -1,0 +1,257 @@
+"""Reading OpenFOAM case directories (mrfg_case_read_foam): the reference
+tutorial flowOverHeatedPlate (monolithic set-up) from the committed fixture,
+and structured box cases written as polyMesh.
+
+Pinning of the ingest, independent of its code:
+* the fixture rebuilds the tutorial exactly (against the reference's own
+  case directory where it is present);
+* geometry from bounding boxes of the hex cells (numpy, independent of the
+  pyramid / triangle-fan decomposition the reader uses): cell volumes in the
+  Euler ddt terms, the Gauss-linear Laplacian coefficients k |S| / |d|, the
+  upwind convection split rho*cp*phi with phi = U . S for the uniform U;
+* the interface coefficient's closed form, series resistance
+  |S| / (d_A / k_A + d_B / k_B);
+* a box case written as polyMesh assembles to the native box builder's
+  system (itself pinned against the oracle) entry by entry;
+* the oracle's solve of the read tutorial equals a direct sparse solve.
+The GPU solve of the tutorial, with the tutorial's own Tcoupled solver entry,
+is held to the oracle's (bit-identical FP64, as for every other case).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import multiregionfoam_amd as mrf
+from multiregionfoam_amd import cases
+import foam_fixture as ff
+import oracle_bind as ob
+
+REF_CASE = "/root/reference/tutorials/conjugateHeatTransfer/flowOverHeatedPlate"
+
+
+def _lib_or_skip():
+    try:
+        mrf.lib()
+    except ImportError as e:
+        pytest.skip("libmrfg.so not built: %s" % e)
+
+
+@pytest.fixture(scope="module")
+def tutorial_dir(tmp_path_factory):
+    d = tmp_path_factory.mktemp("flowOverHeatedPlate")
+    ff.write_fixture_case(str(d))
+    return str(d)
+
+
+def _arrays(case):
+    return case.regions(), case.interfaces()
+
+
+def _hex_geometry(mesh):
+    """Per-cell bounding boxes and per-face bounding boxes of an axis-aligned
+    hex mesh, from the points alone."""
+    P = mesh["points"]
+    sizes, verts = mesh["face_sizes"], mesh["face_verts"]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    fid = np.repeat(np.arange(len(sizes)), sizes)
+    flo = np.full((len(sizes), 3), np.inf)
+    fhi = np.full((len(sizes), 3), -np.inf)
+    np.minimum.at(flo, fid, P[verts])
+    np.maximum.at(fhi, fid, P[verts])
+    owner, nei = mesh["owner"], mesh["neighbour"]
+    n = int(max(owner.max(), nei.max())) + 1
+    clo = np.full((n, 3), np.inf)
+    chi = np.full((n, 3), -np.inf)
+    for cells, faces in ((owner, np.arange(len(owner))), (nei, np.arange(len(nei)))):
+        np.minimum.at(clo, cells, flo[faces])
+        np.maximum.at(chi, cells, fhi[faces])
+    return flo, fhi, clo, chi
+
+
+def test_fixture_rebuilds_reference_case(tutorial_dir):
+    """The fixture's case directory assembles to exactly the system of the
+    reference's own case directory."""
+    _lib_or_skip()
+    if not os.path.isdir(REF_CASE):
+        pytest.skip("reference case not present")
+    a = mrf.Case.from_foam(REF_CASE)
+    b = mrf.Case.from_foam(tutorial_dir)
+    for x, y in zip(a.regions() + a.interfaces(), b.regions() + b.interfaces()):
+        assert x.keys() == y.keys()
+        for k in x:
+            assert np.array_equal(np.asarray(x[k]), np.asarray(y[k])), k
+
+
+def test_tutorial_structure(tutorial_dir):
+    _lib_or_skip()
+    meta, meshes = ff.load_fixture()
+    c = mrf.Case.from_foam(tutorial_dir)
+    R, I = _arrays(c)
+    assert [r["nCells"] for r in R] == [13612, 8200]
+    assert [r["nFaces"] for r in R] == [len(meshes["fluid"]["neighbour"]), len(meshes["solid"]["neighbour"])]
+    assert not R[0]["symmetric"] and R[1]["symmetric"]
+    assert len(I) == 1 and I[0]["nA"] == 200 and I[0]["nB"] == 200
+    # conformal (directMap): each face gathers exactly one partner with weight 1
+    for s in "AB":
+        assert np.array_equal(np.diff(I[0]["off" + s]), np.ones(200, np.int32))
+        assert np.array_equal(I[0]["w" + s], np.ones(200))
+    # the face pairing is a bijection
+    assert sorted(I[0]["idxA"]) == list(range(200))
+    assert np.array_equal(I[0]["idxB"][I[0]["idxA"]], np.arange(200))
+
+
+def test_tutorial_geometry_and_coefficients(tutorial_dir):
+    """Euler ddt volumes, Laplacian and convection coefficients of the read
+    tutorial against bounding-box geometry of the fixture mesh."""
+    _lib_or_skip()
+    meta, meshes = ff.load_fixture()
+    c = mrf.Case.from_foam(tutorial_dir)
+    R, I = _arrays(c)
+    dt = meta["deltaT"]
+    tp = meta["transportProperties"]
+    for g, region in enumerate(("fluid", "solid")):
+        m = meshes[region]
+        flo, fhi, clo, chi = _hex_geometry(m)
+        V = np.prod(chi - clo, axis=1)
+        C = 0.5 * (clo + chi)
+        rhoC = tp[region]["rho"] * (tp[region]["cp"] if region == "fluid" else tp[region]["cv"])
+        k = tp[region]["k"]
+        T0 = meta["fields"][region]["T"]["internal"]
+        # cells whose faces are all internal or zeroGradient / empty: source
+        # is the Euler term rho*c*V*T_old/dt alone
+        patches = m["patches"]
+        bc = meta["fields"][region]["T"]["patches"]
+        touched = np.zeros(R[g]["nCells"], bool)
+        for p in patches:
+            if bc[p["name"]]["type"] in ("fixedValue",):
+                touched[m["owner"][p["startFace"]:p["startFace"] + p["nFaces"]]] = True
+        src = R[g]["source"]
+        np.testing.assert_allclose(src[~touched], rhoC * V[~touched] * T0 / dt, rtol=1e-12)
+        # Laplacian: -(k |S| / |d|) in the upper coefficient (solid), and the
+        # same plus the upwind split in the fluid
+        nf = R[g]["nFaces"]
+        ext = fhi[:nf] - flo[:nf]
+        normal = np.argmin(ext, axis=1)
+        area = np.prod(np.where(np.eye(3, dtype=bool)[normal], 1.0, ext), axis=1)
+        o, n = m["owner"][:nf], m["neighbour"]
+        dist = np.abs(C[n, normal] - C[o, normal])
+        lap = k * area / dist
+        if region == "solid":
+            np.testing.assert_allclose(-R[g]["upper"], lap, rtol=1e-12)
+            np.testing.assert_allclose(R[g]["lower"], R[g]["upper"], rtol=0)
+        else:
+            # U uniform (1 0 0): phi = S_x, signed owner -> neighbour
+            U = np.asarray(meta["fields"]["fluid"]["U"]["internal"])
+            phi = np.where(normal == 0, area * np.sign(C[n, 0] - C[o, 0]) * U[0], 0.0)
+            np.testing.assert_allclose(R[g]["upper"] - R[g]["lower"], rhoC * phi, rtol=1e-12, atol=1e-9)
+            np.testing.assert_allclose(-np.minimum(R[g]["upper"], R[g]["lower"]),
+                                       lap + rhoC * np.abs(phi) * 1.0, rtol=1e-12)
+    # interface coefficient: series resistance of the two half cells
+    it = I[0]
+    mf, ms = meshes["fluid"], meshes["solid"]
+    gf, gs = _hex_geometry(mf), _hex_geometry(ms)
+    pf = next(p for p in mf["patches"] if p["name"] == "interface")
+    ps = next(p for p in ms["patches"] if p["name"] == "top")
+    ff_ = np.arange(pf["startFace"], pf["startFace"] + pf["nFaces"])
+    fs_ = np.arange(ps["startFace"], ps["startFace"] + ps["nFaces"])
+    dA = np.abs(gf[0][ff_, 1] - 0.5 * (gf[2][mf["owner"][ff_], 1] + gf[3][mf["owner"][ff_], 1]))
+    dB = np.abs(gs[0][fs_, 1] - 0.5 * (gs[2][ms["owner"][fs_], 1] + gs[3][ms["owner"][fs_], 1]))
+    areaA = (gf[1][ff_, 0] - gf[0][ff_, 0]) * (gf[1][ff_, 2] - gf[0][ff_, 2])
+    kA, kB = tp["fluid"]["k"], tp["solid"]["k"]
+    expect = areaA / (dA / kA + dB[it["idxA"]] / kB)
+    np.testing.assert_allclose(it["coeffA"], expect, rtol=1e-12)
+
+
+def _plate_small():
+    return cases.plate2d(nx=12, nyf=5, nys=4)
+
+
+def _duct_small():
+    """3-D two-box case with a non-conformal GGI interface (offset, refined
+    solid faces), uniform k and a two-axis parabolic inflow."""
+    fluid = cases.region(
+        "fluid", "transportTemperature", (6, 5, 4), (0.0, 0.0, 0.0), (0.6, 0.5, 0.4),
+        k=dict(uniform=0.6), T0=300.0, rhoCp=4.18e6,
+        velocity=dict(axis=0, profile=(1, 2), Umax=0.2, noise=0.0, seed=0),
+        patches=dict(xmin=dict(type="fixedValue", value=300.0, flux="inlet"),
+                     xmax=dict(type="zeroGradient", flux="outlet"),
+                     ymin=dict(type="monolithicTemperature")))
+    solid = cases.region(
+        "solid", "conductTemperature", (7, 3, 6), (0.037, -0.2, -0.02), (0.63, 0.2, 0.45),
+        k=dict(uniform=16.0), T0=350.0,
+        patches=dict(ymin=dict(type="fixedValue", value=350.0), ymax=dict(type="monolithicTemperature")))
+    return cases.case([fluid, solid], [cases.interface("fluidsolid", ("fluid", "ymin"), ("solid", "ymax"))])
+
+
+@pytest.mark.parametrize("make", [_plate_small, _duct_small], ids=["plate2d", "duct_ggi"])
+def test_box_case_matches_native_builder(tmp_path, make):
+    """A box case written as an OpenFOAM polyMesh case assembles to the
+    native box builder's system: addressing exactly, coefficients to
+    rounding."""
+    _lib_or_skip()
+    case = make()
+    ff.write_box_case(str(tmp_path), case)
+    a = mrf.Case(case)
+    b = mrf.Case.from_foam(str(tmp_path))
+    for x, y in zip(a.regions(), b.regions()):
+        for k in ("nCells", "nFaces", "symmetric"):
+            assert x[k] == y[k], k
+        for k in ("lAddr", "uAddr"):
+            assert np.array_equal(x[k], y[k]), k
+        scale = np.abs(x["diag"]).max()
+        for k in ("diag", "lower", "upper", "source", "psi"):
+            np.testing.assert_allclose(y[k], x[k], rtol=1e-12, atol=1e-13 * scale, err_msg=k)
+    for x, y in zip(a.interfaces(), b.interfaces()):
+        for k in ("regionA", "regionB", "nA", "nB"):
+            assert x[k] == y[k], k
+        for k in ("fcA", "fcB", "offA", "offB", "idxA", "idxB"):
+            assert np.array_equal(x[k], y[k]), k
+        for k in ("wA", "wB", "coeffA", "coeffB"):
+            np.testing.assert_allclose(y[k], x[k], rtol=1e-12, err_msg=k)
+
+
+def test_tutorial_oracle_solve_matches_direct(tutorial_dir):
+    """The oracle's DILU-BiCGStab on the read tutorial system against a
+    direct sparse solve."""
+    _lib_or_skip()
+    import scipy.sparse.linalg as sla
+    c = mrf.Case.from_foam(tutorial_dir)
+    O = ob.OracleSystem.from_arrays(*_arrays(c))
+    psi, perf = O.solve(tol=1e-13, maxIter=1000)
+    assert perf["converged"]
+    A, b = O.to_scipy(), np.concatenate([r["source"] for r in O.regions])
+    x = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(np.concatenate(psi) - x) / np.linalg.norm(x) < 1e-10
+
+
+def test_read_errors(tmp_path):
+    _lib_or_skip()
+    with pytest.raises(mrf.MrfgError, match="multiRegionProperties"):
+        mrf.Case.from_foam(str(tmp_path))
+    ff.write_box_case(str(tmp_path), _plate_small())
+    with pytest.raises(mrf.MrfgError, match="ddt scheme"):
+        mrf.Case.from_foam(str(tmp_path), ddt="CrankNicolson")
+
+
+@pytest.mark.gpu
+def test_tutorial_gpu_matches_oracle(gpu, tutorial_dir):
+    """The tutorial's coupled T system, read from disk and solved on the GPU
+    with the tutorial's own Tcoupled entry (BiCGStab, Cholesky, tolerance
+    1e-15, maxIter 200), equals the oracle's solve."""
+    meta, _ = ff.load_fixture()
+    t = meta["Tcoupled"]
+    fvs = ("solvers { Tcoupled { solver %s; preconditioner { preconditioner %s; } tolerance %s; relTol %s; "
+           "minIter %s; maxIter %s; } }" % (t["solver"], t["preconditioner"], t["tolerance"], t["relTol"],
+                                              t["minIter"], t["maxIter"]))
+    c = gpu.Case.from_foam(tutorial_dir)
+    A = gpu.CoupledFvMatrix(0)
+    A.add_case(c)
+    A.commit()
+    perf = A.solve(fv_solution=fvs, entry="Tcoupled")[0]
+    O = ob.OracleSystem.from_arrays(*_arrays(c))
+    psi_ref, pref = O.solve(tol=float(t["tolerance"]), relTol=float(t["relTol"]), minIter=int(t["minIter"]),
+                            maxIter=int(t["maxIter"]))
+    assert perf.n_iterations == pref["nIterations"]
+    assert perf.final_residual == pref["finalResidual"]
+    assert np.array_equal(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
