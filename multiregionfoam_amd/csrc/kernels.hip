@@ -17,6 +17,7 @@ namespace mrfg {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kEllChunk = 8;      // ELL entries loaded together per SpMV row
 constexpr double kGreat = 1e20;   // lduMatrix::great_
 constexpr double kSmall = 1e-20;  // lduMatrix::small_
 
@@ -224,9 +225,25 @@ __device__ inline T spmvRow(const MatView<T> &A, int s, int lane, int p, const T
 {
     T y = A.diag[p] * x[p];
     const int e0 = A.ellOff[s], e1 = A.ellOff[s + 1];
-    for (int k = e0; k < e1; ++k) {
-        const size_t e = (size_t)k * 64 + lane;
-        y += A.ellVal[e] * x[A.ellCol[e]];
+    // chunks of kEllChunk entries: every column, value and gathered x of a
+    // chunk is loaded before the first product (loads at clamped entries, so
+    // they are unconditional), then the products are added in entry order
+    for (int k0 = e0; k0 < e1; k0 += kEllChunk) {
+        int col[kEllChunk];
+        T val[kEllChunk], xv[kEllChunk];
+#pragma unroll
+        for (int j = 0; j < kEllChunk; ++j) {
+            const size_t e = (size_t)min(k0 + j, e1 - 1) * 64 + lane;
+            col[j] = A.ellCol[e];
+            val[j] = A.ellVal[e];
+        }
+#pragma unroll
+        for (int j = 0; j < kEllChunk; ++j) xv[j] = x[col[j]];
+#pragma unroll
+        for (int j = 0; j < kEllChunk; ++j) {
+            const T t = y + val[j] * xv[j];
+            y = k0 + j < e1 ? t : y;
+        }
     }
     const unsigned long long m = A.ifMask[s];
     if ((m >> lane) & 1ull) {
