@@ -109,6 +109,64 @@ def run_exchanger(m, cases, args):
     print(json.dumps(out))
 
 
+def run_tutorial(m, args):
+    """The reference tutorial flowOverHeatedPlate (monolithic set-up), read
+    from disk as an OpenFOAM case (polyMesh, 0/, constant/, system/; the
+    committed fixture written out to a temporary directory) and solved with
+    the tutorial's own Tcoupled entry: the first time step's coupled T
+    system, 21,812 cells on the graded 3-block fluid mesh and the solid."""
+    import shutil
+    import tempfile
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import foam_fixture
+    d = tempfile.mkdtemp(prefix="flowOverHeatedPlate_")
+    try:
+        meta = foam_fixture.write_fixture_case(d)
+        t0 = time.perf_counter()
+        case = m.Case.from_foam(d)
+        t_read = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    t = meta["Tcoupled"]
+    fvs = ("solvers { Tcoupled { solver %s; preconditioner { preconditioner %s; } tolerance %s; relTol %s; "
+           "minIter %s; maxIter %s; } }" % (t["solver"], t["preconditioner"], t["tolerance"], t["relTol"],
+                                              t["minIter"], t["maxIter"]))
+    ctrl = m.parse_controls(fvs, "Tcoupled")
+    A = m.CoupledFvMatrix(0)
+    A.add_case(case)
+    A.commit()
+    info = A.layout_info(0)
+    total, lat, perf = run_steps(A, ctrl, args.warmup, args.steps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle_bind
+        O = oracle_bind.OracleSystem.from_arrays(case.regions(), case.interfaces())
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, pref = O.solve(tol=float(t["tolerance"]), relTol=float(t["relTol"]), minIter=int(t["minIter"]),
+                              maxIter=int(t["maxIter"]))
+        tc = (time.perf_counter() - t0) / reps
+        cpu = dict(value=1.0 / tc, unit="solves/s", cores=1, kind="port",
+                   sample="%d oracle solves of the same system on one host core (%d iterations, %.1f ms each)"
+                          % (reps, pref["nIterations"], 1e3 * tc))
+    out = {
+        "metric": "coupled solves/s of the flowOverHeatedPlate tutorial read from disk",
+        "value": args.steps / total, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "reference tutorial case (fixture of its mesh and dictionaries)",
+        "config": {"workload": "flowOverHeatedPlate (monolithic), first time step, Tcoupled: %s/%s tol %s"
+                               % (t["solver"], t["preconditioner"], t["tolerance"]),
+                   "cells": info["cells"], "internal_faces": info["faces"], "ggi_pairs": info["pairs"]},
+        "iterations": perf.n_iterations, "final_residual": perf.final_residual, "converged": perf.converged,
+        "p50_ms": 1e3 * pct(lat, 0.5), "p99_ms": 1e3 * pct(lat, 0.99), "case_read_s": t_read,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def run_batch(m, cases, args):
     """64 independent two-region systems of seeded uneven size (cells per
     region log-uniform in [10k, 500k], conductivity ratio log-uniform in
@@ -202,12 +260,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--workload", default="duct3d", choices=["duct3d", "exchanger3", "batch64", "serving"],
+    ap.add_argument("--workload", default="duct3d",
+                    choices=["duct3d", "exchanger3", "batch64", "serving", "tutorial"],
                     help="duct3d (default, the headline line): the 1.404M-cell two-region GGI solve; "
                          "exchanger3: three regions of 2.4M cells, two GGI interfaces, mixed precision "
                          "with the FP64 interface-flux check; batch64: 64 independent uneven systems "
                          "solved together; serving: Poisson arrivals at 50 req/s of 20k-2M-cell solves "
-                         "through the continuous-batching server")
+                         "through the continuous-batching server; tutorial: the reference's "
+                         "flowOverHeatedPlate case read from disk, with its own Tcoupled solver entry")
     ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
@@ -229,6 +289,8 @@ def main():
         return run_batch(m, cases, args)
     if args.workload == "serving":
         return run_serving(m, cases, args)
+    if args.workload == "tutorial":
+        return run_tutorial(m, args)
 
     case = cases.duct3d()
     A = m.CoupledFvMatrix(0)

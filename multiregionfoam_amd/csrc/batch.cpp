@@ -99,6 +99,9 @@ Batch::Batch(int device) : device_(device)
 {
     HIP_OK(hipSetDevice(device_));
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
 }
 
 Batch::~Batch()
@@ -108,6 +111,12 @@ Batch::~Batch()
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
     }
+    if (side_) {
+        (void)hipStreamSynchronize(side_);
+        (void)hipStreamDestroy(side_);
+    }
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
 }
 
 namespace {
@@ -486,6 +495,9 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.boxInB = (char *)hB + (size_t)nHaloB_ * sizeof(T);
     D.boxInC = (char *)haloC64_.p + (size_t)nHaloF_ * sizeof(double);
     D.err = err_.p;
+    D.side = side_;
+    D.fork = fork_;
+    D.join = join_;
     return D;
 }
 

@@ -149,6 +149,8 @@ private:
 
     int device_;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;               // box DILU sweeps beside general ones
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
     bool preferLevels_ = false;
     std::vector<std::unique_ptr<SystemLayout>> lay_;
     std::vector<const CoupledSystem *> sys_;

@@ -24,6 +24,8 @@
 // and the fast path has no divergent branch.
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include "dilu.hpp"
 
 namespace mrfg {
@@ -513,7 +515,8 @@ static hipError_t launchK(const DiluView<T> &V, const SysState *st, const T *in,
     const int mask = effectiveHaloMask(V.K, MODE == kDiluBwd ? V.haloMaskB : V.haloMaskF);
     switch (V.K) {
     case 3:
-        if (mask == 3) k_dilu_wave<T, 3, MODE, 3><<<V.nWaves, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        if (mask == 0) k_dilu_wave<T, 3, MODE, 0><<<V.nWaves, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        else if (mask == 3) k_dilu_wave<T, 3, MODE, 3><<<V.nWaves, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
         else k_dilu_wave<T, 3, MODE, 7><<<V.nWaves, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
         break;
     case 6: k_dilu_wave<T, 6, MODE, 63><<<V.nWaves, 64, 0, s>>>(V, st, in, out, use, clr, nClr); break;
@@ -534,22 +537,50 @@ static hipError_t launchBox(const DiluView<T> &V, const SysState *st, const T *i
     return hipGetLastError();
 }
 
+// The general sweep on s and the box sweep on V.side when both exist and a
+// side stream is set (the box sweep forked after the work already on s and
+// joined before the next), else both on s.
+template <typename Gen, typename Box>
+static hipError_t bothSweeps(const DiluView<Gen> &V, hipStream_t s, const std::function<hipError_t(hipStream_t)> &gen,
+                             const std::function<hipError_t(hipStream_t)> &box)
+{
+    const bool fork = V.side && V.fork && V.join && V.nWaves > 0 && V.nBoxWaves > 0;
+    if (!fork) {
+        hipError_t e = gen(s);
+        return e != hipSuccess ? e : box(s);
+    }
+    hipError_t e = hipEventRecord(V.fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(V.side, V.fork, 0);
+    if (e == hipSuccess) e = box(V.side);
+    if (e == hipSuccess) e = hipEventRecord(V.join, V.side);
+    if (e == hipSuccess) e = gen(s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, V.join, 0);
+    return e;
+}
+
 template <typename T>
 hipError_t diluForward(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s)
 {
     using U = typename Bits<T>::U;
-    hipError_t e = launchK<T, kDiluFwd>(V, st, in, out, (U *)V.haloF, (U *)V.haloB, V.nHaloB, s);
-    if (e != hipSuccess) return e;
-    return launchBox<T, kDiluFwd>(V, st, in, out, (U *)V.boxInF, (U *)V.boxInB, V.nBoxIn, s);
+    return bothSweeps<T, T>(
+        V, s, [&](hipStream_t q) { return launchK<T, kDiluFwd>(V, st, in, out, (U *)V.haloF, (U *)V.haloB, V.nHaloB, q); },
+        [&](hipStream_t q) {
+            return launchBox<T, kDiluFwd>(V, st, in, out, (U *)V.boxInF, (U *)V.boxInB, V.nBoxIn, q);
+        });
 }
 
 template <typename T>
 hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipStream_t s)
 {
     using U = typename Bits<T>::U;
-    hipError_t e = launchK<T, kDiluBwd>(V, st, inout, inout, (U *)V.haloB, (U *)V.haloF, V.nHaloF, s);
-    if (e != hipSuccess) return e;
-    return launchBox<T, kDiluBwd>(V, st, inout, inout, (U *)V.boxInB, (U *)V.boxInF, V.nBoxIn, s);
+    return bothSweeps<T, T>(
+        V, s,
+        [&](hipStream_t q) {
+            return launchK<T, kDiluBwd>(V, st, inout, inout, (U *)V.haloB, (U *)V.haloF, V.nHaloF, q);
+        },
+        [&](hipStream_t q) {
+            return launchBox<T, kDiluBwd>(V, st, inout, inout, (U *)V.boxInB, (U *)V.boxInF, V.nBoxIn, q);
+        });
 }
 
 __global__ void k_fill_u64(unsigned long long *a, int n, unsigned long long v)

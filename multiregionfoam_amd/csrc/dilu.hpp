@@ -49,7 +49,11 @@ constexpr int codeInts(int K) { return packChunks(K, 4) + K; }
 // Dependency slots that are given an inbox for a K-slot schedule whose
 // cross-wave dependencies occur in `mask`: structured-box schedules (K = 3)
 // only cross waves in the first two slots; anything else gets all slots.
-inline int effectiveHaloMask(int K, int mask) { return (K == 3 && (mask & ~3) == 0) ? 3 : (1 << K) - 1; }
+inline int effectiveHaloMask(int K, int mask)
+{
+    if (K == 3 && mask == 0) return 0;   // no cross-wave dependency: no inbox at all
+    return (K == 3 && (mask & ~3) == 0) ? 3 : (1 << K) - 1;
+}
 
 template <typename T>
 struct DiluView {
@@ -77,6 +81,11 @@ struct DiluView {
     // hits and steps (4 entries each); general waves by dispatch index, then
     // box waves by dispatch index
     unsigned long long *trace = nullptr;
+    // host side: when both kinds of waves exist, the box sweep runs on `side`
+    // concurrently with the general sweep (regions never depend on each
+    // other), forked and joined with the two events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // Forward sweep out = (D+L)^-1 in (with the D^-1 scaling of DILU) and the

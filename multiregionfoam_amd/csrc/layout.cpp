@@ -119,6 +119,26 @@ void levelWaves(int regionIdx, int c0, int c1, const RowDeps &D, RegionSchedule 
     }
     std::vector<std::vector<int32_t>> byLevel(maxLevel + 1);
     for (int c = c0; c < c1; ++c) byLevel[level[c - c0]].push_back(c);
+    size_t maxWidth = 0;
+    for (const auto &b : byLevel) maxWidth = std::max(maxWidth, b.size());
+    if (maxWidth <= (size_t)kSlice) {
+        // every level fits one slice (e.g. 2-D and multi-block meshes whose
+        // levels are anti-diagonals of at most 64 cells): one wave walks the
+        // levels as its steps, so a dependency on the previous levels comes
+        // from the wave's own LDS ring instead of another wave's inbox
+        rs.kind = "chain";
+        WaveBuild w;
+        w.key = 0;
+        w.region = regionIdx;
+        w.steps = maxLevel + 1;
+        w.cells.assign((size_t)w.steps * kSlice, -1);
+        for (int lv = 0; lv <= maxLevel; ++lv)
+            for (size_t l = 0; l < byLevel[lv].size(); ++l) w.cells[(size_t)lv * kSlice + l] = byLevel[lv][l];
+        out.push_back(std::move(w));
+        rs.nWaves++;
+        rs.maxSteps = maxLevel + 1;
+        return;
+    }
     rs.kind = "levels";
     for (int lv = 0; lv <= maxLevel; ++lv)
         for (size_t i = 0; i < byLevel[lv].size(); i += kSlice) {
