@@ -151,6 +151,26 @@ def run_tutorial(m, args):
         cpu = dict(value=1.0 / tc, unit="solves/s", cores=1, kind="port",
                    sample="%d oracle solves of the same system on one host core (%d iterations, %.1f ms each)"
                           % (reps, pref["nIterations"], 1e3 * tc))
+    # the tutorial's time loop for T with the flow fixed: endTime / deltaT
+    # backward steps (the first Euler), the ddt terms rebuilt on the device
+    n_steps = int(round(1.0 / meta["deltaT"]))
+    A.reset_psi()
+    t0 = time.perf_counter()
+    steps = A.time_steps(n_steps, ddt=meta["ddt"]["fluid"], ctrl=ctrl)
+    t_run = time.perf_counter() - t0
+    its = [s[0].n_iterations for s in steps]
+    transient = dict(steps=n_steps, ddt=meta["ddt"]["fluid"], seconds=t_run, ms_per_step=1e3 * t_run / n_steps,
+                     iterations_first=its[0], iterations_last=its[-1], iterations_mean=sum(its) / len(its),
+                     all_converged=all(s[0].converged for s in steps),
+                     T_range=[float(min(x.min() for x in A.psi(0))), float(max(x.max() for x in A.psi(0)))])
+    if not args.no_cpu_baseline:
+        k = 5
+        t0 = time.perf_counter()
+        O.time_steps([({kk: R[kk] for kk in ("ddtCoeff", "diagBase", "sourceBase")} if "ddtCoeff" in R else None)
+                      for R in case.regions()], k, scheme=meta["ddt"]["fluid"], tol=float(t["tolerance"]),
+                     relTol=float(t["relTol"]), minIter=int(t["minIter"]), maxIter=int(t["maxIter"]))
+        transient["cpu_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / k
+        transient["cpu_sample"] = "first %d steps on one host core (oracle)" % k
     out = {
         "metric": "coupled solves/s of the flowOverHeatedPlate tutorial read from disk",
         "value": args.steps / total, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -162,7 +182,7 @@ def run_tutorial(m, args):
                    "cells": info["cells"], "internal_faces": info["faces"], "ggi_pairs": info["pairs"]},
         "iterations": perf.n_iterations, "final_residual": perf.final_residual, "converged": perf.converged,
         "p50_ms": 1e3 * pct(lat, 0.5), "p99_ms": 1e3 * pct(lat, 0.99), "case_read_s": t_read,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "transient": transient,
     }
     print(json.dumps(out))
 

@@ -98,6 +98,14 @@ int mrfg_set_region(mrfg_ctx *ctx, int sys, int region, int n_cells, int n_faces
                     const double *diag, const double *lower, const double *upper,
                     const double *source, const double *psi0);
 
+/* The ddt term of a transient region, for mrfg_time_steps: per cell the
+   Euler ddt coefficient a = rho*c*V/dt (the diagonal fvm::ddt adds,
+   EulerDdtScheme) and the region's diagonal and source without it
+   (ddt_coeff == NULL makes the region steady again).  The region's
+   diag / source set by mrfg_set_region are the first step's system. */
+int mrfg_set_region_ddt(mrfg_ctx *ctx, int sys, int region, const double *ddt_coeff,
+                        const double *diag_base, const double *source_base);
+
 /* One monolithic interface (a coupledPatchPair of regionInterfaceProperties;
    monolithicCouplingFvPatchField.C:381-464).  Side A faces gather B face
    cells with weights w_a over pairs [off_a[i], off_a[i+1]) (idx_a = B face
@@ -125,6 +133,20 @@ int mrfg_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_perf *perf);
 /* Same, with the controls read from an fvSolution text: `entry` names the
    solver dictionary under `solvers` (e.g. "Tcoupled"). */
 int mrfg_solve_dict(mrfg_ctx *ctx, const char *fv_solution, const char *entry, mrfg_perf *perf);
+
+/* n_steps time steps of the committed systems: the time loop of
+   multiRegionFoam around assembleAndSolveCoupledMatrix
+   (multiRegionSystem.C:61-191) for a fixed flow, where each step's
+   setCoupledEqns changes only the ddt terms.  They are rebuilt on the
+   device from the current and previous fields (no matrix upload per step):
+   Euler, or "backward" (constant time step, coefficients 3/2, 2, 1/2; the
+   first step of a run, after mrfg_commit or mrfg_reset_psi, is Euler, as
+   backwardDdtScheme's first step).  Each step solves from the current field.
+   perf receives n_steps x n_systems entries (step-major).  Needs at least
+   one transient region (mrfg_set_region_ddt or a case read with a ddt
+   scheme); precision fp64. */
+int mrfg_time_steps(mrfg_ctx *ctx, const mrfg_controls *controls, const char *ddt_scheme, int n_steps,
+                    mrfg_perf *perf);
 
 /* Parse a solver entry of an fvSolution text (lduSolver::New selection by
    name, keys solver / preconditioner / tolerance / relTol / minIter /
@@ -163,6 +185,13 @@ int mrfg_case_sizes(const mrfg_case *c, int *n_regions, int *n_interfaces);
 int mrfg_case_region_sizes(const mrfg_case *c, int region, int *n_cells, int *n_faces, int *symmetric);
 int mrfg_case_region(const mrfg_case *c, int region, int32_t *l_addr, int32_t *u_addr,
                      double *diag, double *lower, double *upper, double *source, double *psi0);
+/* The ddt split of a transient region of a built case (a case read with an
+   Euler / backward ddt): *transient = 1 and, per cell, the Euler ddt
+   coefficient a = rho*c*V/dt and the diagonal and source without the ddt
+   term (diag = diag_base + a, source = source_base + a*psi0); *transient = 0
+   for a steady region (arrays untouched).  Null arrays are skipped. */
+int mrfg_case_region_ddt(const mrfg_case *c, int region, int *transient, double *ddt_coeff,
+                         double *diag_base, double *source_base);
 int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int *region_b,
                               int *n_faces_a, int *n_faces_b, int *n_pairs_a, int *n_pairs_b);
 int mrfg_case_interface(const mrfg_case *c, int iface,

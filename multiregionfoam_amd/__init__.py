@@ -77,10 +77,12 @@ def lib():
         "mrfg_set_interface": ([vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                 C.c_int, I32P, F64P, I32P, I32P, F64P,
                                 C.c_int, I32P, F64P, I32P, I32P, F64P], C.c_int),
+        "mrfg_set_region_ddt": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
         "mrfg_commit": ([vp], C.c_int),
         "mrfg_set_schedule": ([vp, C.c_int], C.c_int),
         "mrfg_solve": ([vp, C.POINTER(Controls), C.POINTER(Perf)], C.c_int),
         "mrfg_solve_dict": ([vp, C.c_char_p, C.c_char_p, C.POINTER(Perf)], C.c_int),
+        "mrfg_time_steps": ([vp, C.POINTER(Controls), C.c_char_p, C.c_int, C.POINTER(Perf)], C.c_int),
         "mrfg_parse_controls": ([C.c_char_p, C.c_char_p, C.POINTER(Controls), C.c_char_p, C.c_int], C.c_int),
         "mrfg_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_reset_psi": ([vp], C.c_int),
@@ -92,6 +94,7 @@ def lib():
         "mrfg_case_region_sizes": ([vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
                                    C.c_int),
         "mrfg_case_region": ([vp, C.c_int, I32P, I32P, F64P, F64P, F64P, F64P, F64P], C.c_int),
+        "mrfg_case_region_ddt": ([vp, C.c_int, C.POINTER(C.c_int), F64P, F64P, F64P], C.c_int),
         "mrfg_case_interface_sizes": ([vp, C.c_int] + [C.POINTER(C.c_int)] * 6, C.c_int),
         "mrfg_case_interface": ([vp, C.c_int, I32P, F64P, I32P, I32P, F64P, I32P, F64P, I32P, I32P, F64P],
                                 C.c_int),
@@ -124,9 +127,11 @@ def lib():
 
 
 EXPORTED = (
-    "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region "
-    "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_parse_controls mrfg_get_psi mrfg_reset_psi "
-    "mrfg_set_psi mrfg_case_build mrfg_case_read_foam mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes mrfg_case_region "
+    "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region mrfg_set_region_ddt "
+    "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_time_steps mrfg_parse_controls "
+    "mrfg_get_psi mrfg_reset_psi "
+    "mrfg_set_psi mrfg_case_build mrfg_case_read_foam mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes "
+    "mrfg_case_region mrfg_case_region_ddt "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
@@ -255,6 +260,14 @@ class Case:
                                _p(d["diag"], C.c_double), _p(d["lower"], C.c_double),
                                _p(d["upper"], C.c_double), _p(d["source"], C.c_double),
                                _p(d["psi"], C.c_double))
+        # transient regions: the Euler ddt coefficient and the diagonal and
+        # source without the ddt term
+        tr = C.c_int()
+        lib().mrfg_case_region_ddt(self._h, r, C.byref(tr), None, None, None)
+        if tr.value:
+            d["ddtCoeff"], d["diagBase"], d["sourceBase"] = np.zeros(n), np.zeros(n), np.zeros(n)
+            lib().mrfg_case_region_ddt(self._h, r, C.byref(tr), _p(d["ddtCoeff"], C.c_double),
+                                       _p(d["diagBase"], C.c_double), _p(d["sourceBase"], C.c_double))
         return d
 
     def interface(self, i):
@@ -338,6 +351,10 @@ class CoupledFvMatrix:
                 self._h, s, r, int(R["nCells"]), int(R["nFaces"]), _p(arr["lAddr"], C.c_int32),
                 _p(arr["uAddr"], C.c_int32), _p(arr["diag"], C.c_double), _p(lower, C.c_double),
                 _p(arr["upper"], C.c_double), _p(arr["source"], C.c_double), _p(arr["psi"], C.c_double)))
+            if R.get("ddtCoeff") is not None:
+                dd = [np.ascontiguousarray(R[k], dtype=np.float64) for k in ("ddtCoeff", "diagBase", "sourceBase")]
+                keep.append(dd)
+                self._check(lib().mrfg_set_region_ddt(self._h, s, r, *[_p(x, C.c_double) for x in dd]))
         for i, I in enumerate(interfaces):
             a = {k: np.ascontiguousarray(I[k], dtype=np.int32 if k[:2] in ("fc", "of", "id") else np.float64)
                  for k in ("fcA", "coeffA", "offA", "idxA", "wA", "fcB", "coeffB", "offB", "idxB", "wB")}
@@ -374,6 +391,25 @@ class CoupledFvMatrix:
         return [SolverPerformance(p.initial_residual, p.final_residual, p.n_iterations, bool(p.converged),
                                   bool(p.singular), p.n_outer, p.seconds, p.bytes_moved)
                 for p in perf[:self.n_systems]]
+
+    def time_steps(self, n_steps, ddt="backward", ctrl=None, fv_solution=None, entry="Tcoupled", **kw):
+        """``n_steps`` time steps of the transient systems (the ddt terms
+        rebuilt on the device each step; Euler, or backward whose first step
+        in a run is Euler).  Returns, per step, the per-system
+        SolverPerformance list."""
+        if fv_solution is not None:
+            ctrl = parse_controls(fv_solution, entry)
+        elif ctrl is None:
+            ctrl = controls(**kw)
+        n = max(1, self.n_systems)
+        perf = (Perf * (n * n_steps))()
+        self._check(lib().mrfg_time_steps(self._h, C.byref(ctrl), _text(ddt), int(n_steps), perf))
+        out = []
+        for k in range(n_steps):
+            out.append([SolverPerformance(p.initial_residual, p.final_residual, p.n_iterations, bool(p.converged),
+                                          bool(p.singular), p.n_outer, p.seconds, p.bytes_moved)
+                        for p in perf[k * n:(k + 1) * n]])
+        return out
 
     def psi(self, sys=0):
         out = []

@@ -162,6 +162,8 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     hEllVal_.clear(); hDiag_.clear(); hIfCoeff_.clear(); hIfPairW_.clear(); hFVal_.clear(); hBVal_.clear();
     std::vector<BlockEntry> blocks;
     hInfo_.assign(nS, SysInfo{});
+    std::vector<double> ddtA, diagBase, srcBase;
+    bool anyTransient = false;
 
     int slice = 0, ellBase = 0, ifRowBase = 0, ifFaceBase = 0, ifPairBase = 0, hfBase = 0, hbBase = 0;
     for (int q = 0; q < nS; ++q) {
@@ -182,6 +184,21 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
         source.insert(source.end(), L.source.begin(), L.source.end());
         sumA.insert(sumA.end(), L.sumA.begin(), L.sumA.end());
         psi0.insert(psi0.end(), L.psi0.begin(), L.psi0.end());
+        // time stepping: steady regions have a = 0 and their own diag / source
+        ddtA.resize(diag.size(), 0.0);
+        diagBase.insert(diagBase.end(), L.diag.begin(), L.diag.end());
+        srcBase.insert(srcBase.end(), L.source.begin(), L.source.end());
+        for (int r = 0; r < (int)systems[q]->regions.size(); ++r) {
+            const LduRegion &R = systems[q]->regions[r];
+            if (!R.transient()) continue;
+            anyTransient = true;
+            for (int c = 0; c < R.nCells; ++c) {
+                const size_t p = pos0 + L.posOfCell[L.regionOffset[r] + c];
+                ddtA[p] = R.ddtCoeff[c];
+                diagBase[p] = R.diagBase[c];
+                srcBase[p] = R.sourceBase[c];
+            }
+        }
         hEllVal_.insert(hEllVal_.end(), L.ellVal.begin(), L.ellVal.end());
         hIfCoeff_.insert(hIfCoeff_.end(), L.ifCoeff.begin(), L.ifCoeff.end());
         hIfPairW_.insert(hIfPairW_.end(), L.ifPairW.begin(), L.ifPairW.end());
@@ -398,6 +415,16 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     m64_.ifPairW.upload(hIfPairW_, st);
     sumA_.upload(sumA, st);
     b64_.upload(source, st);
+    transient_ = anyTransient;
+    stepsDone_ = 0;
+    diagCoefft_ = 1.0;
+    if (transient_) {
+        ddtA_.upload(ddtA, st);
+        diagBase_.upload(diagBase, st);
+        srcBase_.upload(srcBase, st);
+        xPrev_.alloc(diag.size());
+        xPrev_.zero(st);
+    }
     x64_.upload(psi0, st);
     x0_.upload(psi0, st);
     for (auto &v : v64_) {
@@ -755,6 +782,39 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
 void Batch::resetPsi()
 {
     HIP_OK(hipMemcpyAsync(x64_.p, x0_.p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    stepsDone_ = 0;
+}
+
+void Batch::setDdtDiag(double coefft)
+{
+    if (coefft == diagCoefft_) return;
+    HIP_OK(launchDdtDiag(view(m64_), blocks_.p, nBlk_, K_, coefft, diagBase_.p, ddtA_.p, m64_.diag.p,
+                         prodPack_.p, sumA_.p, stream_));
+    // the host copy the FP32 matrix is derived from
+    HIP_OK(hipMemcpyAsync(hDiag_.data(), m64_.diag.p, hDiag_.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    haveFloat_ = false;
+    diagCoefft_ = coefft;
+}
+
+void Batch::timeSteps(const SolverControls &c, bool backward, int nSteps, std::vector<SolvePerf> &perf)
+{
+    HIP_OK(hipSetDevice(device_));
+    if (!transient_) throw std::runtime_error("time stepping needs a transient system (ddt coefficients)");
+    if (c.precision != SolverControls::FP64) throw std::runtime_error("time stepping runs in precision fp64");
+    if (nSteps < 1) throw std::runtime_error("time stepping needs nSteps >= 1");
+    perf.clear();
+    for (int n = 0; n < nSteps; ++n) {
+        const bool bwd = backward && stepsDone_ >= 1;
+        setDdtDiag(bwd ? 1.5 : 1.0);
+        HIP_OK(launchDdtSource(blocks_.p, nBlk_, bwd, srcBase_.p, ddtA_.p, x64_.p, xPrev_.p, b64_.p, stream_));
+        HIP_OK(hipMemcpyAsync(xPrev_.p, x64_.p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        std::vector<SolvePerf> p;
+        solve(c, p);
+        perf.insert(perf.end(), p.begin(), p.end());
+        ++stepsDone_;
+    }
 }
 
 void Batch::setPsi(int sys, int region, const double *psi)

@@ -88,7 +88,17 @@ public:
     // Solve every system; perf gets one entry per system.
     void solve(const SolverControls &c, std::vector<SolvePerf> &perf);
 
-    // Reset every system's field to the committed initial guess (device copy).
+    // Time stepping of the transient systems (regions with ddt coefficients;
+    // steady systems are re-solved from their current field): per step the
+    // ddt terms are rebuilt on the device from the current and previous
+    // fields, then every system is solved from its current field.  Euler,
+    // or backward (constant time step; the first step of a run is Euler).
+    // perf gets nSteps x nSystems entries.  FP64 only.
+    void timeSteps(const SolverControls &c, bool backward, int nSteps, std::vector<SolvePerf> &perf);
+    bool transient() const { return transient_; }
+
+    // Reset every system's field to the committed initial guess (device
+    // copy); a following time-step run starts anew.
     void resetPsi();
     // Per-region initial guess / solution of one system (original cell order).
     void setPsi(int sys, int region, const double *psi);
@@ -146,6 +156,7 @@ private:
     template <typename T>
     void bindKrylov(Krylov<T> &K);
     int posBegin(int sys) const { return sliceBegin_[sys] * kSlice; }
+    void setDdtDiag(double coefft);
 
     int device_;
     hipStream_t stream_ = nullptr;
@@ -190,6 +201,12 @@ private:
     };
     std::vector<std::vector<std::array<std::unique_ptr<IfaceDev>, 2>>> ifDev_;
     DevBuf<double> partials_;
+    // time stepping (position space): ddt coefficient, diagonal and source
+    // without the ddt term, previous field
+    DevBuf<double> ddtA_, diagBase_, srcBase_, xPrev_;
+    bool transient_ = false;
+    int stepsDone_ = 0;
+    double diagCoefft_ = 1.0;
     std::vector<SysInfo> hInfo_;
 };
 

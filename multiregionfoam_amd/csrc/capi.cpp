@@ -207,6 +207,29 @@ int mrfg_set_region(mrfg_ctx *ctx, int sys, int region, int n_cells, int n_faces
     });
 }
 
+int mrfg_set_region_ddt(mrfg_ctx *ctx, int sys, int region, const double *ddt_coeff, const double *diag_base,
+                        const double *source_base)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (region < 0 || region >= (int)S.regions.size()) throw ArgError("region index out of range");
+        mrfg::LduRegion &R = S.regions[region];
+        if (R.nCells < 1) throw ArgError("set the region before its ddt terms");
+        if (!ddt_coeff) {
+            R.ddtCoeff.clear();
+            R.diagBase.clear();
+            R.sourceBase.clear();
+        } else {
+            if (!diag_base || !source_base) throw ArgError("null ddt array");
+            R.ddtCoeff.assign(ddt_coeff, ddt_coeff + R.nCells);
+            R.diagBase.assign(diag_base, diag_base + R.nCells);
+            R.sourceBase.assign(source_base, source_base + R.nCells);
+        }
+        ctx->committed = false;
+    });
+}
+
 int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int region_b, int n_faces_a,
                        const int32_t *fc_a, const double *coeff_a, const int32_t *off_a, const int32_t *idx_a,
                        const double *w_a, int n_faces_b, const int32_t *fc_b, const double *coeff_b,
@@ -278,6 +301,20 @@ int mrfg_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_perf *perf)
         mrfg::Batch &B = committedBatch(ctx);
         std::vector<mrfg::SolvePerf> p;
         B.solve(fromControls(controls), p);
+        fillPerf(p, perf);
+    });
+}
+
+int mrfg_time_steps(mrfg_ctx *ctx, const mrfg_controls *controls, const char *ddt_scheme, int n_steps,
+                    mrfg_perf *perf)
+{
+    if (!ctx || !controls || !ddt_scheme || !perf || n_steps < 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        const std::string s = ddt_scheme;
+        if (s != "Euler" && s != "backward") throw ArgError("ddt scheme must be Euler or backward");
+        mrfg::Batch &B = committedBatch(ctx);
+        std::vector<mrfg::SolvePerf> p;
+        B.timeSteps(fromControls(controls), s == "backward", n_steps, p);
         fillPerf(p, perf);
     });
 }
@@ -396,6 +433,22 @@ int mrfg_case_region(const mrfg_case *c, int region, int32_t *l_addr, int32_t *u
     cp(upper, R.upper);
     cp(source, R.source);
     cp(psi0, R.psi);
+    return MRFG_OK;
+}
+
+int mrfg_case_region_ddt(const mrfg_case *c, int region, int *transient, double *ddt_coeff, double *diag_base,
+                         double *source_base)
+{
+    if (!c || region < 0 || region >= (int)c->sys.regions.size()) return MRFG_ERR_ARG;
+    const auto &R = c->sys.regions[region];
+    if (transient) *transient = R.transient() ? 1 : 0;
+    if (!R.transient()) return MRFG_OK;
+    auto cp = [](double *dst, const std::vector<double> &src) {
+        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(double));
+    };
+    cp(ddt_coeff, R.ddtCoeff);
+    cp(diag_base, R.diagBase);
+    cp(source_base, R.sourceBase);
     return MRFG_OK;
 }
 

@@ -579,6 +579,9 @@ void CoupledSystem::check() const
             throw std::runtime_error("region '" + r.name + "': array sizes do not match nCells/nFaces");
         if (!r.symmetric && (int)r.lower.size() != r.nFaces)
             throw std::runtime_error("region '" + r.name + "': asymmetric matrix needs lower coefficients");
+        if (r.transient() && ((int)r.ddtCoeff.size() != r.nCells || (int)r.diagBase.size() != r.nCells ||
+                              (int)r.sourceBase.size() != r.nCells))
+            throw std::runtime_error("region '" + r.name + "': ddt arrays do not match nCells");
         for (int f = 0; f < r.nFaces; ++f) {
             int l = r.lAddr[f], u = r.uAddr[f];
             if (l < 0 || u >= r.nCells || l >= u)

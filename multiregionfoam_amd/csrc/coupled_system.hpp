@@ -18,6 +18,11 @@ struct LduRegion {
     std::vector<double> source;             // includes non-coupled boundaryCoeffs
     std::vector<double> psi;                // initial guess / solution
     bool symmetric = false;
+    // transient regions (empty when steady): the Euler ddt coefficient
+    // a = rho*c*V/dt per cell and the diagonal and source without the ddt
+    // term, so that diag = diagBase + a and source = sourceBase + a*psi
+    std::vector<double> ddtCoeff, diagBase, sourceBase;
+    bool transient() const { return !ddtCoeff.empty(); }
 };
 
 // One side of a monolithic interface: faces of this side gather the other

@@ -558,16 +558,6 @@ FvMatrixG divUpwind(const FoamRegion &R)
     return M;
 }
 
-// Euler ddt with coefficient rho (1 for fvm::ddt(T)): diag = rho V / dt,
-// source = rho V T_old / dt; steadyState adds nothing.
-void addDdt(const FoamRegion &R, double rDeltaT, double rho, FvMatrixG &M)
-{
-    const PolyMesh &m = R.mesh;
-    for (int c = 0; c < m.nCells; ++c) {
-        M.diag[c] += rDeltaT * rho * m.V[c];
-        M.source[c] += rDeltaT * rho * R.T[c] * m.V[c];
-    }
-}
 
 struct Rect {
     double lo[2], hi[2];
@@ -813,8 +803,9 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
     }
 }
 
-// the region's TEqn, as setCoupledEqns builds it
-FvMatrixG setCoupledEqns(const FoamRegion &R, double rDeltaT)
+// the region's TEqn as setCoupledEqns builds it, without the ddt term (added
+// by the caller as diag += a, source += a*T_old with a = rho*c*V/dt)
+FvMatrixG setCoupledEqns(const FoamRegion &R)
 {
     const PolyMesh &m = R.mesh;
     FvMatrixG lap = laplacian(R);
@@ -823,7 +814,6 @@ FvMatrixG setCoupledEqns(const FoamRegion &R, double rDeltaT)
     if (R.transport) {
         // rho*cp*(fvm::ddt(T) + fvm::div(phi,T)) == fvm::laplacian(k,T)
         FvMatrixG A = divUpwind(R);
-        if (rDeltaT > 0.0) addDdt(R, rDeltaT, 1.0, A);
         for (int c = 0; c < m.nCells; ++c) {
             E.diag[c] = R.rhoCp * A.diag[c] - lap.diag[c];
             E.source[c] = R.rhoCp * A.source[c] - lap.source[c];
@@ -839,12 +829,9 @@ FvMatrixG setCoupledEqns(const FoamRegion &R, double rDeltaT)
             }
     } else {
         // fvm::ddt(rho*cv, T) == fvm::laplacian(k, T)
-        FvMatrixG D;
-        D.init(m, false);
-        if (rDeltaT > 0.0) addDdt(R, rDeltaT, R.rhoCp, D);
         for (int c = 0; c < m.nCells; ++c) {
-            E.diag[c] = D.diag[c] - lap.diag[c];
-            E.source[c] = D.source[c] - lap.source[c];
+            E.diag[c] = 0.0 - lap.diag[c];
+            E.source[c] = 0.0 - lap.source[c];
         }
         for (int f = 0; f < m.nInternal; ++f) E.upper[f] = 0.0 - lap.upper[f];
         for (size_t p = 0; p < m.patches.size(); ++p)
@@ -968,7 +955,7 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
     for (size_t r = 0; r < R.size(); ++r) {
         const FoamRegion &rc = R[r];
         const PolyMesh &m = rc.mesh;
-        FvMatrixG E = setCoupledEqns(rc, rDeltaT[r]);
+        FvMatrixG E = setCoupledEqns(rc);
         LduRegion &L = sys.regions[r];
         L.name = m.name;
         L.nCells = m.nCells;
@@ -998,6 +985,18 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
                 InterfaceSide &S = s ? sys.interfaces[i].b : sys.interfaces[i].a;
                 S.coeff = E.boundaryCoeffs[sides[i][2 * s + 1]];
             }
+        // Euler ddt (fvm::ddt with coefficient rho*c): a = rho*c*V/dt on the
+        // diagonal, a*T_old in the source; a `backward` run continues from here
+        if (rDeltaT[r] > 0.0) {
+            L.diagBase = L.diag;
+            L.sourceBase = L.source;
+            L.ddtCoeff.resize(m.nCells);
+            for (int c = 0; c < m.nCells; ++c) {
+                L.ddtCoeff[c] = rDeltaT[r] * rc.rhoCp * m.V[c];
+                L.diag[c] = L.diagBase[c] + L.ddtCoeff[c];
+                L.source[c] = L.sourceBase[c] + L.ddtCoeff[c] * rc.T[c];
+            }
+        }
     }
     if (info) {
         info->volumes.clear();

@@ -659,6 +659,58 @@ hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K3
     MRFG_CHECK_LAUNCH();
 }
 
+// ---------------------------------------------------------------------------
+// Time stepping: the ddt terms of transient systems on the device
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_ddt_diag(MatView<double> A, const BlockEntry *blocks, int K,
+                                                     double coefft, const double *diagBase, const double *a,
+                                                     double *diag, double2 *prodPack, double *sumA)
+{
+    int sys, s, lane, p;
+    if (!blockRow(blocks, sys, s, lane, p)) return;
+    const double d = diagBase[p] + coefft * a[p];
+    diag[p] = d;
+    const int G = (K + 2) / 2;
+    ((double *)&prodPack[((size_t)s * G + K / 2) * 64 + lane])[K % 2] = d;
+    // lduMatrix::sumA: the diagonal, the face coefficients in face order (the
+    // row's ELL order; padding entries are zeros at the end), minus the
+    // interface coefficients in interface order
+    double r = d;
+    for (int k = A.ellOff[s]; k < A.ellOff[s + 1]; ++k) r += A.ellVal[(size_t)k * 64 + lane];
+    const unsigned long long m = A.ifMask[s];
+    if ((m >> lane) & 1ull) {
+        const int row = A.ifBase[s] + __popcll(m & ((1ull << lane) - 1ull));
+        for (int f = A.ifRowFace[row]; f < A.ifRowFace[row + 1]; ++f) r -= A.ifCoeff[f];
+    }
+    sumA[p] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ddt_source(const BlockEntry *blocks, int backward,
+                                                       const double *sourceBase, const double *a, const double *x,
+                                                       const double *xPrev, double *b)
+{
+    int sys, s, lane, p;
+    if (!blockRow(blocks, sys, s, lane, p)) return;
+    const double t = backward ? 2.0 * x[p] - 0.5 * xPrev[p] : x[p];
+    b[p] = sourceBase[p] + a[p] * t;
+}
+
+hipError_t launchDdtDiag(const MatView<double> &A, const BlockEntry *blocks, int nBlk, int K, double coefft,
+                         const double *diagBase, const double *a, double *diag, double2 *prodPack, double *sumA,
+                         hipStream_t st)
+{
+    k_ddt_diag<<<nBlk, kBlock, 0, st>>>(A, blocks, K, coefft, diagBase, a, diag, prodPack, sumA);
+    MRFG_CHECK_LAUNCH();
+}
+
+hipError_t launchDdtSource(const BlockEntry *blocks, int nBlk, bool backward, const double *sourceBase,
+                           const double *a, const double *x, const double *xPrev, double *b, hipStream_t st)
+{
+    k_ddt_source<<<nBlk, kBlock, 0, st>>>(blocks, backward ? 1 : 0, sourceBase, a, x, xPrev, b);
+    MRFG_CHECK_LAUNCH();
+}
+
 // Interface heat flow of one side of a monolithic interface,
 // Q = sum_f coeff_f * (Tn_f - To_f) with Tn_f = sum_p w_p x[pos_p] (the
 // patch-neighbour field through the GGI weights), as a double-double sum:

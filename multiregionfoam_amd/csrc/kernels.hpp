@@ -59,4 +59,15 @@ hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &
 hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st);
 hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st);
 
+// Time stepping of transient systems (rows of steady systems have a = 0):
+// diag = diagBase + coefft*a, written to the SpMV diagonal and to the factor
+// records' diagonal slot (record width K), and the row sums sumA recomputed
+// in the order of lduMatrix::sumA; source = sourceBase + a*x (Euler) or
+// sourceBase + a*(2 x - 0.5 xPrev) (backward, constant time step).
+hipError_t launchDdtDiag(const MatView<double> &A, const BlockEntry *blocks, int nBlk, int K, double coefft,
+                         const double *diagBase, const double *a, double *diag, double2 *prodPack, double *sumA,
+                         hipStream_t st);
+hipError_t launchDdtSource(const BlockEntry *blocks, int nBlk, bool backward, const double *sourceBase,
+                           const double *a, const double *x, const double *xPrev, double *b, hipStream_t st);
+
 }  // namespace mrfg

@@ -301,6 +301,42 @@ class OracleSystem:
                          nIterations=perf.nIterations, converged=bool(perf.converged),
                          singular=bool(perf.singular))
 
+    def set_values(self, r, diag=None, source=None):
+        """Overwrite region r's diagonal and / or source (in the arrays the
+        oracle solves with, and in self.regions)."""
+        R = self.sys.regions[r]
+        n = R.nCells
+        if diag is not None:
+            np.ctypeslib.as_array(R.diag, shape=(n,))[:] = diag
+            self.regions[r]["diag"] = np.array(diag, dtype=np.float64)
+        if source is not None:
+            np.ctypeslib.as_array(R.source, shape=(n,))[:] = source
+            self.regions[r]["source"] = np.array(source, dtype=np.float64)
+
+    def time_steps(self, ddt, n_steps, scheme="backward", psi0=None, **solve_kw):
+        """The time loop for a fixed flow: per step the ddt terms of the
+        transient regions (ddt[r] = dict(ddtCoeff, diagBase, sourceBase), or
+        None for a steady region) are rebuilt from the current and previous
+        fields -- diag = diagBase + c*a with c = 1 (Euler) or 3/2 (backward),
+        source = sourceBase + a*T (Euler) or a*(2 T - T_prev / 2) (backward,
+        constant time step; the first step is Euler, as backwardDdtScheme's)
+        -- and the system is solved from the current field.  Returns the
+        (psi, perf) of every step."""
+        x = [np.array(p, dtype=np.float64) for p in (psi0 or [R["psi"] for R in self.regions])]
+        prev, out = None, []
+        for k in range(n_steps):
+            bwd = scheme == "backward" and k >= 1
+            for r, D in enumerate(ddt):
+                if D is None:
+                    continue
+                a = D["ddtCoeff"]
+                t = 2.0 * x[r] - 0.5 * prev[r] if bwd else x[r]
+                self.set_values(r, diag=D["diagBase"] + (1.5 if bwd else 1.0) * a, source=D["sourceBase"] + a * t)
+            psi, perf = self.solve(psi0=x, **solve_kw)
+            prev, x = x, psi
+            out.append((psi, perf))
+        return out
+
     # ---- global (concatenated) sparse matrix, for brute-force checks ----
     def offsets(self):
         off = [0]

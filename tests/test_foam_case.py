@@ -255,3 +255,105 @@ def test_tutorial_gpu_matches_oracle(gpu, tutorial_dir):
     assert perf.n_iterations == pref["nIterations"]
     assert perf.final_residual == pref["finalResidual"]
     assert np.array_equal(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
+
+
+# ---------------------------------------------------------------------------
+# time stepping (ddt terms rebuilt per step; fixed flow)
+# ---------------------------------------------------------------------------
+
+def _ddt(case):
+    return [({k: R[k] for k in ("ddtCoeff", "diagBase", "sourceBase")} if "ddtCoeff" in R else None)
+            for R in case.regions()]
+
+
+def _conduct_pair():
+    """Two conducting boxes with a conformal interface and insulated outer
+    walls: the coupled matrix without the ddt terms has zero column sums."""
+    a = cases.region("hot", "conductTemperature", (8, 5, 1), (0.0, 0.0, 0.0), (0.8, 0.5, 0.1),
+                     k=dict(uniform=3.0), T0=350.0,
+                     patches=dict(ymin=dict(type="monolithicTemperature"), zmin=dict(type="empty"),
+                                  zmax=dict(type="empty")))
+    b = cases.region("cold", "conductTemperature", (8, 4, 1), (0.0, -0.4, 0.0), (0.8, 0.4, 0.1),
+                     k=dict(uniform=40.0), T0=300.0,
+                     patches=dict(ymax=dict(type="monolithicTemperature"), zmin=dict(type="empty"),
+                                  zmax=dict(type="empty")))
+    return cases.case([a, b], [cases.interface("hotcold", ("hot", "ymin"), ("cold", "ymax"), method="directMap")])
+
+
+def test_ddt_split(tutorial_dir):
+    """diag = diagBase + a and source = sourceBase + a*T0 exactly, with
+    a = rho*c*V/dt from bounding-box cell volumes."""
+    _lib_or_skip()
+    meta, meshes = ff.load_fixture()
+    c = mrf.Case.from_foam(tutorial_dir)
+    tp = meta["transportProperties"]
+    for g, (region, R) in enumerate(zip(("fluid", "solid"), c.regions())):
+        assert np.array_equal(R["diag"], R["diagBase"] + R["ddtCoeff"])
+        assert np.array_equal(R["source"], R["sourceBase"] + R["ddtCoeff"] * R["psi"])
+        _, _, clo, chi = _hex_geometry(meshes[region])
+        rhoC = tp[region]["rho"] * (tp[region]["cp"] if region == "fluid" else tp[region]["cv"])
+        np.testing.assert_allclose(R["ddtCoeff"], rhoC * np.prod(chi - clo, axis=1) / meta["deltaT"], rtol=1e-12)
+    s = mrf.Case.from_foam(tutorial_dir, ddt="steadyState")
+    assert all("ddtCoeff" not in R for R in s.regions())
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "backward"])
+def test_time_steps_conserve_energy(tmp_path, scheme):
+    """Insulated conduction pair: Euler keeps sum(a T) and backward keeps
+    sum(a (3 T_n+1 - T_n) / 2) from step to step, to the solver tolerance."""
+    _lib_or_skip()
+    ff.write_box_case(str(tmp_path), _conduct_pair(), deltaT=0.05)
+    c = mrf.Case.from_foam(str(tmp_path), ddt=scheme)
+    O = ob.OracleSystem.from_arrays(*_arrays(c))
+    ddt = _ddt(c)
+    a = np.concatenate([D["ddtCoeff"] for D in ddt])
+    T0 = np.concatenate([R["psi"] for R in c.regions()])
+    steps = O.time_steps(ddt, 6, scheme=scheme, tol=1e-15, maxIter=2000)
+    T = [T0] + [np.concatenate(p) for p, _ in steps]
+    assert np.abs(T[-1] - T[0]).max() > 1.0           # heat did flow
+    if scheme == "Euler":
+        e = [np.dot(a, t) for t in T]
+    else:
+        e = [np.dot(a, T[1])] + [np.dot(a, 1.5 * T[n + 1] - 0.5 * T[n]) for n in range(1, len(T) - 1)]
+    np.testing.assert_allclose(e, e[0], rtol=1e-12)
+
+
+def test_time_steps_reach_steady_state(tutorial_dir):
+    """With a very large time step the ddt terms vanish: Euler steps of the
+    tutorial approach the steady system's solution, the first one to 1e-6."""
+    _lib_or_skip()
+    import scipy.sparse.linalg as sla
+    s = mrf.Case.from_foam(tutorial_dir, ddt="steadyState")
+    Os = ob.OracleSystem.from_arrays(*_arrays(s))
+    Ts = sla.spsolve(Os.to_scipy().tocsc(), np.concatenate([r["source"] for r in Os.regions]))
+    c = mrf.Case.from_foam(tutorial_dir, ddt="Euler", deltaT=1e6)
+    O = ob.OracleSystem.from_arrays(*_arrays(c))
+    steps = O.time_steps(_ddt(c), 2, scheme="Euler", tol=1e-15, maxIter=2000)
+    err = [np.linalg.norm(np.concatenate(p) - Ts) / np.linalg.norm(Ts) for p, _ in steps]
+    assert err[0] < 1e-6 and err[1] < err[0]
+
+
+@pytest.mark.gpu
+def test_time_steps_gpu_match_oracle(gpu, tutorial_dir):
+    """Backward time steps of the tutorial (the first Euler) on the GPU,
+    ddt terms rebuilt on the device, equal the oracle's step by step; after
+    reset_psi a run starts anew."""
+    meta, _ = ff.load_fixture()
+    t = meta["Tcoupled"]
+    kw = dict(tol=float(t["tolerance"]), relTol=float(t["relTol"]), minIter=int(t["minIter"]),
+              maxIter=int(t["maxIter"]))
+    c = gpu.Case.from_foam(tutorial_dir)
+    A = gpu.CoupledFvMatrix(0)
+    A.add_case(c)
+    A.commit()
+    ctrl = gpu.controls(solver=t["solver"], tolerance=kw["tol"], relTol=kw["relTol"], minIter=kw["minIter"],
+                        maxIter=kw["maxIter"])
+    perf = A.time_steps(4, ddt="backward", ctrl=ctrl)
+    O = ob.OracleSystem.from_arrays(*_arrays(c))
+    ref = O.time_steps(_ddt(c), 4, scheme="backward", **kw)
+    for (p,), (_, pr) in zip(perf, ref):
+        assert p.n_iterations == pr["nIterations"] and p.final_residual == pr["finalResidual"]
+    assert np.array_equal(np.concatenate(A.psi(0)), np.concatenate(ref[-1][0]))
+    A.reset_psi()
+    again = A.time_steps(2, ddt="backward", ctrl=ctrl)
+    assert [q[0].final_residual for q in again] == [q[0].final_residual for q in perf[:2]]
