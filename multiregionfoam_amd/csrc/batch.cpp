@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 #include "kernels.hpp"
 
@@ -145,7 +146,27 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     sys_ = systems;
     lay_.clear();
     ifDev_.clear();
-    for (auto *s : systems) lay_.emplace_back(new SystemLayout(buildLayout(*s, preferLevels_)));
+    // layouts of several systems are built concurrently (each depends only
+    // on its own system)
+    lay_.resize(systems.size());
+    {
+        std::vector<std::thread> th;
+        std::vector<std::string> errs(systems.size());
+        for (size_t q = 0; q < systems.size(); ++q) {
+            auto job = [&, q] {
+                try {
+                    lay_[q].reset(new SystemLayout(buildLayout(*systems[q], preferLevels_)));
+                } catch (const std::exception &e) {
+                    errs[q] = e.what();
+                }
+            };
+            if (systems.size() == 1) job();
+            else th.emplace_back(job);
+        }
+        for (auto &t : th) t.join();
+        for (auto &e : errs)
+            if (!e.empty()) throw std::runtime_error(e);
+    }
     const int nS = (int)systems.size();
     sliceBegin_.assign(nS, 0);
     K_ = 3;

@@ -24,8 +24,6 @@
 // and the fast path has no divergent branch.
 #include <hip/hip_runtime.h>
 
-#include <functional>
-
 #include "dilu.hpp"
 
 namespace mrfg {
@@ -540,9 +538,8 @@ static hipError_t launchBox(const DiluView<T> &V, const SysState *st, const T *i
 // The general sweep on s and the box sweep on V.side when both exist and a
 // side stream is set (the box sweep forked after the work already on s and
 // joined before the next), else both on s.
-template <typename Gen, typename Box>
-static hipError_t bothSweeps(const DiluView<Gen> &V, hipStream_t s, const std::function<hipError_t(hipStream_t)> &gen,
-                             const std::function<hipError_t(hipStream_t)> &box)
+template <typename T, typename GenF, typename BoxF>
+static hipError_t bothSweeps(const DiluView<T> &V, hipStream_t s, GenF &&gen, BoxF &&box)
 {
     const bool fork = V.side && V.fork && V.join && V.nWaves > 0 && V.nBoxWaves > 0;
     if (!fork) {
@@ -562,7 +559,7 @@ template <typename T>
 hipError_t diluForward(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s)
 {
     using U = typename Bits<T>::U;
-    return bothSweeps<T, T>(
+    return bothSweeps(
         V, s, [&](hipStream_t q) { return launchK<T, kDiluFwd>(V, st, in, out, (U *)V.haloF, (U *)V.haloB, V.nHaloB, q); },
         [&](hipStream_t q) {
             return launchBox<T, kDiluFwd>(V, st, in, out, (U *)V.boxInF, (U *)V.boxInB, V.nBoxIn, q);
@@ -573,7 +570,7 @@ template <typename T>
 hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipStream_t s)
 {
     using U = typename Bits<T>::U;
-    return bothSweeps<T, T>(
+    return bothSweeps(
         V, s,
         [&](hipStream_t q) {
             return launchK<T, kDiluBwd>(V, st, inout, inout, (U *)V.haloB, (U *)V.haloF, V.nHaloF, q);

@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <stdexcept>
+#include <thread>
 
 namespace mrfg {
 
