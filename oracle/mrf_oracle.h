@@ -143,6 +143,13 @@ void orc_interface_flux(const orc_system *sys, int i, double *const *psi,
 int orc_solve(orc_system *sys, int solver, double tol, double relTol,
               int minIter, int maxIter, orc_perf *perf);
 
+/* The same BiCGStab solve on all host cores (mrf_oracle_mt.c): row-parallel
+   A*x, level-scheduled DILU sweeps (bit-identical to the sequential ones),
+   compensated sums merged across threads.  For the all-cores CPU baseline. */
+int orc_solve_mt(orc_system *sys, double tol, double relTol, int minIter, int maxIter, orc_perf *perf);
+/* threads orc_solve_mt uses (OMP_NUM_THREADS or the host's cores) */
+int orc_threads(void);
+
 /* Deterministic hash RNG used by the case generator (splitmix64). */
 double orc_u01(uint64_t seed, uint64_t index);
 

@@ -98,6 +98,8 @@ def lib():
         L.orc_dilu_factor.argtypes = [C.POINTER(System), PP]
         L.orc_dilu_precondition.argtypes = [C.POINTER(System), PP, PP, PP]
         L.orc_interface_flux.argtypes = [C.POINTER(System), C.c_int, PP] + [C.POINTER(C.c_double)] * 4
+        L.orc_solve_mt.argtypes = [C.POINTER(System), C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(Perf)]
+        L.orc_threads.restype = C.c_int
         L.orc_u01.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_u01.restype = C.c_double
         _lib = L
@@ -336,6 +338,20 @@ class OracleSystem:
             prev, x = x, psi
             out.append((psi, perf))
         return out
+
+    def solve_mt(self, tol=1e-10, relTol=0.0, minIter=0, maxIter=1000, psi0=None):
+        """BiCGStab on all host cores (orc_solve_mt); returns (psi list,
+        perf dict, threads)."""
+        s = self.sys
+        for r in range(s.nRegions):
+            src = psi0[r] if psi0 is not None else self.regions[r]["psi"]
+            np.ctypeslib.as_array(s.regions[r].psi, shape=(s.regions[r].nCells,))[:] = src
+        perf = Perf()
+        lib().orc_solve_mt(C.byref(s), tol, relTol, minIter, maxIter, C.byref(perf))
+        psi = [_arr(s.regions[r].psi, s.regions[r].nCells, np.float64) for r in range(s.nRegions)]
+        return psi, dict(initialResidual=perf.initialResidual, finalResidual=perf.finalResidual,
+                         nIterations=perf.nIterations, converged=bool(perf.converged),
+                         singular=bool(perf.singular)), lib().orc_threads()
 
     # ---- global (concatenated) sparse matrix, for brute-force checks ----
     def offsets(self):

@@ -353,3 +353,18 @@ def test_golden_direct_solves():
         psi, perf = O.solve(tol=1e-14, maxIter=5000)
         assert perf["converged"]
         assert rel(np.concatenate(psi), ref) < 1e-10, entry["name"]
+
+
+@pytest.mark.parametrize("name", ["plate2d", "duct_ggi"])
+def test_multithreaded_solve_matches_sequential(name):
+    """orc_solve_mt (all host cores; level-scheduled DILU, row-parallel A*x)
+    restates orc_solve: the same iterations (+-1, from the last bit of the
+    merged sums) and the same field."""
+    case = {"plate2d": cases.plate2d(), "duct_ggi": cases.duct3d(nf=(24, 12, 12), ns=(30, 15, 5))}[name]
+    O = ob.OracleSystem(case)
+    psi, perf = O.solve(tol=1e-12, maxIter=2000)
+    psi_mt, perf_mt, threads = O.solve_mt(tol=1e-12, maxIter=2000)
+    assert threads >= 1 and perf_mt["converged"]
+    assert abs(perf_mt["nIterations"] - perf["nIterations"]) <= 1
+    a, b = np.concatenate(psi_mt), np.concatenate(psi)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12
