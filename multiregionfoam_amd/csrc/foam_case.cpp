@@ -19,9 +19,9 @@
 //
 // Limits (each raises an error): internal faces must be in upper-triangular
 // order; non-orthogonality above kMaxNonOrthDeg (the explicit non-orthogonal
-// correction of `corrected` snGrad is not assembled); coupled patches must be
-// planar and normal to a coordinate axis with faces that are axis-aligned
-// rectangles (blockMesh hex meshes), which makes the GGI overlaps exact.
+// correction of `corrected` snGrad is not assembled); the two patches of an
+// interface must lie in one plane (any orientation) and have convex faces,
+// whose overlaps in that plane give the GGI weights.
 #include "coupled_system.hpp"
 #include "dictionary.hpp"
 
@@ -559,58 +559,134 @@ FvMatrixG divUpwind(const FoamRegion &R)
 }
 
 
-struct Rect {
-    double lo[2], hi[2];
+// The plane of a coupled patch: origin, unit normal (of the total area
+// vector) and two in-plane unit vectors.
+struct Frame {
+    Vec o, n, e1, e2;
 };
 
-// a planar patch normal to axis `axis`: each face's rectangle in the two
-// tangential coordinates (faces must be axis-aligned rectangles)
-std::vector<Rect> patchRects(const PolyMesh &m, const Patch &P, int &axis)
+using P2 = std::array<double, 2>;
+
+// A face projected into the interface frame: counter-clockwise vertices, its
+// bounding box and its area.
+struct Poly {
+    std::vector<P2> v;
+    double lo[2], hi[2], area;
+};
+
+inline double cross2(const P2 &a, const P2 &b, const P2 &c)   // (b - a) x (c - a)
 {
-    Vec sum{0, 0, 0};
-    double area = 0.0;
-    for (int i = 0; i < P.size; ++i) {
-        sum = sum + m.Sf[P.start + i];
-        area += m.magSf[P.start + i];
-    }
-    axis = 0;
-    for (int a = 1; a < 3; ++a)
-        if (std::fabs(sum[a]) > std::fabs(sum[axis])) axis = a;
-    if (std::fabs(sum[axis]) < (1.0 - 1e-9) * area)
-        throw FoamError("patch '" + P.name + "' of region '" + m.name + "' is not planar and axis-normal");
-    const int b = axis == 0 ? 1 : 0, c = axis == 2 ? 1 : 2;
-    std::vector<Rect> r(P.size);
-    for (int i = 0; i < P.size; ++i) {
-        const int f = P.start + i;
-        Rect q{{1e300, 1e300}, {-1e300, -1e300}};
-        for (int j = m.faceOff[f]; j < m.faceOff[f + 1]; ++j) {
-            const Vec &p = m.points[m.faceVerts[j]];
-            q.lo[0] = std::fmin(q.lo[0], p[b]);
-            q.hi[0] = std::fmax(q.hi[0], p[b]);
-            q.lo[1] = std::fmin(q.lo[1], p[c]);
-            q.hi[1] = std::fmax(q.hi[1], p[c]);
-        }
-        double ra = (q.hi[0] - q.lo[0]) * (q.hi[1] - q.lo[1]);
-        if (std::fabs(ra - m.magSf[f]) > 1e-9 * m.magSf[f])
-            throw FoamError("patch '" + P.name + "' of region '" + m.name + "': face " + std::to_string(i) +
-                            " is not an axis-aligned rectangle");
-        r[i] = q;
-    }
-    return r;
+    return (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
 }
 
-// GGI weights of side A's faces gathering side B's faces: exact rectangle
-// overlap areas rescaled to sum 1 per covered face; overlaps below 1e-10 of
-// the smaller face are dropped.  A bucket grid over B keeps the search local.
-void ggiWeights(const std::vector<Rect> &A, const std::vector<Rect> &B, InterfaceSide &S)
+double polyArea(const std::vector<P2> &v)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < v.size(); ++i) {
+        const P2 &p = v[i], &q = v[(i + 1) % v.size()];
+        s += p[0] * q[1] - q[0] * p[1];
+    }
+    return 0.5 * s;
+}
+
+Frame patchFrame(const PolyMesh &m, const Patch &P)
+{
+    Vec sum{0, 0, 0}, c{0, 0, 0};
+    double area = 0.0;
+    for (int i = 0; i < P.size; ++i) {
+        const int f = P.start + i;
+        sum = sum + m.Sf[f];
+        area += m.magSf[f];
+        c = c + m.magSf[f] * m.Cf[f];
+    }
+    if (!(area > 0.0)) throw FoamError("coupled patch '" + P.name + "' of region '" + m.name + "' has no faces");
+    Frame F;
+    F.o = (1.0 / area) * c;
+    F.n = (1.0 / mag(sum)) * sum;
+    // every face parallel to the plane and on it
+    if (mag(sum) < (1.0 - 1e-9) * area)
+        throw FoamError("coupled patch '" + P.name + "' of region '" + m.name + "' is not planar");
+    for (int i = 0; i < P.size; ++i) {
+        const int f = P.start + i;
+        if (std::fabs(dot(F.n, m.Cf[f] - F.o)) > 1e-6 * std::sqrt(m.magSf[f]))
+            throw FoamError("coupled patch '" + P.name + "' of region '" + m.name + "' is not planar");
+    }
+    int a = 0;
+    for (int q = 1; q < 3; ++q)
+        if (std::fabs(F.n[q]) < std::fabs(F.n[a])) a = q;
+    Vec u{0, 0, 0};
+    u[a] = 1.0;
+    Vec e1 = u - dot(u, F.n) * F.n;
+    F.e1 = (1.0 / mag(e1)) * e1;
+    F.e2 = cross(F.n, F.e1);
+    return F;
+}
+
+// the patch's faces in frame F (counter-clockwise; faces must be convex)
+std::vector<Poly> patchPolys(const PolyMesh &m, const Patch &P, const Frame &F)
+{
+    std::vector<Poly> out(P.size);
+    for (int i = 0; i < P.size; ++i) {
+        const int f = P.start + i;
+        Poly &q = out[i];
+        for (int j = m.faceOff[f]; j < m.faceOff[f + 1]; ++j) {
+            const Vec d = m.points[m.faceVerts[j]] - F.o;
+            q.v.push_back(P2{dot(d, F.e1), dot(d, F.e2)});
+        }
+        if (polyArea(q.v) < 0.0) std::reverse(q.v.begin(), q.v.end());
+        q.area = polyArea(q.v);
+        const size_t n = q.v.size();
+        for (size_t k = 0; k < n; ++k)
+            if (cross2(q.v[k], q.v[(k + 1) % n], q.v[(k + 2) % n]) < -1e-12 * q.area)
+                throw FoamError("coupled patch '" + P.name + "' of region '" + m.name + "': face " + std::to_string(i) +
+                                " is not convex");
+        q.lo[0] = q.lo[1] = 1e300;
+        q.hi[0] = q.hi[1] = -1e300;
+        for (const P2 &p : q.v)
+            for (int d = 0; d < 2; ++d) {
+                q.lo[d] = std::fmin(q.lo[d], p[d]);
+                q.hi[d] = std::fmax(q.hi[d], p[d]);
+            }
+    }
+    return out;
+}
+
+// Area of the intersection of two convex counter-clockwise polygons
+// (Sutherland-Hodgman: the first clipped by each edge of the second).
+double overlapArea(const std::vector<P2> &a, const std::vector<P2> &b)
+{
+    std::vector<P2> cur = a, nxt;
+    for (size_t i = 0; i < b.size() && !cur.empty(); ++i) {
+        const P2 &c0 = b[i], &c1 = b[(i + 1) % b.size()];
+        nxt.clear();
+        for (size_t j = 0; j < cur.size(); ++j) {
+            const P2 &s0 = cur[j], &s1 = cur[(j + 1) % cur.size()];
+            const double d0 = cross2(c0, c1, s0), d1 = cross2(c0, c1, s1);
+            if (d0 >= 0.0) nxt.push_back(s0);
+            if ((d0 >= 0.0) != (d1 >= 0.0)) {
+                const double t = d0 / (d0 - d1);
+                nxt.push_back(P2{s0[0] + t * (s1[0] - s0[0]), s0[1] + t * (s1[1] - s0[1])});
+            }
+        }
+        cur.swap(nxt);
+    }
+    return cur.size() < 3 ? 0.0 : std::fabs(polyArea(cur));
+}
+
+// GGI weights of side A's faces gathering side B's faces: overlap areas of
+// the faces in the interface plane, rescaled to sum 1 per covered face;
+// overlaps below 1e-10 of the smaller face are dropped.  A bucket grid over
+// B's bounding boxes keeps the search local.
+void ggiWeights(const std::vector<Poly> &A, const std::vector<Poly> &B, InterfaceSide &S)
 {
     double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300}, size = 0.0;
-    for (const Rect &r : B)
+    for (const Poly &r : B) {
         for (int d = 0; d < 2; ++d) {
             lo[d] = std::fmin(lo[d], r.lo[d]);
             hi[d] = std::fmax(hi[d], r.hi[d]);
         }
-    for (const Rect &r : B) size += std::sqrt((r.hi[0] - r.lo[0]) * (r.hi[1] - r.lo[1]));
+        size += std::sqrt(r.area);
+    }
     size = B.empty() ? 1.0 : 2.0 * size / (double)B.size();
     int nb[2];
     for (int d = 0; d < 2; ++d) nb[d] = std::max(1, std::min(4096, (int)std::ceil((hi[d] - lo[d]) / size)));
@@ -627,7 +703,7 @@ void ggiWeights(const std::vector<Rect> &A, const std::vector<Rect> &B, Interfac
     S.w.clear();
     std::vector<int> cand;
     for (size_t i = 0; i < A.size(); ++i) {
-        const Rect &a = A[i];
+        const Poly &a = A[i];
         S.off[i] = (int32_t)S.w.size();
         cand.clear();
         for (int y = bucket(a.lo[1], 1); y <= bucket(a.hi[1], 1); ++y)
@@ -635,22 +711,18 @@ void ggiWeights(const std::vector<Rect> &A, const std::vector<Rect> &B, Interfac
                 for (int j : grid[(size_t)y * nb[0] + x]) cand.push_back(j);
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-        const double areaA = (a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]);
         const size_t p0 = S.w.size();
         double sum = 0.0;
         for (int j : cand) {
-            const Rect &b = B[j];
-            double ox = std::fmin(a.hi[0], b.hi[0]) - std::fmax(a.lo[0], b.lo[0]);
-            double oy = std::fmin(a.hi[1], b.hi[1]) - std::fmax(a.lo[1], b.lo[1]);
-            if (ox <= 0.0 || oy <= 0.0) continue;
-            double ar = ox * oy;
-            double areaB = (b.hi[0] - b.lo[0]) * (b.hi[1] - b.lo[1]);
-            if (ar <= 1e-10 * std::fmin(areaA, areaB)) continue;
+            const Poly &b = B[j];
+            if (b.lo[0] >= a.hi[0] || b.hi[0] <= a.lo[0] || b.lo[1] >= a.hi[1] || b.hi[1] <= a.lo[1]) continue;
+            const double ar = overlapArea(a.v, b.v);
+            if (ar <= 1e-10 * std::fmin(a.area, b.area)) continue;
             S.idx.push_back(j);
             S.w.push_back(ar);
             sum += ar;
         }
-        if (sum > 1e-12 * areaA) {
+        if (sum > 1e-12 * a.area) {
             for (size_t q = p0; q < S.w.size(); ++q) S.w[q] = S.w[q] / sum;
         } else {
             S.idx.resize(p0);
@@ -929,10 +1001,15 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
                     if (R[reg[s]].bcT[pat[s]] != BcT::Coupled)
                         throw FoamError(ifile + ": patch '" + pn + "' of '" + rn + "' is not monolithicTemperature");
                 }
-                int axA, axB;
-                auto rA = patchRects(R[reg[0]].mesh, R[reg[0]].mesh.patches[pat[0]], axA);
-                auto rB = patchRects(R[reg[1]].mesh, R[reg[1]].mesh.patches[pat[1]], axB);
-                if (axA != axB) throw FoamError(ifile + ": coupled patches of '" + it.dict->name + "' are not parallel");
+                // both sides in the frame of side A's plane (side B faces the
+                // other way and must lie in the same plane)
+                const Patch &PA = R[reg[0]].mesh.patches[pat[0]], &PB = R[reg[1]].mesh.patches[pat[1]];
+                const Frame FA = patchFrame(R[reg[0]].mesh, PA), FB = patchFrame(R[reg[1]].mesh, PB);
+                if (dot(FA.n, FB.n) > -(1.0 - 1e-9) ||
+                    std::fabs(dot(FA.n, FB.o - FA.o)) > 1e-6 * std::sqrt(R[reg[0]].mesh.magSf[PA.start]))
+                    throw FoamError(ifile + ": coupled patches of '" + it.dict->name + "' are not in one plane");
+                auto rA = patchPolys(R[reg[0]].mesh, PA, FA);
+                auto rB = patchPolys(R[reg[1]].mesh, PB, FA);
                 CoupledInterface I;
                 I.name = it.dict->name;
                 I.a.region = reg[0];

@@ -206,10 +206,21 @@ def box_mesh(n, origin, length):
     return np.array(points), faces, owner, neighbour, sides
 
 
-def write_box_case(dest, case, deltaT=1.0):
+def rotation(axis, degrees):
+    """Rotation matrix about `axis` (Rodrigues)."""
+    k = np.asarray(axis, dtype=np.float64)
+    k = k / np.linalg.norm(k)
+    t = np.radians(degrees)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * (K @ K)
+
+
+def write_box_case(dest, case, deltaT=1.0, rotate=None):
     """Write a case of multiregionfoam_amd.cases (uniform k, fixedValue /
     zeroGradient / empty / monolithicTemperature patches, steady) as a
-    polyMesh case directory."""
+    polyMesh case directory; `rotate` = (axis, degrees) turns the whole case
+    (points and velocities) about the origin."""
+    Rm = rotation(*rotate) if rotate else None
     regions_txt = " ".join("(%s (%s))" % (r["name"], r["type"]) for r in case["regions"])
     write_dict(os.path.join(dest, "constant", "multiRegionProperties"), "regions (%s);\n" % regions_txt)
     pairs = " ".join("%s { coupledPatchPair ((%s %s) (%s %s)); }" % (i["name"], *i["A"], *i["B"])
@@ -219,6 +230,8 @@ def write_box_case(dest, case, deltaT=1.0):
     write_dict(os.path.join(dest, "system", "controlDict"), "deltaT %s;\n" % _num(deltaT))
     for r in case["regions"]:
         pts, faces, owner, neighbour, sides = box_mesh(r["nCells"], r["origin"], r["length"])
+        if Rm is not None:
+            pts = pts @ Rm.T
         patches, allf, start = [], list(faces), len(faces)
         for name in SIDES:
             p = r["patches"][name]
@@ -272,4 +285,11 @@ def write_box_case(dest, case, deltaT=1.0):
                     Up[name] = {"type": "zeroGradient"}
                 else:
                     Up[name] = {"type": "fixedValue", "value": [0.0, 0.0, 0.0]}
+            if Rm is not None:
+                U = U @ Rm.T
+                for p in Up.values():
+                    if "value" in p:
+                        p["value"] = (np.atleast_2d(np.asarray(p["value"], dtype=np.float64)) @ Rm.T)
+                        if p["value"].shape[0] == 1:
+                            p["value"] = list(p["value"][0])
             write_field(os.path.join(dest, "0", r["name"], "U"), "volVectorField", U, Up)
