@@ -209,8 +209,9 @@ int mrfg_case_interface(const mrfg_case *c, int iface,
    (conductTemperature.C:135-152, transportTemperature.C:129-150) with
    phi = interpolate(U) & Sf (transportTemperature.C:86-94); a `backward`
    ddt is taken at its first step (Euler).  Meshes must be orthogonal to
-   1 degree, coupled patches planar, axis-normal, of axis-aligned
-   rectangular faces. */
+   1 degree; the two patches of an interface must lie in one plane (any
+   orientation) and have convex faces, whose overlap areas in that plane,
+   rescaled per face, are the GGI weights (ggiInterfaceToInterfaceMapping.C:62-77). */
 int mrfg_case_read_foam(const char *case_dir, const char *options, mrfg_case **out, char *err, int err_len);
 /* Add a built case as a new system of the context (begin_system +
    set_region + set_interface). */
