@@ -114,8 +114,15 @@ def load_fixture():
     return meta, meshes
 
 
-def write_fixture_case(dest):
+def write_fixture_case(dest, params=None):
+    """params: the test suite's parameter map (genCases.py update_params),
+    dict(kf=..., cp=...): the fluid's conductivity (transportProperties and
+    0/fluid/k) and specific heat."""
     meta, meshes = load_fixture()
+    if params:
+        meta["transportProperties"]["fluid"]["k"] = float(params["kf"])
+        meta["transportProperties"]["fluid"]["cp"] = float(params["cp"])
+        meta["fields"]["fluid"]["k"]["internal"] = float(params["kf"])
     regions_txt = " ".join("(%s (%s))" % (r, " ".join(t)) for r, t in meta["regions"])
     write_dict(os.path.join(dest, "constant", "multiRegionProperties"), "regions (%s);\n" % regions_txt)
     pairs = " ".join("%s%s { coupledPatchPair ((%s %s) (%s %s)); }" % (a[0], b[0], a[0], a[1], b[0], b[1])

@@ -20,7 +20,9 @@ import re
 import numpy as np
 
 CASE = "/root/reference/tutorials/conjugateHeatTransfer/flowOverHeatedPlate"
+SUITE_REFS = "/root/reference/tutorials/conjugateHeatTransfer/flowOverHeatedPlate_testSuite/references"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "flowOverHeatedPlate.npz")
+CSV_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "vynnycky")
 
 
 def strip(text):
@@ -163,6 +165,14 @@ def main():
     arrays["meta_json"] = np.array(json.dumps(meta, sort_keys=True))
     np.savez_compressed(OUT, **arrays)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
+    # the test suite's reference curves theta(x) (Vynnycky et al. 1998),
+    # data files copied as they are
+    os.makedirs(CSV_DIR, exist_ok=True)
+    for f in sorted(os.listdir(SUITE_REFS)):
+        if f.endswith(".csv"):
+            with open(os.path.join(SUITE_REFS, f)) as src, open(os.path.join(CSV_DIR, f), "w") as dst:
+                dst.write(src.read())
+    print("copied the reference curves to", CSV_DIR)
 
 
 if __name__ == "__main__":
