@@ -171,6 +171,43 @@ int mrfg_set_psi(mrfg_ctx *ctx, int sys, int region, const double *psi);
    A conserving coupled solution gives q_a + q_b = 0. */
 int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b);
 
+/* ---- finite-volume assembly on the device (cases read from disk) ------ */
+
+/* Replace one field of region `region` of committed system `sys`, a case
+   read by mrfg_case_read_foam (the fields setCoupledEqns reads):
+   "k"     conductivity, n = n_cells values (positive);
+   "U"     velocity, n = 3 * n_cells (x y z per cell; transportTemperature
+           regions only; phi = linearInterpolate(U) & Sf is formed on the
+           device, transportTemperature.C:86-94);
+   "T"     temperature, n = n_cells: the old-time field of the ddt term and
+           the system's initial guess for the region;
+   "rhoCp" rho*cp (rho*cv for conductTemperature), n = 1.
+   The field takes effect at the next mrfg_assemble. */
+int mrfg_set_field(mrfg_ctx *ctx, int sys, int region, const char *name, const double *values,
+                   long long n);
+
+/* Assemble, on the device, every committed system read from disk from its
+   current fields, and put the result in place of its committed matrix
+   (SpMV values, DILU records, diagonal, source, interface coefficients,
+   ddt terms; no matrix crosses the bus).  This is the assembly half of
+   assembleAndSolveCoupledMatrix (multiRegionSystem.C:115 rg.setCoupledEqns()
+   per region, then coupledEqns.set at :143-145): the region equations of
+   transportTemperature.C:129-150 (upwind convection, Gauss-linear
+   Laplacian, fixedValue / zeroGradient / empty / monolithicTemperature
+   patches), the interface conductivity of
+   monolithicThermalDiffusivityFvPatchScalarField.C:168 (calcThermalDiffusivity,
+   radiation off) and the Euler ddt term.  The result equals the host
+   reader's assembly of the same fields bit for bit. */
+int mrfg_assemble(mrfg_ctx *ctx);
+
+/* The device-assembled LDU matrix of one region in its own cell and face
+   order (diag with patch internalCoeffs, source with non-coupled
+   boundaryCoeffs; lower == upper for a symmetric region), and one side's
+   interface coefficients (side 0 = a, 1 = b).  After mrfg_assemble. */
+int mrfg_get_region_matrix(mrfg_ctx *ctx, int sys, int region, double *diag, double *lower,
+                           double *upper, double *source);
+int mrfg_get_interface_coeff(mrfg_ctx *ctx, int sys, int iface, int side, double *coeff);
+
 /* ---- case builder ------------------------------------------------------ */
 
 /* Assemble a case dictionary (regions of transportTemperature /

@@ -109,6 +109,10 @@ def lib():
         "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
         "mrfg_copy_probe": ([vp, C.c_longlong, C.c_int, F64P], C.c_int),
         "mrfg_interface_flux": ([vp, C.c_int, C.c_int, F64P, F64P], C.c_int),
+        "mrfg_set_field": ([vp, C.c_int, C.c_int, C.c_char_p, F64P, C.c_longlong], C.c_int),
+        "mrfg_assemble": ([vp], C.c_int),
+        "mrfg_get_region_matrix": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, F64P], C.c_int),
+        "mrfg_get_interface_coeff": ([vp, C.c_int, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_server_create": ([C.POINTER(vp), C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
                                 C.c_int], C.c_int),
         "mrfg_server_destroy": ([vp], None),
@@ -134,6 +138,7 @@ EXPORTED = (
     "mrfg_case_region mrfg_case_region_ddt "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
+    "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
     "mrfg_server_get_psi mrfg_server_last_error").split()
 
@@ -311,6 +316,8 @@ class CoupledFvMatrix:
             raise MrfgError(rc, "cannot create a context on device %d (is a GPU present?)" % device)
         self._h = h
         self._sizes = []
+        self._faces = []
+        self._ifaces = []
 
     def __del__(self):
         try:
@@ -332,7 +339,10 @@ class CoupledFvMatrix:
             case = Case(case)
         sid = C.c_int()
         self._check(lib().mrfg_add_case(self._h, case.handle, C.byref(sid)))
-        self._sizes.append([r["nCells"] for r in case.regions()])
+        regions = case.regions()
+        self._sizes.append([r["nCells"] for r in regions])
+        self._faces.append([r["nFaces"] for r in regions])
+        self._ifaces.append([(I["nA"], I["nB"]) for I in case.interfaces()])
         return sid.value
 
     def add_system(self, regions, interfaces):
@@ -366,6 +376,8 @@ class CoupledFvMatrix:
                 int(I["nB"]), _p(a["fcB"], C.c_int32), _p(a["coeffB"], C.c_double), _p(a["offB"], C.c_int32),
                 _p(a["idxB"], C.c_int32), _p(a["wB"], C.c_double)))
         self._sizes.append([int(R["nCells"]) for R in regions])
+        self._faces.append([int(R["nFaces"]) for R in regions])
+        self._ifaces.append([(int(I["nA"]), int(I["nB"])) for I in interfaces])
         return s
 
     def commit(self, schedule="auto"):
@@ -410,6 +422,34 @@ class CoupledFvMatrix:
                                           bool(p.singular), p.n_outer, p.seconds, p.bytes_moved)
                         for p in perf[k * n:(k + 1) * n]])
         return out
+
+    # ---- finite-volume assembly on the device (cases read from disk) --------
+    def set_field(self, sys, region, name, values):
+        """Replace a field of a system read from disk, for the next
+        :meth:`assemble`: "k" (per cell), "U" (n_cells x 3, transport
+        regions), "T" (per cell; also the initial guess), "rhoCp" (scalar)."""
+        a = np.ascontiguousarray(np.asarray(values, dtype=np.float64).ravel())
+        self._check(lib().mrfg_set_field(self._h, sys, region, _text(name), _p(a, C.c_double), a.size))
+
+    def assemble(self):
+        """Assemble every system read from disk from its current fields on
+        the device (setCoupledEqns + interface coefficients + ddt), in place
+        of the committed matrices."""
+        self._check(lib().mrfg_assemble(self._h))
+
+    def region_matrix(self, sys, region):
+        """The device-assembled LDU arrays of one region (after assemble)."""
+        n, nf = self._sizes[sys][region], self._faces[sys][region]
+        d, lo, up, src = np.zeros(n), np.zeros(nf), np.zeros(nf), np.zeros(n)
+        self._check(lib().mrfg_get_region_matrix(self._h, sys, region, _p(d, C.c_double), _p(lo, C.c_double),
+                                                 _p(up, C.c_double), _p(src, C.c_double)))
+        return dict(diag=d, lower=lo, upper=up, source=src)
+
+    def interface_coeff(self, sys, iface, side):
+        """The device-assembled coefficients of one interface side (0 = a, 1 = b)."""
+        c = np.zeros(self._ifaces[sys][iface][side])
+        self._check(lib().mrfg_get_interface_coeff(self._h, sys, iface, side, _p(c, C.c_double)))
+        return c
 
     def psi(self, sys=0):
         out = []

@@ -57,6 +57,8 @@ template struct DevBuf<double>;
 template struct DevBuf<float>;
 template struct DevBuf<int32_t>;
 template struct DevBuf<unsigned long long>;
+template struct DevBuf<int8_t>;
+template struct DevBuf<FvRegionParams>;
 
 // ---------------------------------------------------------------------------
 // fvSolution solver entry
@@ -457,12 +459,15 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     partials_.alloc((size_t)std::max(1, nBlk_) * 8);   // 4 double-double sums per block
     factorised_ = false;
     haveFloat_ = false;   // FP32 copies are rebuilt (in the buffers kept) on demand
+    hostStale_ = false;
+    buildFv();
     HIP_OK(hipStreamSynchronize(st));
 }
 
 void Batch::ensureFloat()
 {
     if (haveFloat_) return;
+    if (hostStale_) refreshHostCopies();
     auto toF = [](const std::vector<double> &a) {
         std::vector<float> b(a.size());
         for (size_t i = 0; i < a.size(); ++i) b[i] = (float)a[i];
@@ -936,6 +941,14 @@ void Batch::interfaceFlux(int sys, int iface, double *qa, double *qb)
             slot->off.upload(O.off, stream_);
             slot->coeff.upload(O.coeff.empty() ? std::vector<double>{0.0} : O.coeff, stream_);
             slot->w.upload(O.w.empty() ? std::vector<double>{0.0} : O.w, stream_);
+        }
+        if (hasModel(sys) && fv_->assembled && slot->nFaces > 0) {
+            // the coefficients assembled on the device
+            size_t off = fv_->sideBase[sys];
+            for (int i = 0; i < iface; ++i) off += S.interfaces[i].a.nFaces() + S.interfaces[i].b.nFaces();
+            if (q) off += I.a.nFaces();
+            HIP_OK(hipMemcpyAsync(slot->coeff.p, fv_->ifCoeff.p + off, slot->nFaces * sizeof(double),
+                                  hipMemcpyDeviceToDevice, stream_));
         }
         const int nBlk = (slot->nFaces + 255) / 256;
         if ((size_t)nBlk * 8 > partials_.n) throw std::runtime_error("interface larger than the reduction buffer");

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "assembly.hpp"
 #include "coupled_system.hpp"
 #include "device.hpp"
 #include "dictionary.hpp"
@@ -73,6 +74,22 @@ template <typename T>
 struct DevMatrix {
     DevBuf<T> ellVal, diag, ifCoeff, ifPairW;
     DevBuf<typename Pack<T>::V> fPack, bPack;   // DILU records
+};
+
+// The finite-volume models of a batch's systems read from disk, numbered
+// batch-wide (systems without a model contribute nothing), with the maps
+// that put assembled values into the solver's arrays.
+struct FvBatch {
+    std::vector<int> cellBase, faceBase, sideBase, regionBase;   // per system, -1 without a model
+    std::vector<FvRegionParams> hRegion;
+    int nCells = 0, nFaces = 0, nSideFaces = 0;
+    DevBuf<FvRegionParams> region;
+    DevBuf<int32_t> cellRegion, faceRegion, own, nei, cfOff, cf, cbOff, bSide, sCell, sOff, sPairCell;
+    DevBuf<int8_t> bT, bU, bKFixed;
+    DevBuf<double> w, dc, magSf, Sf, V, k, U, T, bNDist, bMagSf, bSf, bTb, bUb, bKb, sNDist, sW, sPairNDist;
+    DevBuf<double> lapU, aLo, aUp, faceVal, sK, sDelta, ifCoeff, diag, source, ddtA, diagBase, srcBase;
+    DevBuf<int32_t> cellOfPos, ellSrc, fSrc, bSrc, ifSrc;
+    bool assembled = false;
 };
 
 class Batch {
@@ -137,6 +154,20 @@ public:
     // at the next setSystems.
     void setPreferLevels(bool v) { preferLevels_ = v; }
 
+    // Finite-volume models (systems read from disk).  setField replaces a
+    // field of one region: "k" (per cell), "U" (3 per cell, transport
+    // regions), "T" (per cell; also the system's initial guess for that
+    // region, as when the case is read), "rhoCp" (one value).  assemble()
+    // re-assembles every system that has a model from its current fields,
+    // on the device, and puts the values in place; nothing is uploaded but
+    // the fields set since.  The getters return the device-assembled region
+    // matrix and interface coefficients in the original order.
+    bool hasModel(int sys) const;
+    void setField(int sys, int region, const std::string &name, const double *v, size_t n);
+    void assemble();
+    void getRegionMatrix(int sys, int region, double *diag, double *lower, double *upper, double *source);
+    void getInterfaceCoeff(int sys, int iface, int side, double *coeff);
+
 private:
     void ensureFloat();
     void factorise();
@@ -157,6 +188,9 @@ private:
     void bindKrylov(Krylov<T> &K);
     int posBegin(int sys) const { return sliceBegin_[sys] * kSlice; }
     void setDdtDiag(double coefft);
+    void buildFv();
+    // the host copies of the FP64 values after the device changed them
+    void refreshHostCopies();
 
     int device_;
     hipStream_t stream_ = nullptr;
@@ -208,6 +242,8 @@ private:
     int stepsDone_ = 0;
     double diagCoefft_ = 1.0;
     std::vector<SysInfo> hInfo_;
+    std::unique_ptr<FvBatch> fv_;
+    bool hostStale_ = false;
 };
 
 }  // namespace mrfg

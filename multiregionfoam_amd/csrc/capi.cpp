@@ -531,6 +531,56 @@ int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *
     });
 }
 
+// ---- finite-volume assembly ---------------------------------------------------
+
+int mrfg_set_field(mrfg_ctx *ctx, int sys, int region, const char *name, const double *values, long long n)
+{
+    if (!ctx || !name || !values || n < 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        mrfg::Batch &B = committedBatch(ctx);
+        if (!B.hasModel(sys)) throw ArgError("system has no finite-volume model (read a case with mrfg_case_read_foam)");
+        B.setField(sys, region, name, values, (size_t)n);
+    });
+}
+
+int mrfg_assemble(mrfg_ctx *ctx)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::Batch &B = committedBatch(ctx);
+        bool any = false;
+        for (int q = 0; q < B.nSystems(); ++q) any = any || B.hasModel(q);
+        if (!any) throw ArgError("no committed system has a finite-volume model (read a case with mrfg_case_read_foam)");
+        B.assemble();
+    });
+}
+
+int mrfg_get_region_matrix(mrfg_ctx *ctx, int sys, int region, double *diag, double *lower, double *upper,
+                           double *source)
+{
+    if (!ctx || !diag || !lower || !upper || !source) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        const mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (region < 0 || region >= (int)S.regions.size()) throw ArgError("region index out of range");
+        mrfg::Batch &B = committedBatch(ctx);
+        if (!B.hasModel(sys)) throw ArgError("system has no finite-volume model");
+        B.getRegionMatrix(sys, region, diag, lower, upper, source);
+    });
+}
+
+int mrfg_get_interface_coeff(mrfg_ctx *ctx, int sys, int iface, int side, double *coeff)
+{
+    if (!ctx || !coeff || side < 0 || side > 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        const mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (iface < 0 || iface >= (int)S.interfaces.size()) throw ArgError("interface index out of range");
+        mrfg::Batch &B = committedBatch(ctx);
+        if (!B.hasModel(sys)) throw ArgError("system has no finite-volume model");
+        B.getInterfaceCoeff(sys, iface, side, coeff);
+    });
+}
+
 int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w)
 {
     if (!ctx || !r || !w) return MRFG_ERR_ARG;

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -44,9 +45,48 @@ struct CoupledInterface {
     InterfaceSide a, b;
 };
 
+// The finite-volume model a system read from disk was assembled from: mesh
+// geometry, boundary conditions and the fields k, U, T, so the system can be
+// re-assembled on the device when fields change (transportTemperature.C:
+// 129-150 setCoupledEqns, monolithicTemperature's calcThermalDiffusivity).
+enum FvTBc : int8_t { kBcFixedValue = 0, kBcZeroGradient = 1, kBcEmpty = 2, kBcCoupled = 3 };
+enum FvUBc : int8_t { kUFixedValue = 0, kUZeroGradient = 1, kUSlip = 2, kUZero = 3 };
+
+struct FvRegionModel {
+    bool transport = false;             // rho*cp*(ddt + div(phi,T)) - laplacian(k,T), else ddt - laplacian
+    double rhoCp = 0.0, rDeltaT = 0.0;  // rDeltaT = 0: steady
+    // internal faces, in the region's face order
+    std::vector<double> w, deltaCoeffs, magSf, Sf;   // Sf: 3 per face
+    std::vector<double> V;                            // cell volumes
+    // boundary faces, patch by patch (the reader's loop order)
+    std::vector<int32_t> bOwner;
+    std::vector<int8_t> bT, bU, bKFixed;              // FvTBc, FvUBc, k given (else the owner cell's)
+    std::vector<double> bNDist, bMagSf, bSf;          // bSf: 3 per face
+    std::vector<double> bTb, bUb, bKb;                // fixed T, fixed U (3 per face), fixed k
+    std::vector<int32_t> bSide;                       // coupled faces: the interface-side face, else -1
+    // fields
+    std::vector<double> k, U, T;                      // U: 3 per cell (transport regions)
+};
+
+// Per face of one interface side (same order as InterfaceSide::fc): the
+// face's normal distance and boundary-face index; per GGI pair: the partner
+// face's normal distance and owner cell (in the other region).
+struct FvSideModel {
+    std::vector<double> nDist;
+    std::vector<int32_t> bFace;
+    std::vector<double> pairNDist;
+    std::vector<int32_t> pairCell;
+};
+
+struct FvModel {
+    std::vector<FvRegionModel> regions;
+    std::vector<FvSideModel> sides;     // interface i: side a = 2i, side b = 2i + 1
+};
+
 struct CoupledSystem {
     std::vector<LduRegion> regions;
     std::vector<CoupledInterface> interfaces;
+    std::shared_ptr<const FvModel> fv;   // cases read from disk; null otherwise
     long long nCells() const
     {
         long long n = 0;

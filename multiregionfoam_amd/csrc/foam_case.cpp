@@ -452,6 +452,9 @@ struct FoamRegion {
     std::vector<std::vector<double>> Tb, kb;   // per patch face values (fixedValue T; k)
     std::vector<double> phi;            // internal faces
     std::vector<std::vector<double>> phiB;     // per patch
+    std::vector<double> U;              // 3 per cell (transport regions)
+    std::vector<int8_t> bcU, kFixed;    // per patch: FvUBc, k given on the patch
+    std::vector<std::vector<double>> Ub;       // per patch: fixed U, 3 per face
     // coupled patches: face conductivity and delta coefficients, and the
     // boundary coefficients handed to the interface
     std::map<int, std::vector<double>> coupledK, coupledDelta, coupledCoeff;
@@ -821,12 +824,14 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
             throw FoamError(fdir + "T: patch '" + P.name + "' is empty in only one of mesh and field");
     }
     // conductivity: 0/<region>/k, else transportProperties k (uniform)
+    R.kFixed.assign(nP, 0);
     if (fileExists(fdir + "k")) {
         FieldFile K = readField(fdir + "k", m, 1);
         R.k = K.internal;
         for (size_t p = 0; p < nP; ++p) {
             const Patch &P = m.patches[p];
             const PatchBC &bc = K.patches.at(P.name);
+            R.kFixed[p] = bc.type == "fixedValue";
             R.kb[p].resize(P.size);
             for (int i = 0; i < P.size; ++i)
                 R.kb[p][i] = bc.type == "fixedValue" ? bc.value.at(i) : R.k[m.owner[P.start + i]];
@@ -844,9 +849,12 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
 
     // flux: phi = linearInterpolate(U) & Sf, boundary U_b & Sf
     R.phiB.resize(nP);
+    R.bcU.assign(nP, kUZero);
+    R.Ub.assign(nP, {});
     if (R.transport) {
         std::string up = dir + "/0/" + name + "/U";
         FieldFile U = readField(up, m, 3);
+        R.U = U.internal;
         auto Uc = [&](int c) { return Vec{U.internal[3 * c], U.internal[3 * c + 1], U.internal[3 * c + 2]}; };
         R.phi.resize(m.nInternal);
         for (int f = 0; f < m.nInternal; ++f) {
@@ -857,11 +865,16 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
             const Patch &P = m.patches[p];
             const PatchBC &bc = U.patches.at(P.name);
             R.phiB[p].assign(P.size, 0.0);
+            R.bcU[p] = bc.type == "fixedValue" ? kUFixedValue : bc.type == "zeroGradient" ? kUZeroGradient
+                     : bc.type == "slip" ? kUSlip : kUZero;
+            R.Ub[p].assign(3 * (size_t)P.size, 0.0);
             for (int i = 0; i < P.size; ++i) {
                 const int f = P.start + i;
                 Vec ub{0, 0, 0};
-                if (bc.type == "fixedValue") ub = Vec{bc.value.at(3 * i), bc.value.at(3 * i + 1), bc.value.at(3 * i + 2)};
-                else if (bc.type == "zeroGradient") ub = Uc(m.owner[f]);
+                if (bc.type == "fixedValue") {
+                    ub = Vec{bc.value.at(3 * i), bc.value.at(3 * i + 1), bc.value.at(3 * i + 2)};
+                    for (int a = 0; a < 3; ++a) R.Ub[p][3 * i + a] = ub[a];
+                } else if (bc.type == "zeroGradient") ub = Uc(m.owner[f]);
                 else if (bc.type == "slip") {
                     Vec n = (1.0 / m.magSf[f]) * m.Sf[f], u = Uc(m.owner[f]);
                     ub = u - dot(u, n) * n;
@@ -913,6 +926,77 @@ FvMatrixG setCoupledEqns(const FoamRegion &R)
             }
     }
     return E;
+}
+
+// The model the device re-assembles from (geometry, boundary conditions,
+// fields), in the loop orders of laplacian / divUpwind / setCoupledEqns and
+// calcThermalDiffusivity above.
+std::shared_ptr<const FvModel> buildFvModel(const std::vector<FoamRegion> &R, const std::vector<double> &rDeltaT,
+                                            const std::vector<std::array<int, 4>> &sides, const CoupledSystem &sys)
+{
+    auto M = std::make_shared<FvModel>();
+    M->regions.resize(R.size());
+    std::vector<std::vector<int>> patchBase(R.size());
+    for (size_t r = 0; r < R.size(); ++r) {
+        const FoamRegion &rc = R[r];
+        const PolyMesh &m = rc.mesh;
+        FvRegionModel &F = M->regions[r];
+        F.transport = rc.transport;
+        F.rhoCp = rc.rhoCp;
+        F.rDeltaT = rDeltaT[r];
+        F.w = m.w;
+        F.deltaCoeffs = m.deltaCoeffs;
+        F.magSf.assign(m.magSf.begin(), m.magSf.begin() + m.nInternal);
+        F.Sf.resize(3 * (size_t)m.nInternal);
+        for (int f = 0; f < m.nInternal; ++f)
+            for (int a = 0; a < 3; ++a) F.Sf[3 * (size_t)f + a] = m.Sf[f][a];
+        F.V = m.V;
+        for (size_t p = 0; p < m.patches.size(); ++p) {
+            const Patch &P = m.patches[p];
+            patchBase[r].push_back((int)F.bOwner.size());
+            for (int i = 0; i < P.size; ++i) {
+                const int f = P.start + i;
+                F.bOwner.push_back(m.owner[f]);
+                F.bT.push_back((int8_t)rc.bcT[p]);
+                F.bU.push_back(rc.transport ? rc.bcU[p] : (int8_t)kUZero);
+                F.bKFixed.push_back(rc.kFixed[p]);
+                F.bNDist.push_back(m.nDist(f));
+                F.bMagSf.push_back(m.magSf[f]);
+                for (int a = 0; a < 3; ++a) F.bSf.push_back(m.Sf[f][a]);
+                F.bTb.push_back(rc.bcT[p] == BcT::FixedValue ? rc.Tb[p][i] : 0.0);
+                for (int a = 0; a < 3; ++a)
+                    F.bUb.push_back(rc.transport && rc.bcU[p] == kUFixedValue ? rc.Ub[p][3 * i + a] : 0.0);
+                F.bKb.push_back(rc.kb[p][i]);
+                F.bSide.push_back(-1);
+            }
+        }
+        F.k = rc.k;
+        F.U = rc.U;
+        F.T = rc.T;
+    }
+    M->sides.resize(2 * sides.size());
+    int sideFace = 0;
+    for (size_t i = 0; i < sides.size(); ++i)
+        for (int q = 0; q < 2; ++q) {
+            const int rO = sides[i][2 * q], pO = sides[i][2 * q + 1];
+            const int rN = sides[i][2 * (1 - q)], pN = sides[i][2 * (1 - q) + 1];
+            const PolyMesh &mo = R[rO].mesh, &mn = R[rN].mesh;
+            const InterfaceSide &S = q ? sys.interfaces[i].b : sys.interfaces[i].a;
+            FvSideModel &D = M->sides[2 * i + q];
+            const Patch &Po = mo.patches[pO], &Pn = mn.patches[pN];
+            for (int j = 0; j < Po.size; ++j) {
+                D.nDist.push_back(mo.nDist(Po.start + j));
+                D.bFace.push_back(patchBase[rO][pO] + j);
+                M->regions[rO].bSide[patchBase[rO][pO] + j] = sideFace + j;
+            }
+            for (size_t p = 0; p < S.idx.size(); ++p) {
+                const int g = Pn.start + S.idx[p];
+                D.pairNDist.push_back(mn.nDist(g));
+                D.pairCell.push_back(mn.owner[g]);
+            }
+            sideFace += Po.size;
+        }
+    return M;
 }
 
 // "regions ( (fluid (icoFluid transportTemperature)) (solid (conductTemperature)) );"
@@ -1075,6 +1159,7 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
             }
         }
     }
+    sys.fv = buildFvModel(R, rDeltaT, sides, sys);
     if (info) {
         info->volumes.clear();
         info->maxNonOrthDeg.clear();

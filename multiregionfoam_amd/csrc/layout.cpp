@@ -403,12 +403,22 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
     }
     L.ellCol.assign((size_t)L.ellOff[L.nSlices] * kSlice, 0);
     L.ellVal.assign((size_t)L.ellOff[L.nSlices] * kSlice, 0.0);
+    L.ellSrc.assign((size_t)L.ellOff[L.nSlices] * kSlice, -1);
+    L.faceOffset.assign(nR, 0);
+    for (int r = 1; r < nR; ++r) L.faceOffset[r] = L.faceOffset[r - 1] + sys.regions[r - 1].nFaces;
+    std::vector<int32_t> sideOffset(2 * sys.interfaces.size() + 1, 0);
+    for (size_t i = 0; i < sys.interfaces.size(); ++i) {
+        sideOffset[2 * i + 1] = sideOffset[2 * i] + sys.interfaces[i].a.nFaces();
+        sideOffset[2 * i + 2] = sideOffset[2 * i + 1] + sys.interfaces[i].b.nFaces();
+    }
     const size_t nDep = (size_t)L.nSlices * K * kSlice;
     L.fCode.assign(nDep, kNoDep);
     L.bCode.assign(nDep, kNoDep);
     L.fVal.assign(nDep, 0.0);
     L.fProd.assign(nDep, 0.0);
     L.bVal.assign(nDep, 0.0);
+    L.fSrc.assign(nDep, -1);
+    L.bSrc.assign(nDep, -1);
     L.fHalo.assign(L.nPos, -1);
     L.bHalo.assign(L.nPos, -1);
 
@@ -431,16 +441,20 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
             int r = regionOf(c);
             const LduRegion &R = sys.regions[r];
             const std::vector<double> &lo = R.symmetric ? R.upper : R.lower;
+            auto loSrc = [&](int f) { return 2 * (L.faceOffset[r] + f) + (R.symmetric ? 1 : 0); };
+            auto upSrc = [&](int f) { return 2 * (L.faceOffset[r] + f) + 1; };
             int k = 0;
             for (int q = D.lo_off[c]; q < D.lo_off[c + 1]; ++q, ++k) {
                 size_t e = (size_t)(L.ellOff[s] + k) * kSlice + l;
                 L.ellCol[e] = L.posOfCell[D.lo_nb[q]];
                 L.ellVal[e] = lo[D.lo_face[q]];
+                L.ellSrc[e] = loSrc(D.lo_face[q]);
             }
             for (int q = D.up_off[c]; q < D.up_off[c + 1]; ++q, ++k) {
                 size_t e = (size_t)(L.ellOff[s] + k) * kSlice + l;
                 L.ellCol[e] = L.posOfCell[D.up_nb[q]];
                 L.ellVal[e] = R.upper[D.up_face[q]];
+                L.ellSrc[e] = upSrc(D.up_face[q]);
             }
 
             int w, t, lane;
@@ -460,11 +474,13 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
                     size_t e = ((size_t)s * K + sl) * kSlice + l;
                     L.fVal[e] = lo[f];
                     L.fProd[e] = R.upper[f] * lo[f];
+                    L.fSrc[e] = loSrc(f);
                 }
                 for (int q = D.up_off[c]; q < D.up_off[c + 1]; ++q) {
                     int sl = slotOf(D.up_nb[q] - c);
                     if (sl < 0) throw std::runtime_error("box region with an unexpected upper neighbour");
                     L.bVal[((size_t)s * K + sl) * kSlice + l] = R.upper[D.up_face[q]];
+                    L.bSrc[((size_t)s * K + sl) * kSlice + l] = upSrc(D.up_face[q]);
                 }
             } else {
                 // forward sweep: lower neighbours, ascending face order
@@ -477,6 +493,7 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
                     size_t e = ((size_t)s * K + k) * kSlice + l;
                     L.fVal[e] = lo[f];
                     L.fProd[e] = R.upper[f] * lo[f];
+                    L.fSrc[e] = loSrc(f);
                     if (wd == w && t - td >= 1 && t - td <= kRing - 1) {
                         L.fCode[e] = -((td % kRing) * kSlice + ld + 1);
                     } else {
@@ -495,6 +512,7 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
                     waveStep(pd, wd, td, ld);
                     size_t e = ((size_t)s * K + k) * kSlice + l;
                     L.bVal[e] = R.upper[D.up_face[q]];
+                    L.bSrc[e] = upSrc(D.up_face[q]);
                     if (wd == w && td - t >= 1 && td - t <= kRing - 1) {
                         int id = T - 1 - td;
                         L.bCode[e] = -((id % kRing) * kSlice + ld + 1);
@@ -516,6 +534,7 @@ SystemLayout buildLayout(const CoupledSystem &sys, bool preferLevels)
                     int oN = L.regionOffset[O.region];
                     int f = pf.second;
                     L.ifCoeff.push_back(S.coeff[f]);
+                    L.ifSrc.push_back(sideOffset[pf.first] + f);
                     for (int q = S.off[f]; q < S.off[f + 1]; ++q) {
                         L.ifPairCol.push_back(L.posOfCell[oN + O.fc[S.idx[q]]]);
                         L.ifPairW.push_back(S.w[q]);

@@ -105,6 +105,15 @@ struct SystemLayout {
     // dependencies follow from the tile geometry in boxMeta
     std::vector<uint8_t> waveBox;            // per wave: 1 if a box tile
     std::vector<int32_t> boxMeta;            // kBoxMeta per wave
+    // where each value came from, so re-assembled coefficients can be put in
+    // place on the device: face values are numbered 2*(faceOffset[r] + f)
+    // (+0 the lower coefficient, +1 the upper; a symmetric region's lower
+    // entries name its upper), interface-side faces in interface order, side
+    // a then side b
+    std::vector<int32_t> faceOffset;         // per region
+    std::vector<int32_t> ellSrc;             // per ELL entry, -1 for padding
+    std::vector<int32_t> fSrc, bSrc;         // per DILU slot (as fVal / bVal), -1 for none
+    std::vector<int32_t> ifSrc;              // per interface-row face (as ifCoeff)
 };
 
 // Build the layout.  `preferLevels` forces the level schedule for every

@@ -114,15 +114,26 @@ def load_fixture():
     return meta, meshes
 
 
-def write_fixture_case(dest, params=None):
+def write_fixture_case(dest, params=None, ddt=None):
     """params: the test suite's parameter map (genCases.py update_params),
     dict(kf=..., cp=...): the fluid's conductivity (transportProperties and
-    0/fluid/k) and specific heat."""
+    0/fluid/k) and specific heat; optionally U (the fluid's uniform internal
+    velocity) and T (dict region -> uniform internal temperature).  ddt
+    replaces every region's ddt scheme."""
     meta, meshes = load_fixture()
-    if params:
+    params = params or {}
+    if "kf" in params:
         meta["transportProperties"]["fluid"]["k"] = float(params["kf"])
-        meta["transportProperties"]["fluid"]["cp"] = float(params["cp"])
         meta["fields"]["fluid"]["k"]["internal"] = float(params["kf"])
+    if "cp" in params:
+        meta["transportProperties"]["fluid"]["cp"] = float(params["cp"])
+    if "U" in params:
+        meta["fields"]["fluid"]["U"]["internal"] = [float(x) for x in params["U"]]
+    for region, value in params.get("T", {}).items():
+        meta["fields"][region]["T"]["internal"] = float(value)
+    if ddt is not None:
+        for region in meta["ddt"]:
+            meta["ddt"][region] = ddt
     regions_txt = " ".join("(%s (%s))" % (r, " ".join(t)) for r, t in meta["regions"])
     write_dict(os.path.join(dest, "constant", "multiRegionProperties"), "regions (%s);\n" % regions_txt)
     pairs = " ".join("%s%s { coupledPatchPair ((%s %s) (%s %s)); }" % (a[0], b[0], a[0], a[1], b[0], b[1])

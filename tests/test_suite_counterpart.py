@@ -209,3 +209,31 @@ def test_suite_batch_gpu_matches_oracle(gpu, suite):
         assert perfs[s].final_residual == perf["finalResidual"], c
         assert np.array_equal(np.concatenate(A.psi(s)), np.concatenate(psi)), c
     A.close()
+
+
+@pytest.mark.gpu
+def test_suite_assembled_on_device_matches_oracle(gpu, suite, tmp_path_factory):
+    """The nine cases as nine copies of the tutorial's mesh whose fluid k
+    and rho*c_p are set as fields and assembled on the device (no matrix
+    upload): per system, the same iterations, residual and field as the
+    oracle's solve of the case read from disk."""
+    d = tmp_path_factory.mktemp("base")
+    ff.write_fixture_case(str(d))
+    base = gpu.Case.from_foam(str(d), ddt="steadyState")
+    n = base.regions()[0]["nCells"]
+    A = gpu.CoupledFvMatrix(0)
+    for _ in CASES:
+        A.add_case(base)
+    A.commit()
+    for s, c in enumerate(CASES):
+        p = suite_params(*c)
+        A.set_field(s, 0, "k", np.full(n, p["kf"]))
+        A.set_field(s, 0, "rhoCp", p["cp"])
+    A.assemble()
+    perfs = A.solve(fv_solution=FV_SOLUTION, entry="Tcoupled")
+    for s, c in enumerate(CASES):
+        _, psi, perf = suite[c]
+        assert perfs[s].n_iterations == perf["nIterations"], c
+        assert perfs[s].final_residual == perf["finalResidual"], c
+        assert np.array_equal(np.concatenate(A.psi(s)), np.concatenate(psi)), c
+    A.close()
