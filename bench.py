@@ -187,6 +187,121 @@ def run_tutorial(m, args):
     print(json.dumps(out))
 
 
+def run_suite(m, args):
+    """The reference's flowOverHeatedPlate test suite (genCases.py): nine
+    parameter cases (Pr, Re, k) on the tutorial's mesh, one batch of nine
+    systems read once from disk.  One step: every system's fluid k and
+    rho*cp are set as fields, the batch is assembled on the device
+    (mrfg_assemble: no matrix crosses the bus) and solved to steady state
+    (DILU-BiCGStab, tolerance 1e-12), each system from its initial field."""
+    import shutil
+    import tempfile
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import foam_fixture
+    suite = [(0.01, 500, 1), (0.01, 500, 5), (0.01, 500, 20), (0.01, 10000, 1), (0.01, 10000, 5),
+             (0.01, 10000, 20), (100, 500, 1), (100, 500, 5), (100, 500, 20)]
+
+    def params(Pr, Re, k):   # genCases.py update_params
+        kf = 100.0 / k
+        return dict(kf=kf, cp=kf * Pr / (1.0 / int(Re)))
+
+    fvs = ("solvers { Tcoupled { solver BiCGStab; preconditioner DILU; tolerance 1e-12; relTol 0; "
+           "minIter 0; maxIter 1000; } }")
+    ctrl = m.parse_controls(fvs, "Tcoupled")
+    d = tempfile.mkdtemp(prefix="testSuite_")
+    try:
+        foam_fixture.write_fixture_case(d)
+        base = m.Case.from_foam(d, ddt="steadyState")
+        # the host path this replaces: read and assemble each case, then
+        # build the batch layout and upload the matrices
+        host_cases, t_read = [], 0.0
+        for c in suite:
+            shutil.rmtree(d, ignore_errors=True)
+            foam_fixture.write_fixture_case(d, params=params(*c))   # writing the files is not timed
+            t1 = time.perf_counter()
+            host_cases.append(m.Case.from_foam(d, ddt="steadyState"))
+            t_read += time.perf_counter() - t1
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    B = m.CoupledFvMatrix(0)
+    for c in host_cases:
+        B.add_case(c)
+    t1 = time.perf_counter()
+    B.commit()
+    t_commit = time.perf_counter() - t1
+    B.close()
+
+    A = m.CoupledFvMatrix(0)
+    for _ in suite:
+        A.add_case(base)
+    A.commit()
+    n_fluid = base.regions()[0]["nCells"]
+    fields = [(np.full(n_fluid, params(*c)["kf"]), params(*c)["cp"]) for c in suite]
+
+    def step(tim):
+        t1 = time.perf_counter()
+        for s, (kf, cp) in enumerate(fields):
+            A.set_field(s, 0, "k", kf)
+            A.set_field(s, 0, "rhoCp", cp)
+        t2 = time.perf_counter()
+        A.reset_psi()
+        A.assemble()
+        t3 = time.perf_counter()
+        perf = A.solve(ctrl)
+        t4 = time.perf_counter()
+        tim.append((t2 - t1, t3 - t2, t4 - t3))
+        return perf
+
+    for _ in range(args.warmup):
+        step([])
+    tim = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        perf = step(tim)
+    total = time.perf_counter() - t0
+    info = A.layout_info(0)
+    reps = max(5, args.kernel_reps)
+    ms7, ms8 = A.time_kernel(7, reps), A.time_kernel(8, reps)
+    nb = A.assembly_bytes()
+    passes = {"assemble": dict(ms=ms7, bytes=nb["assemble"], GBs=nb["assemble"] / (ms7 * 1e-3) / 1e9),
+              "place": dict(ms=ms8, bytes=nb["place"], GBs=nb["place"] / (ms8 * 1e-3) / 1e9)}
+    dom = max(passes, key=lambda k: passes[k]["ms"])
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle_bind
+        t1 = time.perf_counter()
+        its = []
+        for c in host_cases:
+            O = oracle_bind.OracleSystem.from_arrays(c.regions(), c.interfaces())
+            _, pref = O.solve(tol=1e-12, relTol=0.0, minIter=0, maxIter=1000)
+            its.append(pref["nIterations"])
+        tc = time.perf_counter() - t1
+        cpu = dict(value=len(suite) / tc, unit="cases/s", cores=1, kind="port",
+                   sample="oracle solves of the nine host-assembled cases on one host core (%d-%d iterations), "
+                          "%.2f s" % (min(its), max(its), tc))
+    mean = lambda i: 1e3 * sum(t[i] for t in tim) / len(tim)
+    out = {
+        "metric": "test-suite cases/s (fields set, assembled on the device, solved to steady state)",
+        "value": len(suite) * args.steps / total, "unit": "cases/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "reference tutorial mesh (fixture) with the test suite's nine parameter sets",
+        "config": {"workload": "flowOverHeatedPlate_testSuite: 9 cases (Pr, Re, k) of genCases.py as one batch, "
+                               "BiCGStab/DILU tol 1e-12", "systems": len(suite),
+                   "cells_per_system": info["cells"], "internal_faces_per_system": info["faces"]},
+        "iterations": [p.n_iterations for p in perf], "all_converged": all(p.converged for p in perf),
+        "set_fields_ms": mean(0), "assemble_ms": mean(1), "solve_ms": mean(2),
+        "host_path": {"read_and_assemble_9_cases_s": t_read, "commit_layout_and_upload_s": t_commit},
+        "assembly_passes": passes,
+        "roofline": {"kernel": "fv_" + dom, "bound": "hbm", "achieved": passes[dom]["GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": passes[dom]["GBs"] / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def run_batch(m, cases, args):
     """64 independent two-region systems of seeded uneven size (cells per
     region log-uniform in [10k, 500k], conductivity ratio log-uniform in
@@ -281,13 +396,15 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--workload", default="duct3d",
-                    choices=["duct3d", "exchanger3", "batch64", "serving", "tutorial"],
+                    choices=["duct3d", "exchanger3", "batch64", "serving", "tutorial", "suite"],
                     help="duct3d (default, the headline line): the 1.404M-cell two-region GGI solve; "
                          "exchanger3: three regions of 2.4M cells, two GGI interfaces, mixed precision "
                          "with the FP64 interface-flux check; batch64: 64 independent uneven systems "
                          "solved together; serving: Poisson arrivals at 50 req/s of 20k-2M-cell solves "
                          "through the continuous-batching server; tutorial: the reference's "
-                         "flowOverHeatedPlate case read from disk, with its own Tcoupled solver entry")
+                         "flowOverHeatedPlate case read from disk, with its own Tcoupled solver entry; "
+                         "suite: the reference test suite's nine parameter cases as one batch, fields "
+                         "set and assembled on the device, then solved")
     ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
@@ -311,6 +428,8 @@ def main():
         return run_serving(m, cases, args)
     if args.workload == "tutorial":
         return run_tutorial(m, args)
+    if args.workload == "suite":
+        return run_suite(m, args)
 
     case = cases.duct3d()
     A = m.CoupledFvMatrix(0)

@@ -208,6 +208,10 @@ int mrfg_get_region_matrix(mrfg_ctx *ctx, int sys, int region, double *diag, dou
                            double *upper, double *source);
 int mrfg_get_interface_coeff(mrfg_ctx *ctx, int sys, int iface, int side, double *coeff);
 
+/* Bytes the two passes of mrfg_assemble read and write, each array counted
+   once (the algorithmic traffic of mrfg_time_kernel 7 and 8). */
+int mrfg_assembly_bytes(mrfg_ctx *ctx, double *assemble_bytes, double *place_bytes);
+
 /* ---- case builder ------------------------------------------------------ */
 
 /* Assemble a case dictionary (regions of transportTemperature /
@@ -272,7 +276,10 @@ int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w);
    2 = DILU backward sweep, 3 = DILU factorisation, each launched back to
    back; 4 = the first forward sweep, 5 = the first backward sweep and 6 =
    the SpMV with the alpha reduction of each of `reps` BiCGStab iterations
-   (run from the initial guess with tolerance 0; psi is restored after). */
+   (run from the initial guess with tolerance 0; psi is restored after);
+   7 = the finite-volume assembly passes (faces, interface sides, cells) and
+   8 = the placement pass of mrfg_assemble, back to back (systems read from
+   disk; the matrices are left as one mrfg_assemble leaves them). */
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms);
 /* Trace one DILU forward/backward pair (tracing facility; no counterpart
    in the reference): out receives, per wave in dispatch order, [start ns,

@@ -111,6 +111,7 @@ def lib():
         "mrfg_interface_flux": ([vp, C.c_int, C.c_int, F64P, F64P], C.c_int),
         "mrfg_set_field": ([vp, C.c_int, C.c_int, C.c_char_p, F64P, C.c_longlong], C.c_int),
         "mrfg_assemble": ([vp], C.c_int),
+        "mrfg_assembly_bytes": ([vp, F64P, F64P], C.c_int),
         "mrfg_get_region_matrix": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, F64P], C.c_int),
         "mrfg_get_interface_coeff": ([vp, C.c_int, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_server_create": ([C.POINTER(vp), C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
@@ -138,7 +139,7 @@ EXPORTED = (
     "mrfg_case_region mrfg_case_region_ddt "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
-    "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff "
+    "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
     "mrfg_server_get_psi mrfg_server_last_error").split()
 
@@ -436,6 +437,13 @@ class CoupledFvMatrix:
         the device (setCoupledEqns + interface coefficients + ddt), in place
         of the committed matrices."""
         self._check(lib().mrfg_assemble(self._h))
+
+    def assembly_bytes(self):
+        """Bytes read and written by the two passes of :meth:`assemble`
+        (assembly, placement), every array once."""
+        a, b = C.c_double(), C.c_double()
+        self._check(lib().mrfg_assembly_bytes(self._h, C.byref(a), C.byref(b)))
+        return dict(assemble=a.value, place=b.value)
 
     def region_matrix(self, sys, region):
         """The device-assembled LDU arrays of one region (after assemble)."""

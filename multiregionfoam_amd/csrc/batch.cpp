@@ -987,6 +987,7 @@ void Batch::precondition32(const float *rHost, float *wHost, int sys)
 double Batch::timeKernel(int which, int reps)
 {
     HIP_OK(hipSetDevice(device_));
+    if (which == 7 || which == 8) return timeAssembly(which, reps);
     factorise();
     markAllRunning();
     Krylov<double> K;

@@ -83,6 +83,7 @@ struct FvBatch {
     std::vector<int> cellBase, faceBase, sideBase, regionBase;   // per system, -1 without a model
     std::vector<FvRegionParams> hRegion;
     int nCells = 0, nFaces = 0, nSideFaces = 0;
+    long long nBFaces = 0, nPairs = 0, nPosModel = 0, nEll = 0, nSlots = 0, nIfEntries = 0;
     DevBuf<FvRegionParams> region;
     DevBuf<int32_t> cellRegion, faceRegion, own, nei, cfOff, cf, cbOff, bSide, sCell, sOff, sPairCell;
     DevBuf<int8_t> bT, bU, bKFixed;
@@ -167,6 +168,11 @@ public:
     void assemble();
     void getRegionMatrix(int sys, int region, double *diag, double *lower, double *upper, double *source);
     void getInterfaceCoeff(int sys, int iface, int side, double *coeff);
+    // Average device time (ms) of one assembly pass over the batch (which 7:
+    // faces, interface sides and cells; 8: placement into the solver's
+    // arrays) and the bytes each pass reads and writes, every array once.
+    double timeAssembly(int which, int reps);
+    void assemblyBytes(double &assemble, double &place) const;
 
 private:
     void ensureFloat();
@@ -189,6 +195,8 @@ private:
     int posBegin(int sys) const { return sliceBegin_[sys] * kSlice; }
     void setDdtDiag(double coefft);
     void buildFv();
+    FvDev fvDevView() const;
+    FvPlace fvPlaceView();
     // the host copies of the FP64 values after the device changed them
     void refreshHostCopies();
 

@@ -177,6 +177,16 @@ void Batch::buildFv()
     F->nCells = cellBase;
     F->nFaces = faceBase;
     F->nSideFaces = sideBase;
+    F->nBFaces = (long long)bT.size();
+    F->nPairs = pairBase;
+    for (int q = 0; q < nS; ++q) {
+        if (F->cellBase[q] < 0) continue;
+        const SystemLayout &L = *lay_[q];
+        F->nPosModel += L.nPos;
+        F->nEll += (long long)L.ellVal.size();
+        F->nSlots += (long long)L.nSlices * K_ * kSlice;
+        F->nIfEntries += (long long)L.ifCoeff.size();
+    }
     // every index the assembly kernels follow stays inside its array
     {
         auto fail = [](const char *what) {
@@ -305,11 +315,9 @@ void Batch::setField(int sys, int region, const std::string &name, const double 
     }
 }
 
-void Batch::assemble()
+FvDev Batch::fvDevView() const
 {
-    HIP_OK(hipSetDevice(device_));
-    if (!fv_) throw std::runtime_error("no system of the batch has a finite-volume model");
-    FvBatch &F = *fv_;
+    const FvBatch &F = *fv_;
     FvDev m;
     m.nCells = F.nCells;
     m.nFaces = F.nFaces;
@@ -358,8 +366,12 @@ void Batch::assemble()
     m.ddtA = F.ddtA.p;
     m.diagBase = F.diagBase.p;
     m.srcBase = F.srcBase.p;
-    HIP_OK(launchFvAssemble(m, stream_));
+    return m;
+}
 
+FvPlace Batch::fvPlaceView()
+{
+    const FvBatch &F = *fv_;
     FvPlace q;
     q.nPos = nPos_;
     q.K = K_;
@@ -391,13 +403,76 @@ void Batch::assemble()
     q.ddtAOut = transient_ ? ddtA_.p : nullptr;
     q.diagBaseOut = transient_ ? diagBase_.p : nullptr;
     q.srcBaseOut = transient_ ? srcBase_.p : nullptr;
-    HIP_OK(launchFvPlace(q, stream_));
+    return q;
+}
+
+void Batch::assemble()
+{
+    HIP_OK(hipSetDevice(device_));
+    if (!fv_) throw std::runtime_error("no system of the batch has a finite-volume model");
+    HIP_OK(launchFvAssemble(fvDevView(), stream_));
+    HIP_OK(launchFvPlace(fvPlaceView(), stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    F.assembled = true;
+    fv_->assembled = true;
     factorised_ = false;
     haveFloat_ = false;
     hostStale_ = true;
     diagCoefft_ = 1.0;   // diag = diagBase + a, as assembled
+}
+
+double Batch::timeAssembly(int which, int reps)
+{
+    HIP_OK(hipSetDevice(device_));
+    if (!fv_) throw std::runtime_error("no system of the batch has a finite-volume model");
+    const FvDev m = fvDevView();
+    const FvPlace q = fvPlaceView();
+    // both passes once, so the timed one reads what a real assembly reads
+    HIP_OK(launchFvAssemble(m, stream_));
+    HIP_OK(launchFvPlace(q, stream_));
+    std::vector<hipEvent_t> ev(2 * reps);
+    for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+    for (int i = 0; i < reps; ++i) {
+        HIP_OK(hipEventRecord(ev[2 * i], stream_));
+        if (which == 7) HIP_OK(launchFvAssemble(m, stream_));
+        else HIP_OK(launchFvPlace(q, stream_));
+        HIP_OK(hipEventRecord(ev[2 * i + 1], stream_));
+    }
+    HIP_OK(hipStreamSynchronize(stream_));
+    double tot = 0;
+    for (int i = 0; i < reps; ++i) {
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        tot += ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    fv_->assembled = true;
+    factorised_ = false;
+    haveFloat_ = false;
+    hostStale_ = true;
+    diagCoefft_ = 1.0;
+    return tot / reps;
+}
+
+void Batch::assemblyBytes(double &assemble, double &place) const
+{
+    if (!fv_) throw std::runtime_error("no system of the batch has a finite-volume model");
+    const FvBatch &F = *fv_;
+    const double nF = F.nFaces, nC = F.nCells, nB = (double)F.nBFaces, nSd = F.nSideFaces, nP = F.nPairs;
+    // each array the passes read or write, once: per internal face the
+    // addressing, region, w, deltaCoeff, |S|, S (60), the cell-face lists
+    // (8), the two coefficients (16) and the Laplacian / convection parts
+    // written by the face pass and read by the cell pass (48); per cell the
+    // region, V, list offsets (20), the fields k, U, T (40) and five results
+    // (40); per boundary face 87; per interface-side face 40, per GGI pair 20
+    assemble = nF * 132.0 + nC * 100.0 + nB * 87.0 + nSd * 40.0 + nP * 20.0;
+    // per position of a model system: its cell (4), diag, source (16) and
+    // diag, b, sumA written (24), ddt terms (48 when transient); per ELL
+    // entry source + value (12); per DILU slot two sources + three values
+    // (32) and the diagonal slot (8 per position); per interface-row face
+    // source, gathered and written coefficient (20); the face values (16 per
+    // face) and side coefficients (8) read once
+    place = (double)F.nPosModel * (4 + 16 + 24 + 8 + (transient_ ? 48 : 0)) + (double)F.nEll * 12.0 +
+            (double)F.nSlots * 32.0 + (double)F.nIfEntries * 20.0 + nF * 16.0 + nSd * 8.0;
 }
 
 void Batch::getRegionMatrix(int sys, int region, double *diag, double *lower, double *upper, double *source)

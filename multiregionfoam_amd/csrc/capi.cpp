@@ -533,6 +533,12 @@ int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *
 
 // ---- finite-volume assembly ---------------------------------------------------
 
+int mrfg_assembly_bytes(mrfg_ctx *ctx, double *assemble_bytes, double *place_bytes)
+{
+    if (!ctx || !assemble_bytes || !place_bytes) return MRFG_ERR_ARG;
+    return guard(ctx, [&] { committedBatch(ctx).assemblyBytes(*assemble_bytes, *place_bytes); });
+}
+
 int mrfg_set_field(mrfg_ctx *ctx, int sys, int region, const char *name, const double *values, long long n)
 {
     if (!ctx || !name || !values || n < 1) return MRFG_ERR_ARG;
@@ -592,7 +598,7 @@ int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w)
 
 int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms)
 {
-    if (!ctx || !ms || reps < 1 || which < 0 || which > 6) return MRFG_ERR_ARG;
+    if (!ctx || !ms || reps < 1 || which < 0 || which > 8) return MRFG_ERR_ARG;
     return guard(ctx, [&] { *ms = committedBatch(ctx).timeKernel(which, reps); });
 }
 
