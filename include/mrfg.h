@@ -148,6 +148,37 @@ int mrfg_solve_dict(mrfg_ctx *ctx, const char *fv_solution, const char *entry, m
 int mrfg_time_steps(mrfg_ctx *ctx, const mrfg_controls *controls, const char *ddt_scheme, int n_steps,
                     mrfg_perf *perf);
 
+/* Result of the partitioned coupling of one case (dnaControl). */
+typedef struct {
+    int iterations;          /* DNA iterations run ("DNA: converged in N iterations") */
+    int converged;           /* residual control met (else maxCoupleIter reached) */
+    int region_solves;       /* region solves run */
+    int solver_iterations;   /* Krylov iterations over all region solves */
+    double jump_residual;    /* last normalised temperature-jump residual (max over pairs) */
+    double flux_residual;    /* last normalised heat-flux residual (max over pairs) */
+    double seconds;          /* wall time of the whole batch's coupling */
+} mrfg_dna_perf;
+
+/* Partitioned (Dirichlet-Neumann) coupling of every committed system, each
+   a case read with regionCoupledTemperatureJump / regionCoupledHeatFlux
+   patches: the loop of multiRegionSystem::solve for partitioned fields
+   (multiRegionSystem.C:571-610) under dnaControl::loop (dnaControl.C:
+   993-1061), steady.  Per DNA iteration every region in order gets its
+   partitioned patches updated on the device -- the Dirichlet side the other
+   side's patch values times K, relaxed (fixed or Aitken); the Neumann side
+   the gradient that matches the other side's heat flux, relaxed -- and the
+   region's systems of all cases still iterating are solved in one batched
+   solve (controls, fp64).  Each case stops when its jump and flux residuals
+   are within maxJumpRes / maxFluxRes (after the first iteration) or after
+   maxCoupleIter iterations, and runs from the committed initial state.  perf
+   receives one entry per system; the coupled fields become the systems'
+   fields (mrfg_get_psi). */
+int mrfg_dna_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_dna_perf *perf);
+/* Per DNA iteration of the last mrfg_dna_solve for system `sys`: the jump
+   and flux residuals and the relaxation factor of its first Dirichlet side
+   (at most max_iter values each; *n receives the number of iterations). */
+int mrfg_dna_history(mrfg_ctx *ctx, int sys, int max_iter, double *jump, double *flux, double *omega, int *n);
+
 /* Parse a solver entry of an fvSolution text (lduSolver::New selection by
    name, keys solver / preconditioner / tolerance / relTol / minIter /
    maxIter).  err receives the message on MRFG_ERR_DICT. */
@@ -233,6 +264,27 @@ int mrfg_case_region(const mrfg_case *c, int region, int32_t *l_addr, int32_t *u
    for a steady region (arrays untouched).  Null arrays are skipped. */
 int mrfg_case_region_ddt(const mrfg_case *c, int region, int *transient, double *ddt_coeff,
                          double *diag_base, double *source_base);
+/* Partitioned coupling of a case read from disk (Dirichlet-Neumann through
+   heatTransferInterface pairs: regionInterfacePropertiesPartitioned,
+   0/<region>/T with regionCoupledTemperatureJump / regionCoupledHeatFlux
+   patches, constant/multiRegionProperties DNA { T { maxCoupleIter;
+   residualControl { maxJumpRes; maxFluxRes; } } }, dnaControl.C:40-200).
+   n_pairs = 0 for a monolithic case; a residual limit < 0 means none.
+   Side 0 is the Dirichlet side, 1 the Neumann side: its face cells, source
+   term per unit boundary value (temperature or gradient; added to
+   source[fc[i]] after every other boundary contribution), face conductivity,
+   patch deltaCoeffs, initial patch values, the GGI gather of the other
+   side's faces, acceleration (0 fixed, 1 aitken) and relaxation factor; K
+   is the pair's temperature jump coefficient.  source_base is a region's
+   source without the partitioned faces. */
+int mrfg_case_dna_controls(const mrfg_case *c, int *n_pairs, int *max_couple_iter,
+                           double *max_jump_res, double *max_flux_res);
+int mrfg_case_dna_side_sizes(const mrfg_case *c, int pair, int side, int *region, int *n_faces,
+                             int *n_weights, int *acc_type, double *relax, double *K);
+int mrfg_case_dna_side(const mrfg_case *c, int pair, int side, int32_t *fc, double *coeff,
+                       double *kb, double *dc, double *value0, int32_t *off, int32_t *idx,
+                       double *w);
+int mrfg_case_region_source_base(const mrfg_case *c, int region, double *source_base);
 int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int *region_b,
                               int *n_faces_a, int *n_faces_b, int *n_pairs_a, int *n_pairs_b);
 int mrfg_case_interface(const mrfg_case *c, int iface,

@@ -42,6 +42,12 @@ class Perf(C.Structure):
                 ("n_outer", C.c_int), ("seconds", C.c_double), ("bytes_moved", C.c_double)]
 
 
+class DnaPerf(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("region_solves", C.c_int),
+                ("solver_iterations", C.c_int), ("jump_residual", C.c_double), ("flux_residual", C.c_double),
+                ("seconds", C.c_double)]
+
+
 class Controls(C.Structure):
     _fields_ = [("solver", C.c_int), ("precision", C.c_int), ("tolerance", C.c_double),
                 ("rel_tol", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
@@ -83,6 +89,8 @@ def lib():
         "mrfg_solve": ([vp, C.POINTER(Controls), C.POINTER(Perf)], C.c_int),
         "mrfg_solve_dict": ([vp, C.c_char_p, C.c_char_p, C.POINTER(Perf)], C.c_int),
         "mrfg_time_steps": ([vp, C.POINTER(Controls), C.c_char_p, C.c_int, C.POINTER(Perf)], C.c_int),
+        "mrfg_dna_solve": ([vp, C.POINTER(Controls), C.POINTER(DnaPerf)], C.c_int),
+        "mrfg_dna_history": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, C.POINTER(C.c_int)], C.c_int),
         "mrfg_parse_controls": ([C.c_char_p, C.c_char_p, C.POINTER(Controls), C.c_char_p, C.c_int], C.c_int),
         "mrfg_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_reset_psi": ([vp], C.c_int),
@@ -95,6 +103,10 @@ def lib():
                                    C.c_int),
         "mrfg_case_region": ([vp, C.c_int, I32P, I32P, F64P, F64P, F64P, F64P, F64P], C.c_int),
         "mrfg_case_region_ddt": ([vp, C.c_int, C.POINTER(C.c_int), F64P, F64P, F64P], C.c_int),
+        "mrfg_case_dna_controls": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int), F64P, F64P], C.c_int),
+        "mrfg_case_dna_side_sizes": ([vp, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 4 + [F64P, F64P], C.c_int),
+        "mrfg_case_dna_side": ([vp, C.c_int, C.c_int, I32P, F64P, F64P, F64P, F64P, I32P, I32P, F64P], C.c_int),
+        "mrfg_case_region_source_base": ([vp, C.c_int, F64P], C.c_int),
         "mrfg_case_interface_sizes": ([vp, C.c_int] + [C.POINTER(C.c_int)] * 6, C.c_int),
         "mrfg_case_interface": ([vp, C.c_int, I32P, F64P, I32P, I32P, F64P, I32P, F64P, I32P, I32P, F64P],
                                 C.c_int),
@@ -134,9 +146,11 @@ def lib():
 EXPORTED = (
     "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region mrfg_set_region_ddt "
     "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_time_steps mrfg_parse_controls "
+    "mrfg_dna_solve mrfg_dna_history "
     "mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_read_foam mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes "
-    "mrfg_case_region mrfg_case_region_ddt "
+    "mrfg_case_region mrfg_case_region_ddt mrfg_case_dna_controls mrfg_case_dna_side_sizes mrfg_case_dna_side "
+    "mrfg_case_region_source_base "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
     "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
@@ -291,6 +305,43 @@ class Case:
                                   _p(d["fcB"], C.c_int32), _p(d["coeffB"], C.c_double),
                                   _p(d["offB"], C.c_int32), _p(d["idxB"], C.c_int32), _p(d["wB"], C.c_double))
         return d
+
+    def partitioned(self):
+        """The partitioned (Dirichlet-Neumann) coupling of a case read from
+        disk, or None for a monolithic case: DNA controls and, per pair, the
+        Dirichlet ("dir") and Neumann ("neu") sides."""
+        n, it = C.c_int(), C.c_int()
+        jr, fr = C.c_double(), C.c_double()
+        lib().mrfg_case_dna_controls(self._h, C.byref(n), C.byref(it), C.byref(jr), C.byref(fr))
+        if n.value == 0:
+            return None
+        pairs = []
+        for q in range(n.value):
+            pair = {}
+            for side, key in ((0, "dir"), (1, "neu")):
+                reg, nf, nw, acc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+                relax, K = C.c_double(), C.c_double()
+                lib().mrfg_case_dna_side_sizes(self._h, q, side, C.byref(reg), C.byref(nf), C.byref(nw),
+                                               C.byref(acc), C.byref(relax), C.byref(K))
+                d = dict(region=reg.value, accType=("fixed", "aitken")[acc.value], relax=relax.value,
+                         fc=np.zeros(nf.value, np.int32), coeff=np.zeros(nf.value), kb=np.zeros(nf.value),
+                         dc=np.zeros(nf.value), value0=np.zeros(nf.value), off=np.zeros(nf.value + 1, np.int32),
+                         idx=np.zeros(nw.value, np.int32), w=np.zeros(nw.value))
+                lib().mrfg_case_dna_side(self._h, q, side, *[_p(d[k], C.c_int32 if d[k].dtype == np.int32
+                                                                 else C.c_double)
+                                                              for k in ("fc", "coeff", "kb", "dc", "value0", "off",
+                                                                        "idx", "w")])
+                pair[key] = d
+                pair["K"] = K.value
+            pairs.append(pair)
+        nreg = len(self.regions())
+        base = []
+        for r in range(nreg):
+            b = np.zeros(self.region(r)["nCells"])
+            lib().mrfg_case_region_source_base(self._h, r, _p(b, C.c_double))
+            base.append(b)
+        return dict(pairs=pairs, maxCoupleIter=it.value, maxJumpRes=jr.value, maxFluxRes=fr.value,
+                    sourceBase=base)
 
     def regions(self):
         return [self.region(r) for r in range(self.sizes()[0])]
@@ -458,6 +509,31 @@ class CoupledFvMatrix:
         c = np.zeros(self._ifaces[sys][iface][side])
         self._check(lib().mrfg_get_interface_coeff(self._h, sys, iface, side, _p(c, C.c_double)))
         return c
+
+    # ---- partitioned (Dirichlet-Neumann) coupling ---------------------------
+    def dna_solve(self, ctrl=None, fv_solution=None, entry="T", **kw):
+        """Partitioned coupling of every committed system (cases read with
+        regionCoupledTemperatureJump / regionCoupledHeatFlux patches); the
+        region solves use ``ctrl`` / the fvSolution ``entry`` / keyword
+        controls.  Returns one dict per system."""
+        if fv_solution is not None:
+            ctrl = parse_controls(fv_solution, entry)
+        elif ctrl is None:
+            ctrl = controls(**kw)
+        perf = (DnaPerf * max(1, self.n_systems))()
+        self._check(lib().mrfg_dna_solve(self._h, C.byref(ctrl), perf))
+        return [dict(iterations=p.iterations, converged=bool(p.converged), region_solves=p.region_solves,
+                     solver_iterations=p.solver_iterations, jump=p.jump_residual, flux=p.flux_residual,
+                     seconds=p.seconds) for p in perf[:self.n_systems]]
+
+    def dna_history(self, sys=0, max_iter=1000):
+        """Per DNA iteration of the last dna_solve: (jump, flux, omega) arrays."""
+        j, f, w = np.zeros(max_iter), np.zeros(max_iter), np.zeros(max_iter)
+        n = C.c_int()
+        self._check(lib().mrfg_dna_history(self._h, sys, max_iter, _p(j, C.c_double), _p(f, C.c_double),
+                                           _p(w, C.c_double), C.byref(n)))
+        k = min(n.value, max_iter)
+        return j[:k], f[:k], w[:k]
 
     def psi(self, sys=0):
         out = []

@@ -14,6 +14,7 @@
 #include <stdexcept>
 #include <thread>
 
+#include "dna.hpp"
 #include "kernels.hpp"
 
 namespace mrfg {
@@ -59,6 +60,9 @@ template struct DevBuf<int32_t>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<int8_t>;
 template struct DevBuf<FvRegionParams>;
+template struct DevBuf<DnaSide>;
+template struct DevBuf<DnaCell>;
+template struct DevBuf<DnaPair>;
 
 // ---------------------------------------------------------------------------
 // fvSolution solver entry
@@ -611,7 +615,7 @@ void Batch::checkDeviceError()
     }
 }
 
-void Batch::initState(const SolverControls &c)
+void Batch::initState(const SolverControls &c, const std::vector<char> *active)
 {
     std::vector<SysState> h(sys_.size());
     for (size_t q = 0; q < sys_.size(); ++q) {
@@ -620,7 +624,7 @@ void Batch::initState(const SolverControls &c)
         s.relTol = c.relTol;
         s.minIter = c.minIter;
         s.maxIter = c.precision == SolverControls::Mixed ? c.maxOuter : c.maxIter;
-        s.status = kRunning;
+        s.status = (!active || (*active)[q]) ? kRunning : kIdle;
         s.nCells = hInfo_[q].nCells;
         h[q] = s;
     }
@@ -767,8 +771,9 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
     HIP_OK(hipStreamSynchronize(stream_));
 }
 
-void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
+void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const std::vector<char> *active)
 {
+    if (active && active->size() != sys_.size()) throw std::runtime_error("active mask does not match the batch");
     HIP_OK(hipSetDevice(device_));
     if (c.precision == SolverControls::Mixed) ensureFloat();
     HIP_OK(hipStreamSynchronize(stream_));
@@ -777,7 +782,7 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf)
     // object is constructed per coupledFvMatrix::solve call
     factorised_ = false;
     factorise();
-    initState(c);
+    initState(c, active);
     std::vector<int> inner, fp64More;
     if (c.precision == SolverControls::Mixed) runMixed(c, inner, fp64More);
     else runFp64(c);

@@ -103,8 +103,9 @@ public:
     void setSystems(const std::vector<const CoupledSystem *> &systems);
     int nSystems() const { return (int)sys_.size(); }
 
-    // Solve every system; perf gets one entry per system.
-    void solve(const SolverControls &c, std::vector<SolvePerf> &perf);
+    // Solve every system (or those with active[q] != 0; the others keep
+    // their fields); perf gets one entry per system.
+    void solve(const SolverControls &c, std::vector<SolvePerf> &perf, const std::vector<char> *active = nullptr);
 
     // Time stepping of the transient systems (regions with ddt coefficients;
     // steady systems are re-solved from their current field): per step the
@@ -144,6 +145,11 @@ public:
     int traceDilu(long long *out, int maxWaves);
 
     const SystemLayout &layout(int sys) const { return *lay_[sys]; }
+    // device field and right-hand side (batch positions) and the position of
+    // global cell `cell` of system `sys`
+    const double *fieldDevice() const { return x64_.p; }
+    double *rhsDevice() { return b64_.p; }
+    int position(int sys, int cell) const { return posBegin(sys) + lay_.at(sys)->posOfCell.at(cell); }
     hipStream_t stream() const { return stream_; }
     // Canonical algorithmic bytes of one iteration / one SpMV / one DILU
     // apply for system `sys` at value size sv.
@@ -177,7 +183,7 @@ public:
 private:
     void ensureFloat();
     void factorise();
-    void initState(const SolverControls &c);
+    void initState(const SolverControls &c, const std::vector<char> *active = nullptr);
     void readState(std::vector<SysState> &out);
     void runFp64(const SolverControls &c);
     // Mixed FP32/FP64 iterative refinement; a system whose outer residual

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "dna_run.hpp"
 #include "serve.hpp"
 #include "coupled_system.hpp"
 #include "dictionary.hpp"
@@ -17,7 +18,9 @@ struct mrfg_ctx {
     std::string err;
     std::vector<std::unique_ptr<mrfg::CoupledSystem>> systems;
     std::unique_ptr<mrfg::Batch> batch;
+    std::unique_ptr<mrfg::DnaRun> dna;   // partitioned coupling of the committed systems
     bool committed = false;
+    bool preferLevels = false;
 };
 
 struct mrfg_case {
@@ -271,6 +274,7 @@ int mrfg_set_schedule(mrfg_ctx *ctx, int schedule)
     return guard(ctx, [&] {
         if (schedule != MRFG_SCHEDULE_AUTO && schedule != MRFG_SCHEDULE_LEVELS) throw ArgError("unknown schedule");
         ctx->batch->setPreferLevels(schedule == MRFG_SCHEDULE_LEVELS);
+        ctx->preferLevels = schedule == MRFG_SCHEDULE_LEVELS;
         ctx->committed = false;
     });
 }
@@ -289,8 +293,60 @@ int mrfg_commit(mrfg_ctx *ctx)
             }
             ptrs.push_back(s.get());
         }
+        ctx->dna.reset();
         ctx->batch->setSystems(ptrs);
         ctx->committed = true;
+    });
+}
+
+int mrfg_dna_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_dna_perf *perf)
+{
+    if (!ctx || !controls || !perf) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::Batch &B = committedBatch(ctx);
+        std::vector<const mrfg::CoupledSystem *> cases;
+        for (auto &s : ctx->systems) {
+            if (!s->part) throw ArgError("system " + std::to_string(cases.size()) +
+                                         " is not a partitioned case (read one with regionCoupledTemperatureJump / "
+                                         "regionCoupledHeatFlux patches)");
+            cases.push_back(s.get());
+        }
+        if (!ctx->dna) ctx->dna.reset(new mrfg::DnaRun(ctx->device, cases, ctx->preferLevels));
+        std::vector<mrfg::DnaPerf> out;
+        ctx->dna->run(fromControls(controls), out);
+        for (size_t q = 0; q < out.size(); ++q) {
+            mrfg_dna_perf &p = perf[q];
+            p.iterations = out[q].iterations;
+            p.converged = out[q].converged;
+            p.region_solves = out[q].regionSolves;
+            p.solver_iterations = out[q].solverIterations;
+            p.jump_residual = out[q].jumpResidual;
+            p.flux_residual = out[q].fluxResidual;
+            p.seconds = out[q].seconds;
+            // the coupled fields become the systems' current fields
+            const mrfg::CoupledSystem &S = *ctx->systems[q];
+            for (size_t r = 0; r < S.regions.size(); ++r) {
+                std::vector<double> psi(S.regions[r].nCells);
+                ctx->dna->getPsi((int)q, (int)r, psi.data());
+                B.setPsi((int)q, (int)r, psi.data());
+            }
+        }
+    });
+}
+
+int mrfg_dna_history(mrfg_ctx *ctx, int sys, int max_iter, double *jump, double *flux, double *omega, int *n)
+{
+    if (!ctx || !n || max_iter < 0) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        if (!ctx->dna) throw StateError("no partitioned solve has run");
+        if (sys < 0 || sys >= ctx->dna->nCases()) throw ArgError("system index out of range");
+        const auto &h = ctx->dna->history(sys);
+        *n = (int)h.size();
+        for (int i = 0; i < std::min<int>(max_iter, (int)h.size()); ++i) {
+            if (jump) jump[i] = h[i][0];
+            if (flux) flux[i] = h[i][1];
+            if (omega) omega[i] = h[i][2];
+        }
     });
 }
 
@@ -463,6 +519,64 @@ int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int 
     if (n_faces_b) *n_faces_b = I.b.nFaces();
     if (n_pairs_a) *n_pairs_a = (int)I.a.w.size();
     if (n_pairs_b) *n_pairs_b = (int)I.b.w.size();
+    return MRFG_OK;
+}
+
+int mrfg_case_dna_controls(const mrfg_case *c, int *n_pairs, int *max_couple_iter, double *max_jump_res,
+                           double *max_flux_res)
+{
+    if (!c || !n_pairs) return MRFG_ERR_ARG;
+    const auto &P = c->sys.part;
+    *n_pairs = P ? (int)P->pairs.size() : 0;
+    if (max_couple_iter) *max_couple_iter = P ? P->maxCoupleIter : 0;
+    if (max_jump_res) *max_jump_res = P ? P->maxJumpRes : -1.0;
+    if (max_flux_res) *max_flux_res = P ? P->maxFluxRes : -1.0;
+    return MRFG_OK;
+}
+
+namespace {
+const mrfg::PartitionedSide *dnaSide(const mrfg_case *c, int pair, int side)
+{
+    if (!c || !c->sys.part || pair < 0 || pair >= (int)c->sys.part->pairs.size() || side < 0 || side > 1)
+        return nullptr;
+    const auto &P = c->sys.part->pairs[pair];
+    return side ? &P.neu : &P.dir;
+}
+}  // namespace
+
+int mrfg_case_dna_side_sizes(const mrfg_case *c, int pair, int side, int *region, int *n_faces, int *n_weights,
+                             int *acc_type, double *relax, double *K)
+{
+    const mrfg::PartitionedSide *S = dnaSide(c, pair, side);
+    if (!S) return MRFG_ERR_ARG;
+    if (region) *region = S->region;
+    if (n_faces) *n_faces = S->nFaces();
+    if (n_weights) *n_weights = (int)S->w.size();
+    if (acc_type) *acc_type = S->accType;
+    if (relax) *relax = S->relax;
+    if (K) *K = c->sys.part->pairs[pair].K;
+    return MRFG_OK;
+}
+
+int mrfg_case_dna_side(const mrfg_case *c, int pair, int side, int32_t *fc, double *coeff, double *kb, double *dc,
+                       double *value0, int32_t *off, int32_t *idx, double *w)
+{
+    const mrfg::PartitionedSide *S = dnaSide(c, pair, side);
+    if (!S) return MRFG_ERR_ARG;
+    auto cp = [](auto *dst, const auto &src) {
+        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    cp(fc, S->fc); cp(coeff, S->coeff); cp(kb, S->kb); cp(dc, S->dc); cp(value0, S->value0);
+    cp(off, S->off); cp(idx, S->idx); cp(w, S->w);
+    return MRFG_OK;
+}
+
+int mrfg_case_region_source_base(const mrfg_case *c, int region, double *source_base)
+{
+    if (!c || !source_base || !c->sys.part || region < 0 || region >= (int)c->sys.part->sourceBase.size())
+        return MRFG_ERR_ARG;
+    const auto &v = c->sys.part->sourceBase[region];
+    std::memcpy(source_base, v.data(), v.size() * sizeof(double));
     return MRFG_OK;
 }
 

@@ -83,10 +83,45 @@ struct FvModel {
     std::vector<FvSideModel> sides;     // interface i: side a = 2i, side b = 2i + 1
 };
 
+// Partitioned (Dirichlet-Neumann) coupling through heatTransferInterface
+// pairs (regionInterfacePropertiesPartitioned, dnaControl): each region is
+// solved on its own; the Dirichlet side (regionCoupledTemperatureJump) takes
+// a relaxed fixed value from the other side's patch values, the Neumann side
+// (regionCoupledHeatFlux) the gradient that matches the other side's heat
+// flux.  A side's source term is linear in its boundary value (temperature
+// or gradient): source[fc[i]] += coeff[i] * value[i], after every other
+// boundary contribution of the cell.
+struct PartitionedSide {
+    int region = -1;
+    std::vector<int32_t> fc;          // face cells
+    std::vector<double> coeff;        // source term per unit boundary value
+    std::vector<double> kb, dc;       // face conductivity, patch deltaCoeffs 1/|d|
+    std::vector<double> value0;       // initial patch values
+    std::vector<int32_t> off, idx;    // per face: the other side's faces it gathers (GGI)
+    std::vector<double> w;
+    int accType = 0;                  // 0 fixed relaxation, 1 Aitken
+    double relax = 1.0;
+    int nFaces() const { return (int)fc.size(); }
+};
+
+struct PartitionedPair {
+    std::string name;
+    double K = 1.0;                   // temperature jump coefficient
+    PartitionedSide dir, neu;
+};
+
+struct PartitionedModel {
+    std::vector<PartitionedPair> pairs;
+    std::vector<std::vector<double>> sourceBase;   // per region: source without the partitioned faces
+    int maxCoupleIter = 1;
+    double maxJumpRes = -1.0, maxFluxRes = -1.0;    // < 0: no residual control
+};
+
 struct CoupledSystem {
     std::vector<LduRegion> regions;
     std::vector<CoupledInterface> interfaces;
     std::shared_ptr<const FvModel> fv;   // cases read from disk; null otherwise
+    std::shared_ptr<const PartitionedModel> part;   // partitioned set-ups read from disk
     long long nCells() const
     {
         long long n = 0;

@@ -361,6 +361,8 @@ void computeGeometry(PolyMesh &m)
 struct PatchBC {
     std::string type;
     std::vector<double> value;   // per face (scalar) or 3 per face (vector)
+    std::string accType = "fixed";   // partitioned coupling: acceleration model
+    double relax = 1.0;
 };
 
 struct FieldFile {
@@ -424,6 +426,8 @@ FieldFile readField(const std::string &path, const PolyMesh &m, int nComp)
         bc.type = pd->word("type");
         if (pd->found("value") && bc.type != "empty")
             bc.value = fieldValues(d, pd->lookup("value").tokens, p.size, nComp, path + " patch " + p.name);
+        bc.accType = pd->wordOrDefault("accType", "fixed");
+        bc.relax = pd->numberOrDefault("relax", 1.0);
         F.patches[p.name] = bc;
     }
     return F;
@@ -441,7 +445,11 @@ double property(const Dict &d, const std::string &key, const std::string &file)
 // ---------------------------------------------------------------------------
 // Regions and assembly
 // ---------------------------------------------------------------------------
-enum class BcT { FixedValue, ZeroGradient, Empty, Coupled };
+// PartDirichlet / PartNeumann: the two sides of a partitioned interface
+// (regionCoupledTemperatureJump, a relaxed fixed value; regionCoupledHeatFlux,
+// a fixed gradient).  Their source terms are linear in the boundary value and
+// are assembled per unit value (the driver of the coupling supplies the value).
+enum class BcT { FixedValue, ZeroGradient, Empty, Coupled, PartDirichlet, PartNeumann };
 
 struct FoamRegion {
     PolyMesh mesh;
@@ -454,6 +462,8 @@ struct FoamRegion {
     std::vector<std::vector<double>> phiB;     // per patch
     std::vector<double> U;              // 3 per cell (transport regions)
     std::vector<int8_t> bcU, kFixed;    // per patch: FvUBc, k given on the patch
+    std::vector<std::string> accType;   // per patch (partitioned sides)
+    std::vector<double> relax;
     std::vector<std::vector<double>> Ub;       // per patch: fixed U, 3 per face
     // coupled patches: face conductivity and delta coefficients, and the
     // boundary coefficients handed to the interface
@@ -517,6 +527,19 @@ FvMatrixG laplacian(const FoamRegion &R)
                 M.boundaryCoeffs[p][i] = -gammaMagSf * delta;
                 break;
             }
+            case BcT::PartDirichlet: {   // fixed value, per unit value
+                double deltab = 1.0 / m.nDist(f);
+                double gammaMagSf = R.kb[p][i] * m.magSf[f];
+                M.internalCoeffs[p][i] = gammaMagSf * (-deltab);
+                M.boundaryCoeffs[p][i] = -gammaMagSf * (deltab * 1.0);
+                break;
+            }
+            case BcT::PartNeumann: {     // fixed gradient, per unit gradient
+                double gammaMagSf = R.kb[p][i] * m.magSf[f];
+                M.internalCoeffs[p][i] = gammaMagSf * 0.0;
+                M.boundaryCoeffs[p][i] = -gammaMagSf * 1.0;
+                break;
+            }
             default:
                 break;
             }
@@ -552,6 +575,14 @@ FvMatrixG divUpwind(const FoamRegion &R)
             case BcT::ZeroGradient:
                 M.internalCoeffs[p][i] = ph * 1.0;
                 M.boundaryCoeffs[p][i] = -ph * 0.0;
+                break;
+            case BcT::PartDirichlet:
+                M.internalCoeffs[p][i] = ph * 0.0;
+                M.boundaryCoeffs[p][i] = -ph * 1.0;
+                break;
+            case BcT::PartNeumann:
+                M.internalCoeffs[p][i] = ph * 1.0;
+                M.boundaryCoeffs[p][i] = -ph * (1.0 / (1.0 / m.nDist(m.patches[p].start + i)));
                 break;
             case BcT::Empty:
                 break;
@@ -804,6 +835,8 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
     R.bcT.resize(nP);
     R.Tb.resize(nP);
     R.kb.resize(nP);
+    R.accType.assign(nP, "fixed");
+    R.relax.assign(nP, 1.0);
     for (size_t p = 0; p < nP; ++p) {
         const Patch &P = m.patches[p];
         const PatchBC &bc = T.patches.at(P.name);
@@ -817,6 +850,15 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
             R.bcT[p] = BcT::Empty;
         } else if (bc.type == "monolithicTemperature") {
             R.bcT[p] = BcT::Coupled;
+        } else if (bc.type == "regionCoupledTemperatureJump" || bc.type == "regionCoupledHeatFlux") {
+            R.bcT[p] = bc.type == "regionCoupledHeatFlux" ? BcT::PartNeumann : BcT::PartDirichlet;
+            if ((int)bc.value.size() != P.size) throw FoamError(fdir + "T: patch '" + P.name + "' has no value");
+            if (bc.accType != "fixed" && bc.accType != "aitken")
+                throw FoamError(fdir + "T: patch '" + P.name + "': acceleration '" + bc.accType +
+                                "' is not supported (fixed, aitken)");
+            R.Tb[p] = bc.value;
+            R.accType[p] = bc.accType;
+            R.relax[p] = bc.relax;
         } else {
             throw FoamError(fdir + "T: patch '" + P.name + "': unsupported type '" + bc.type + "'");
         }
@@ -1106,6 +1148,116 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
                 sides.push_back({reg[0], pat[0], reg[1], pat[1]});
             }
     }
+    // partitioned interfaces (heatTransferInterface pairs) and DNA controls
+    std::shared_ptr<PartitionedModel> partModel;
+    std::vector<std::array<int, 2>> partPatch;   // per pair: Dirichlet patch, Neumann patch
+    {
+        bool anyPart = false;
+        for (auto &rc : R)
+            for (BcT t : rc.bcT) anyPart = anyPart || t == BcT::PartDirichlet || t == BcT::PartNeumann;
+        if (anyPart) {
+            std::string pfile = dir + "/constant/" + (opt.interfaceFile.empty() ? "regionInterfaceProperties" : opt.interfaceFile);
+            bool has = fileExists(pfile) && parseDict(readFile(pfile)).found("partitionedCoupledPatches") &&
+                       !parseDict(readFile(pfile)).list("partitionedCoupledPatches").empty();
+            if (!has && opt.interfaceFile.empty()) pfile = dir + "/constant/regionInterfacePropertiesPartitioned";
+            Dict d = parseDict(readFile(pfile));
+            partModel = std::make_shared<PartitionedModel>();
+            partModel->sourceBase.resize(R.size());
+            for (const Value &it : d.list("partitionedCoupledPatches")) {
+                if (!it.isDict()) continue;
+                const Dict &pd = *it.dict;
+                const std::string itype = pd.wordOrDefault("interfaceType", "heatTransferInterface");
+                if (itype != "heatTransferInterface")
+                    throw FoamError(pfile + ": interface type '" + itype + "' is not supported (heatTransferInterface)");
+                const auto &pp = pd.list("coupledPatchPair");
+                if (pp.size() != 2 || !pp[0].isList() || !pp[1].isList())
+                    throw FoamError(pfile + ": coupledPatchPair must be ((regionA patchA) (regionB patchB))");
+                int reg[2], pat[2];
+                for (int q = 0; q < 2; ++q) {
+                    const std::string &rn = pp[q].items.at(0).asWord(), &pn = pp[q].items.at(1).asWord();
+                    auto f = byName.find(rn);
+                    if (f == byName.end()) throw FoamError(pfile + ": unknown region '" + rn + "'");
+                    reg[q] = f->second;
+                    pat[q] = -1;
+                    const PolyMesh &m = R[reg[q]].mesh;
+                    for (size_t p = 0; p < m.patches.size(); ++p)
+                        if (m.patches[p].name == pn) pat[q] = (int)p;
+                    if (pat[q] < 0) throw FoamError(pfile + ": region '" + rn + "' has no patch '" + pn + "'");
+                }
+                const BcT t0 = R[reg[0]].bcT[pat[0]], t1 = R[reg[1]].bcT[pat[1]];
+                if (!((t0 == BcT::PartDirichlet && t1 == BcT::PartNeumann) ||
+                      (t0 == BcT::PartNeumann && t1 == BcT::PartDirichlet)))
+                    throw FoamError(pfile + ": interface '" + pd.name + "' needs regionCoupledTemperatureJump on one "
+                                    "side and regionCoupledHeatFlux on the other");
+                const int qd = t0 == BcT::PartDirichlet ? 0 : 1, qn = 1 - qd;
+                PartitionedPair P;
+                P.name = pd.name;
+                P.K = pd.isDict("heatTransferInterfaceCoeffs")
+                          ? property(pd.subDict("heatTransferInterfaceCoeffs"), "K", pfile) : 1.0;
+                const Patch &PD = R[reg[qd]].mesh.patches[pat[qd]], &PN = R[reg[qn]].mesh.patches[pat[qn]];
+                const Frame FD = patchFrame(R[reg[qd]].mesh, PD), FN = patchFrame(R[reg[qn]].mesh, PN);
+                if (dot(FD.n, FN.n) > -(1.0 - 1e-9) ||
+                    std::fabs(dot(FD.n, FN.o - FD.o)) > 1e-6 * std::sqrt(R[reg[qd]].mesh.magSf[PD.start]))
+                    throw FoamError(pfile + ": coupled patches of '" + pd.name + "' are not in one plane");
+                auto pD = patchPolys(R[reg[qd]].mesh, PD, FD);
+                auto pN = patchPolys(R[reg[qn]].mesh, PN, FD);
+                InterfaceSide gD, gN;
+                ggiWeights(pD, pN, gD);
+                ggiWeights(pN, pD, gN);
+                auto fill = [&](PartitionedSide &S, int rr, int pa, const InterfaceSide &g) {
+                    const FoamRegion &rc = R[rr];
+                    const PolyMesh &m = rc.mesh;
+                    const Patch &Pt = m.patches[pa];
+                    S.region = rr;
+                    for (int i = 0; i < Pt.size; ++i) {
+                        const int f = Pt.start + i;
+                        S.fc.push_back(m.owner[f]);
+                        S.kb.push_back(rc.kb[pa][i]);
+                        S.dc.push_back(1.0 / m.nDist(f));
+                        S.value0.push_back(rc.Tb[pa][i]);
+                        if (g.off[i + 1] == g.off[i])
+                            throw FoamError(pfile + ": face " + std::to_string(i) + " of patch '" + Pt.name +
+                                            "' has no partner face");
+                    }
+                    S.off = g.off;
+                    S.idx = g.idx;
+                    S.w = g.w;
+                    S.accType = rc.accType[pa] == "aitken" ? 1 : 0;
+                    S.relax = rc.relax[pa];
+                };
+                fill(P.dir, reg[qd], pat[qd], gD);
+                fill(P.neu, reg[qn], pat[qn], gN);
+                partModel->pairs.push_back(std::move(P));
+                partPatch.push_back({pat[qd], pat[qn]});
+            }
+            for (size_t r = 0; r < R.size(); ++r)
+                for (size_t p = 0; p < R[r].bcT.size(); ++p)
+                    if ((R[r].bcT[p] == BcT::PartDirichlet || R[r].bcT[p] == BcT::PartNeumann) &&
+                        R[r].mesh.patches[p].size > 0) {
+                        bool found = false;
+                        for (size_t i = 0; i < partPatch.size(); ++i)
+                            for (int q = 0; q < 2; ++q)
+                                found = found || ((q ? partModel->pairs[i].neu : partModel->pairs[i].dir).region ==
+                                                      (int)r && partPatch[i][q] == (int)p);
+                        if (!found)
+                            throw FoamError("partitioned patch '" + R[r].mesh.patches[p].name + "' of region '" +
+                                            R[r].mesh.name + "' has no partitionedCoupledPatches entry");
+                    }
+            // DNA controls of T (constant/multiRegionProperties, dnaControl::read)
+            Dict mrp = parseDict(readFile(dir + "/constant/multiRegionProperties"));
+            if (mrp.isDict("DNA") && mrp.subDict("DNA").isDict("T")) {
+                const Dict &dt = mrp.subDict("DNA").subDict("T");
+                partModel->maxCoupleIter = (int)dt.numberOrDefault("maxCoupleIter", 1);
+                if (dt.isDict("residualControl")) {
+                    const Dict &rcd = dt.subDict("residualControl");
+                    if (!rcd.found("maxJumpRes") || !rcd.found("maxFluxRes"))
+                        throw FoamError("DNA residualControl of T: maxJumpRes and maxFluxRes are required");
+                    partModel->maxJumpRes = rcd.number("maxJumpRes");
+                    partModel->maxFluxRes = rcd.number("maxFluxRes");
+                }
+            }
+        }
+    }
     for (size_t r = 0; r < R.size(); ++r)
         for (size_t p = 0; p < R[r].bcT.size(); ++p)
             if (R[r].bcT[p] == BcT::Coupled && !R[r].coupledK.count((int)p) && R[r].mesh.patches[p].size > 0)
@@ -1136,9 +1288,22 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
             for (int i = 0; i < m.patches[p].size; ++i)
                 L.diag[m.owner[m.patches[p].start + i]] += E.internalCoeffs[p][i];
         for (size_t p = 0; p < m.patches.size(); ++p) {
-            if (rc.bcT[p] == BcT::Coupled) continue;
+            if (rc.bcT[p] == BcT::Coupled || rc.bcT[p] == BcT::PartDirichlet || rc.bcT[p] == BcT::PartNeumann)
+                continue;
             for (int i = 0; i < m.patches[p].size; ++i)
                 L.source[m.owner[m.patches[p].start + i]] += E.boundaryCoeffs[p][i];
+        }
+        if (partModel) {
+            // partitioned sides: per unit value, then the initial values
+            // (Dirichlet: the patch value; Neumann: zero gradient)
+            partModel->sourceBase[r] = L.source;
+            for (auto &pp : partModel->pairs)
+                for (PartitionedSide *ps : {&pp.dir, &pp.neu}) {
+                    if (ps->region != (int)r) continue;
+                    const int pat = ps == &pp.dir ? partPatch[&pp - &partModel->pairs[0]][0]
+                                                  : partPatch[&pp - &partModel->pairs[0]][1];
+                    ps->coeff = E.boundaryCoeffs[pat];
+                }
         }
         for (size_t i = 0; i < sides.size(); ++i)
             for (int s = 0; s < 2; ++s) {
@@ -1159,7 +1324,22 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
             }
         }
     }
-    sys.fv = buildFvModel(R, rDeltaT, sides, sys);
+    if (partModel) {
+        // the initial boundary values: Dirichlet sides their patch values,
+        // Neumann sides a zero gradient
+        for (auto &pp : partModel->pairs)
+            for (const PartitionedSide *ps : {&pp.dir, &pp.neu}) {
+                LduRegion &L = sys.regions[ps->region];
+                for (int i = 0; i < ps->nFaces(); ++i)
+                    L.source[ps->fc[i]] += ps->coeff[i] * (ps == &pp.dir ? ps->value0[i] : 0.0);
+            }
+        for (auto &L : sys.regions)
+            if (L.transient()) throw FoamError("partitioned coupling is supported for steady runs (ddt steadyState)");
+        sys.part = partModel;
+        sys.fv = nullptr;   // device re-assembly covers monolithic set-ups
+    } else {
+        sys.fv = buildFvModel(R, rDeltaT, sides, sys);
+    }
     if (info) {
         info->volumes.clear();
         info->maxNonOrthDeg.clear();

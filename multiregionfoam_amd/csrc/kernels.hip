@@ -9,6 +9,7 @@
 // LDU face loop it restates.
 #include <hip/hip_runtime.h>
 
+#include "dd.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -26,32 +27,6 @@ constexpr double kSmall = 1e-20;  // lduMatrix::small_
 // relative and its rounding to double does not depend on the order of
 // summation: every system reduces to the same scalars alone or in a batch,
 // and the same as a sequential compensated sum over its cells.
-struct DD {
-    double hi, lo;
-};
-
-__device__ inline DD twoSum(double a, double b)
-{
-    const double s = a + b;
-    const double bb = s - a;
-    return DD{s, (a - (s - bb)) + (b - bb)};
-}
-
-__device__ inline DD ddAdd(DD x, DD y)
-{
-    const DD s = twoSum(x.hi, y.hi);
-    const double e = s.lo + (x.lo + y.lo);
-    const double hi = s.hi + e;
-    return DD{hi, e - (hi - s.hi)};
-}
-
-__device__ inline DD waveSum(DD v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = ddAdd(v, DD{__shfl_xor(v.hi, o, 64), __shfl_xor(v.lo, o, 64)});
-    return v;
-}
-
 // Block-reduce NV values and store the block's partial sums (slot j as the
 // double-double pair partials[(blockIdx.x * 4 + j) * 2 + {0, 1}]).  No device-scope fence is needed: the
 // partials are read by the finalise kernel launched after this one.

@@ -85,6 +85,9 @@ def write_field(path, cls, internal, patches):
         lines.append("        type %s;" % p["type"])
         if "value" in p and p["type"] != "empty":
             lines.append("        value %s;" % _value_text(p["value"]))
+        for key, v in p.items():
+            if key not in ("type", "value"):
+                lines.append("        %s %s;" % (key, v))
         lines.append("    }")
     lines.append("}")
     _write(path, cls, os.path.basename(path), "\n".join(lines) + "\n")
@@ -114,12 +117,20 @@ def load_fixture():
     return meta, meshes
 
 
-def write_fixture_case(dest, params=None, ddt=None):
+DNA_T = ("DNA\n{\n    T\n    {\n        maxCoupleIter %d;\n        residualControl\n        {\n"
+         "            maxJumpRes %s;\n            maxFluxRes %s;\n        }\n    }\n}\n")
+
+
+def write_fixture_case(dest, params=None, ddt=None, partitioned=None):
     """params: the test suite's parameter map (genCases.py update_params),
     dict(kf=..., cp=...): the fluid's conductivity (transportProperties and
     0/fluid/k) and specific heat; optionally U (the fluid's uniform internal
     velocity) and T (dict region -> uniform internal temperature).  ddt
-    replaces every region's ddt scheme."""
+    replaces every region's ddt scheme.  partitioned: the tutorial's
+    partitioned set-up instead of the monolithic one (what Allrun -p copies:
+    regionInterfacePropertiesPartitioned, 0/*/orig/partitioned/T, the DNA
+    controls of multiRegionProperties), dict(accType=..., relax=...,
+    maxCoupleIter=20, maxJumpRes=1e-7, maxFluxRes=1e-7, K=1)."""
     meta, meshes = load_fixture()
     params = params or {}
     if "kf" in params:
@@ -135,12 +146,36 @@ def write_fixture_case(dest, params=None, ddt=None):
         for region in meta["ddt"]:
             meta["ddt"][region] = ddt
     regions_txt = " ".join("(%s (%s))" % (r, " ".join(t)) for r, t in meta["regions"])
-    write_dict(os.path.join(dest, "constant", "multiRegionProperties"), "regions (%s);\n" % regions_txt)
-    pairs = " ".join("%s%s { coupledPatchPair ((%s %s) (%s %s)); }" % (a[0], b[0], a[0], a[1], b[0], b[1])
-                     for a, b in meta["coupledPatchPairs"])
-    write_dict(os.path.join(dest, "constant", "regionInterfaceProperties"),
-               "monolithicCoupledPatches ( %s );\ninterfaceTransferMethod %s;\n"
-               % (pairs, meta["interfaceTransferMethod"]))
+    mrp = "regions (%s);\n" % regions_txt
+    if partitioned is not None:
+        pt = dict(accType="fixed", relax=1.0, maxCoupleIter=20, maxJumpRes=1e-7, maxFluxRes=1e-7, K=1.0)
+        pt.update(partitioned)
+        mrp += DNA_T % (pt["maxCoupleIter"], _num(pt["maxJumpRes"]), _num(pt["maxFluxRes"]))
+        pairs = " ".join(
+            "%s%s { interfaceType heatTransferInterface; coupledPatchPair ((%s %s) (%s %s)); coupledFields (T); "
+            "heatTransferInterfaceCoeffs { K K [0 0 0 0 0 0 0] %s; } }" % (a[0], b[0], a[0], a[1], b[0], b[1],
+                                                                         _num(pt["K"]))
+            for a, b in meta["coupledPatchPairs"])
+        write_dict(os.path.join(dest, "constant", "regionInterfaceProperties"),
+                   "partitionedCoupledPatches ( %s );\nmonolithicCoupledPatches ( );\ninterfaceTransferMethod %s;\n"
+                   % (pairs, meta["interfaceTransferMethod"]))
+        for (ra, pa), (rb, pb) in meta["coupledPatchPairs"]:
+            meta["fields"][ra]["T"]["patches"][pa] = dict(
+                type="regionCoupledTemperatureJump", value=meta["fields"][ra]["T"]["internal"],
+                interfaceType="heatTransferInterface", k='"k"', accType=pt["accType"], relax=_num(pt["relax"]),
+                neighbourRegionName=rb, neighbourPatchName=pb, neighbourFieldName="T", secondOrder="false",
+                nonOrthCorr="true")
+            meta["fields"][rb]["T"]["patches"][pb] = dict(
+                type="regionCoupledHeatFlux", value=meta["fields"][rb]["T"]["internal"],
+                interfaceType="heatTransferInterface", k='"k"', neighbourRegionName=ra, neighbourPatchName=pa,
+                neighbourFieldName="T", secondOrder="false", nonOrthCorr="true")
+    else:
+        pairs = " ".join("%s%s { coupledPatchPair ((%s %s) (%s %s)); }" % (a[0], b[0], a[0], a[1], b[0], b[1])
+                         for a, b in meta["coupledPatchPairs"])
+        write_dict(os.path.join(dest, "constant", "regionInterfaceProperties"),
+                   "monolithicCoupledPatches ( %s );\ninterfaceTransferMethod %s;\n"
+                   % (pairs, meta["interfaceTransferMethod"]))
+    write_dict(os.path.join(dest, "constant", "multiRegionProperties"), mrp)
     write_dict(os.path.join(dest, "system", "controlDict"), "deltaT %s;\n" % _num(meta["deltaT"]))
     for region, types in meta["regions"]:
         m = meshes[region]

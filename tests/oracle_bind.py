@@ -6,6 +6,7 @@ cpu_baseline leg use this module, as the checker.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -395,3 +396,160 @@ class OracleSystem:
     def split(self, v):
         off = self.offsets()
         return [np.array(v[off[g]:off[g + 1]]) for g in range(len(self.regions))]
+
+
+# ---------------------------------------------------------------------------
+# Partitioned (Dirichlet-Neumann) coupling: the DNA loop restated
+# ---------------------------------------------------------------------------
+
+SMALL = 1e-15   # foam-extend's doubleScalarSMALL
+
+
+def _fsum(v):
+    return math.fsum(np.asarray(v, dtype=np.float64).tolist())
+
+
+def _gather(side, vals):
+    """interpolateFromNbrField through the GGI weights: out_i = sum_p w_p vals[idx_p]
+    (sequential per face)."""
+    out = np.zeros(len(side["fc"]))
+    for i in range(len(out)):
+        acc = 0.0
+        for p in range(side["off"][i], side["off"][i + 1]):
+            acc += side["w"][p] * vals[side["idx"][p]]
+        out[i] = acc
+    return out
+
+
+class _Acceleration:
+    """accelerationModel / fixedRelaxation / aitkenRelaxation
+    (acceleration/accelerationModel/accelerationModel.C:72-86,
+    fixedRelaxation.C:76-106, aitkenRelaxation.C:83-110,
+    aitkenRelaxationTemplates.C:111-168): prevFld starts at the patch's
+    initial values (initialize(*this) of both patch types); the Aitken factor
+    keeps the initial relaxation for the first two corrections of a time
+    step."""
+
+    def __init__(self, kind, relax, init):
+        self.kind, self.init = kind, float(relax)
+        self.omega = float(relax)
+        self.prev = np.array(init, dtype=np.float64)
+        self.res = None
+        self.prev_res = None
+        self.corr = 0
+
+    def relax(self, cur):
+        if self.corr == 0:
+            self.corr = 1
+        self.prev_res = self.res
+        self.res = cur - self.prev
+        if self.kind == "aitken":
+            if self.corr < 3:
+                self.omega = self.init if self.omega < self.init else self.omega
+            else:
+                d = self.res - self.prev_res
+                w = -self.omega * (_fsum(self.prev_res * d) / (_fsum(d * d) + SMALL))
+                w = abs(w)
+                if w > 1.0:
+                    w = 1.0
+                elif w < SMALL:
+                    w = self.init
+                self.omega = w
+            out = self.prev + self.omega * self.res
+        else:
+            out = self.prev + self.init * self.res
+        self.prev = out
+        self.corr += 1
+        return out
+
+
+def _norm_res(r, a, b):
+    n = max(min(math.sqrt(_fsum(a * a)), math.sqrt(_fsum(b * b))), SMALL)
+    return math.sqrt(_fsum(r * r)) / n
+
+
+def dna_solve(case_regions, part, **solve_kw):
+    """The partitioned coupling loop of multiRegionSystem::solve
+    (multiRegionSystem.C:571-610) under dnaControl::loop (dnaControl.C:993-1061)
+    for heatTransferInterface pairs, steady: per DNA iteration every region in
+    order updates its partitioned patches -- the Dirichlet side
+    (genericRegionCoupledJumpFvPatchField::updateCoeffs, :165-217) takes
+    K * (the other side's patch values), relaxed; the Neumann side
+    (genericRegionCoupledFluxFvPatchField::updateCoeffs, :173-239) takes
+    gradient = -(the other side's flux k snGrad T) / k, relaxed -- and is
+    solved from its current field; a Neumann patch then evaluates to
+    T_P + g / deltaCoeffs (genericRegionCoupledFluxFvPatchFields.C:55-136).
+    After the first iteration, the loop stops when every pair's jump residual
+    (genericRegionCoupledJumpFvPatchField::normResidual, :316-366) and flux
+    residual (genericRegionCoupledFluxFvPatchField::normResidual) are within
+    the limits, or after maxCoupleIter iterations.  Region sources are
+    sourceBase + coeff * value per partitioned face, in pair order, Dirichlet
+    side first.  Returns dict(psi, iterations, converged, history, solves)."""
+    nreg = len(case_regions)
+    systems = [OracleSystem.from_arrays([R], []) for R in case_regions]
+    x = [np.array(R["psi"], dtype=np.float64) for R in case_regions]
+    pairs = part["pairs"]
+    st = []
+    for P in pairs:
+        D, N = P["dir"], P["neu"]
+        st.append(dict(TD=np.array(D["value0"]), g=np.zeros(len(N["fc"])), TN=np.array(N["value0"]),
+                       accD=_Acceleration(D["accType"], D["relax"], D["value0"]),
+                       accN=_Acceleration(N["accType"], N["relax"], N["value0"])))
+
+    def flux_dir(q):   # k snGrad T of the Dirichlet side
+        D = pairs[q]["dir"]
+        return ((st[q]["TD"] - x[D["region"]][D["fc"]]) * D["dc"]) * D["kb"]
+
+    def residuals():
+        jr = fr = 0.0
+        for q, P in enumerate(pairs):
+            D, N = P["dir"], P["neu"]
+            nbr = _gather(D, st[q]["TN"])
+            jump = nbr * (P["K"] - 1.0)
+            jr = max(jr, _norm_res((st[q]["TD"] - nbr) - jump, st[q]["TD"], nbr + jump))
+            fown = st[q]["g"] * N["kb"]
+            fnbr = _gather(N, flux_dir(q))
+            fr = max(fr, _norm_res((fown + fnbr) - 0.0, fown, -1.0 * fnbr + 0.0))
+        return jr, fr
+
+    control = part["maxJumpRes"] >= 0.0
+    history, solves, corr, converged = [], [], 0, False
+    res = (None, None)
+    while True:
+        corr += 1
+        if corr > 1 and control and res[0] <= part["maxJumpRes"] and res[1] <= part["maxFluxRes"]:
+            converged = True
+            break
+        if corr > part["maxCoupleIter"]:
+            break
+        for r in range(nreg):
+            for q, P in enumerate(pairs):
+                if P["dir"]["region"] == r:
+                    nbr = _gather(P["dir"], st[q]["TN"])
+                    cur = nbr + nbr * (P["K"] - 1.0)
+                    st[q]["TD"] = st[q]["accD"].relax(cur)
+                if P["neu"]["region"] == r:
+                    fl = _gather(P["neu"], flux_dir(q))
+                    fl = fl * -1.0
+                    fl = fl + 0.0
+                    st[q]["g"] = st[q]["accN"].relax(fl / P["neu"]["kb"])
+            src = np.array(part["sourceBase"][r], dtype=np.float64)
+            for q, P in enumerate(pairs):
+                for key, vals in (("dir", st[q]["TD"]), ("neu", st[q]["g"])):
+                    S = P[key]
+                    if S["region"] != r:
+                        continue
+                    for i, c in enumerate(S["fc"]):
+                        src[c] = src[c] + S["coeff"][i] * vals[i]
+            systems[r].set_values(0, source=src)
+            psi, perf = systems[r].solve(psi0=[x[r]], **solve_kw)
+            x[r] = psi[0]
+            solves.append(perf)
+            for q, P in enumerate(pairs):
+                N = P["neu"]
+                if N["region"] == r:
+                    st[q]["TN"] = x[r][N["fc"]] + st[q]["g"] / N["dc"]
+        res = residuals()
+        history.append(dict(jump=res[0], flux=res[1], omega=[s["accD"].omega for s in st]))
+    return dict(psi=x, iterations=corr - 1, converged=converged, history=history, solves=solves,
+                boundary=[dict(TD=s["TD"], g=s["g"], TN=s["TN"]) for s in st])

@@ -1,0 +1,173 @@
+"""Partitioned (Dirichlet-Neumann) coupling: the reference tutorial's
+partitioned set-up (Allrun -p: regionInterfacePropertiesPartitioned,
+0/*/orig/partitioned/T with regionCoupledTemperatureJump on the fluid and
+regionCoupledHeatFlux on the solid, the DNA controls of
+multiRegionProperties) and the test suite's partitioned_Aitken cases.
+
+The oracle's DNA loop (oracle_bind.dna_solve) is pinned by what the
+mathematics fixes, on the CPU:
+* at K = 1 on a conformal interface, the converged partitioned solution
+  satisfies both interface conditions of the monolithic discretisation, so it
+  tends to the monolithic solution as the residual limit tightens (error
+  below 1e-6 K at 1e-12, and smaller than at 1e-7);
+* Aitken relaxation needs fewer DNA iterations than fixed relaxation with
+  the same initial factor; without residual control, or with too few
+  iterations, the loop stops at maxCoupleIter, unconverged;
+* the reader's per-face coefficients are the closed forms k |S| / |d|
+  (Dirichlet) and k |S| (Neumann), with |S| the same on both sides of the
+  conformal interface.
+On the GPU, mrfg_dna_solve equals the oracle's loop bit for bit: DNA
+iterations, the residual and relaxation-factor history, and the fields.
+"""
+import numpy as np
+import pytest
+
+import multiregionfoam_amd as mrf
+import foam_fixture as ff
+import oracle_bind as ob
+
+SOLVE = dict(tol=1e-12, relTol=0.0, minIter=0, maxIter=1000)
+FV_SOLUTION = "solvers { T { solver BiCGStab; preconditioner DILU; tolerance 1e-12; relTol 0; maxIter 1000; } }"
+
+
+def _read(tmp_path_factory, partitioned=None, params=None):
+    d = tmp_path_factory.mktemp("case")
+    ff.write_fixture_case(str(d), params=params, partitioned=partitioned)
+    return mrf.Case.from_foam(str(d), ddt="steadyState")
+
+
+@pytest.fixture(scope="module")
+def monolithic(tmp_path_factory):
+    c = _read(tmp_path_factory)
+    O = ob.OracleSystem.from_arrays(c.regions(), c.interfaces())
+    psi, _ = O.solve(tol=1e-15, maxIter=3000)
+    return psi
+
+
+def test_reader_partitioned_setup(tmp_path_factory):
+    c = _read(tmp_path_factory, partitioned=dict(accType="aitken", relax=0.75))
+    P = c.partitioned()
+    assert c.interfaces() == []
+    assert (P["maxCoupleIter"], P["maxJumpRes"], P["maxFluxRes"]) == (20, 1e-7, 1e-7)
+    (pair,) = P["pairs"]
+    D, N = pair["dir"], pair["neu"]
+    assert (D["region"], N["region"], pair["K"]) == (0, 1, 1.0)
+    assert (D["accType"], D["relax"], N["accType"], N["relax"]) == ("aitken", 0.75, "fixed", 1.0)
+    assert np.all(D["value0"] == 300.0) and np.all(N["value0"] == 310.0)
+    # closed forms: Dirichlet k|S|/|d|, Neumann k|S|; one |S| per conformal face pair
+    area_d = D["coeff"] / (D["kb"] * D["dc"])
+    area_n = N["coeff"] / N["kb"]
+    np.testing.assert_allclose(area_d, area_n[D["idx"]], rtol=1e-12)
+    assert np.array_equal(D["w"], np.ones(len(D["fc"])))
+    # the committed sources are the base plus the initial boundary values
+    R = c.regions()
+    src = P["sourceBase"][0].copy()
+    for i, cell in enumerate(D["fc"]):
+        src[cell] = src[cell] + D["coeff"][i] * D["value0"][i]
+    assert np.array_equal(src, R[0]["source"])
+
+
+@pytest.mark.parametrize("acc", [dict(accType="fixed", relax=1.0), dict(accType="aitken", relax=0.75)])
+def test_converges_to_the_monolithic_solution(tmp_path_factory, monolithic, acc):
+    errs = []
+    for lim in (1e-7, 1e-12):
+        c = _read(tmp_path_factory, partitioned=dict(acc, maxJumpRes=lim, maxFluxRes=lim, maxCoupleIter=60))
+        out = ob.dna_solve(c.regions(), c.partitioned(), tol=1e-15, maxIter=3000)
+        assert out["converged"]
+        h = out["history"][-1]
+        assert h["jump"] <= lim and h["flux"] <= lim
+        errs.append(max(np.abs(a - b).max() for a, b in zip(out["psi"], monolithic)))
+    assert errs[1] < 1e-6 and errs[1] < errs[0]
+
+
+def test_aitken_beats_fixed_relaxation(tmp_path_factory):
+    its = {}
+    for kind in ("fixed", "aitken"):
+        c = _read(tmp_path_factory, partitioned=dict(accType=kind, relax=0.75))
+        out = ob.dna_solve(c.regions(), c.partitioned(), **SOLVE)
+        assert out["converged"]
+        its[kind] = out["iterations"]
+    assert its["aitken"] < its["fixed"]
+
+
+def test_iteration_limit(tmp_path_factory):
+    c = _read(tmp_path_factory, partitioned=dict(maxCoupleIter=3))
+    out = ob.dna_solve(c.regions(), c.partitioned(), **SOLVE)
+    assert (out["iterations"], out["converged"]) == (3, False)
+    assert len(out["solves"]) == 6
+
+
+def test_reader_errors(tmp_path_factory):
+    d = tmp_path_factory.mktemp("bad")
+    ff.write_fixture_case(str(d), partitioned=dict(accType="IQN-ILS", relax=0.75))
+    with pytest.raises(mrf.MrfgError, match="IQN-ILS"):
+        mrf.Case.from_foam(str(d), ddt="steadyState")
+    d = tmp_path_factory.mktemp("transient")
+    ff.write_fixture_case(str(d), partitioned=dict())
+    with pytest.raises(mrf.MrfgError, match="steady"):
+        mrf.Case.from_foam(str(d), ddt="Euler")
+
+
+# ---------------------------------------------------------------------------
+# GPU: mrfg_dna_solve against the oracle's loop
+# ---------------------------------------------------------------------------
+
+def _assert_dna_equal(A, s, case, out, dna):
+    assert dna["iterations"] == out["iterations"]
+    assert dna["converged"] == out["converged"]
+    j, f, w = A.dna_history(s)
+    assert np.array_equal(j, [h["jump"] for h in out["history"]])
+    assert np.array_equal(f, [h["flux"] for h in out["history"]])
+    assert np.array_equal(w, [h["omega"][0] for h in out["history"]])
+    assert dna["region_solves"] == len(out["solves"])
+    assert dna["solver_iterations"] == sum(p["nIterations"] for p in out["solves"])
+    for a, b in zip(A.psi(s), out["psi"]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("acc", [dict(accType="fixed", relax=1.0), dict(accType="aitken", relax=0.75)])
+def test_gpu_dna_matches_oracle(gpu, tmp_path_factory, acc):
+    c = _read(tmp_path_factory, partitioned=acc)
+    A = gpu.CoupledFvMatrix(0)
+    A.add_case(c)
+    A.commit()
+    dna = A.dna_solve(fv_solution=FV_SOLUTION, entry="T")[0]
+    out = ob.dna_solve(c.regions(), c.partitioned(), **SOLVE)
+    _assert_dna_equal(A, 0, c, out, dna)
+    # a second run starts again from the committed state
+    again = A.dna_solve(fv_solution=FV_SOLUTION, entry="T")[0]
+    assert again["iterations"] == dna["iterations"]
+    _assert_dna_equal(A, 0, c, out, again)
+    A.close()
+
+
+@pytest.mark.gpu
+def test_gpu_test_suite_partitioned_aitken(gpu, tmp_path_factory):
+    """The test suite's partitioned_Aitken cases (genCases.py: accType aitken,
+    relax 0.75) for the nine (Pr, Re, k) sets, one batch: every case runs its
+    own DNA loop, equal to the oracle's."""
+    from test_suite_counterpart import CASES, suite_params
+    cs = [_read(tmp_path_factory, partitioned=dict(accType="aitken", relax=0.75), params=suite_params(*p))
+          for p in CASES]
+    A = gpu.CoupledFvMatrix(0)
+    for c in cs:
+        A.add_case(c)
+    A.commit()
+    perfs = A.dna_solve(fv_solution=FV_SOLUTION, entry="T")
+    for s, c in enumerate(cs):
+        out = ob.dna_solve(c.regions(), c.partitioned(), **SOLVE)
+        _assert_dna_equal(A, s, c, out, perfs[s])
+    A.close()
+
+
+@pytest.mark.gpu
+def test_gpu_dna_errors(gpu, tmp_path_factory):
+    A = gpu.CoupledFvMatrix(0)
+    A.add_case(_read(tmp_path_factory))
+    A.commit()
+    with pytest.raises(gpu.MrfgError, match="not a partitioned case"):
+        A.dna_solve(fv_solution=FV_SOLUTION, entry="T")
+    with pytest.raises(gpu.MrfgError, match="no partitioned solve"):
+        A.dna_history(0)
+    A.close()
