@@ -302,6 +302,76 @@ def run_suite(m, args):
     print(json.dumps(out))
 
 
+def run_suite_partitioned(m, args):
+    """The reference test suite's partitioned_Aitken cases (genCases.py:
+    accType aitken, relax 0.75) for its nine (Pr, Re, k) sets, one batch:
+    Dirichlet-Neumann coupling of the fluid and solid of each case
+    (mrfg_dna_solve), every case on its own DNA loop (maxCoupleIter 20, jump
+    and flux residuals 1e-7), region solves BiCGStab + DILU to 1e-12.  One
+    step = the whole coupling of the nine from their initial fields."""
+    import shutil
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import foam_fixture
+    suite = [(0.01, 500, 1), (0.01, 500, 5), (0.01, 500, 20), (0.01, 10000, 1), (0.01, 10000, 5),
+             (0.01, 10000, 20), (100, 500, 1), (100, 500, 5), (100, 500, 20)]
+
+    def params(Pr, Re, k):   # genCases.py update_params
+        kf = 100.0 / k
+        return dict(kf=kf, cp=kf * Pr / (1.0 / int(Re)))
+
+    fvs = "solvers { T { solver BiCGStab; preconditioner DILU; tolerance 1e-12; relTol 0; maxIter 1000; } }"
+    ctrl = m.parse_controls(fvs, "T")
+    cs = []
+    for c in suite:
+        d = tempfile.mkdtemp(prefix="testSuite_p_")
+        try:
+            foam_fixture.write_fixture_case(d, params=params(*c), partitioned=dict(accType="aitken", relax=0.75))
+            cs.append(m.Case.from_foam(d, ddt="steadyState"))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    A = m.CoupledFvMatrix(0)
+    for c in cs:
+        A.add_case(c)
+    A.commit()
+    for _ in range(args.warmup):
+        A.dna_solve(ctrl)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        perf = A.dna_solve(ctrl)
+        lat.append(time.perf_counter() - s0)
+    total = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle_bind
+        t1 = time.perf_counter()
+        its = []
+        for c in cs:
+            out = oracle_bind.dna_solve(c.regions(), c.partitioned(), tol=1e-12, relTol=0.0, minIter=0, maxIter=1000)
+            its.append(out["iterations"])
+        tc = time.perf_counter() - t1
+        cpu = dict(value=len(cs) / tc, unit="cases/s", cores=1, kind="port",
+                   sample="the oracle's DNA loop for the nine cases on one host core (%d-%d DNA iterations), %.2f s"
+                          % (min(its), max(its), tc))
+    out = {
+        "metric": "partitioned test-suite cases/s (Dirichlet-Neumann coupling with Aitken relaxation)",
+        "value": len(cs) * args.steps / total, "unit": "cases/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "reference tutorial mesh (fixture) with the test suite's nine parameter sets, partitioned set-up",
+        "config": {"workload": "flowOverHeatedPlate_testSuite partitioned_Aitken: 9 cases, DNA maxCoupleIter 20, "
+                               "residuals 1e-7, region solves BiCGStab/DILU tol 1e-12", "systems": len(cs)},
+        "dna_iterations": [p["iterations"] for p in perf], "all_converged": all(p["converged"] for p in perf),
+        "region_solves": sum(p["region_solves"] for p in perf),
+        "solver_iterations": sum(p["solver_iterations"] for p in perf),
+        "p50_ms": 1e3 * pct(lat, 0.5), "p99_ms": 1e3 * pct(lat, 0.99),
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def run_batch(m, cases, args):
     """64 independent two-region systems of seeded uneven size (cells per
     region log-uniform in [10k, 500k], conductivity ratio log-uniform in
@@ -396,7 +466,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--workload", default="duct3d",
-                    choices=["duct3d", "exchanger3", "batch64", "serving", "tutorial", "suite"],
+                    choices=["duct3d", "exchanger3", "batch64", "serving", "tutorial", "suite", "suite_partitioned"],
                     help="duct3d (default, the headline line): the 1.404M-cell two-region GGI solve; "
                          "exchanger3: three regions of 2.4M cells, two GGI interfaces, mixed precision "
                          "with the FP64 interface-flux check; batch64: 64 independent uneven systems "
@@ -404,7 +474,8 @@ def main():
                          "through the continuous-batching server; tutorial: the reference's "
                          "flowOverHeatedPlate case read from disk, with its own Tcoupled solver entry; "
                          "suite: the reference test suite's nine parameter cases as one batch, fields "
-                         "set and assembled on the device, then solved")
+                         "set and assembled on the device, then solved; suite_partitioned: the test suite's "
+                         "partitioned_Aitken cases, Dirichlet-Neumann coupling of the nine as one batch")
     ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
@@ -430,6 +501,8 @@ def main():
         return run_tutorial(m, args)
     if args.workload == "suite":
         return run_suite(m, args)
+    if args.workload == "suite_partitioned":
+        return run_suite_partitioned(m, args)
 
     case = cases.duct3d()
     A = m.CoupledFvMatrix(0)
