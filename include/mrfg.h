@@ -274,7 +274,7 @@ int mrfg_case_region_ddt(const mrfg_case *c, int region, int *transient, double 
    term per unit boundary value (temperature or gradient; added to
    source[fc[i]] after every other boundary contribution), face conductivity,
    patch deltaCoeffs, initial patch values, the GGI gather of the other
-   side's faces, acceleration (0 fixed, 1 aitken) and relaxation factor; K
+   side's faces, acceleration (0 fixed, 1 aitken, 2 IQN-ILS) and relaxation factor; K
    is the pair's temperature jump coefficient.  source_base is a region's
    source without the partitioned faces. */
 int mrfg_case_dna_controls(const mrfg_case *c, int *n_pairs, int *max_couple_iter,

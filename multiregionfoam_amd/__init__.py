@@ -323,7 +323,7 @@ class Case:
                 relax, K = C.c_double(), C.c_double()
                 lib().mrfg_case_dna_side_sizes(self._h, q, side, C.byref(reg), C.byref(nf), C.byref(nw),
                                                C.byref(acc), C.byref(relax), C.byref(K))
-                d = dict(region=reg.value, accType=("fixed", "aitken")[acc.value], relax=relax.value,
+                d = dict(region=reg.value, accType=("fixed", "aitken", "IQN-ILS")[acc.value], relax=relax.value,
                          fc=np.zeros(nf.value, np.int32), coeff=np.zeros(nf.value), kb=np.zeros(nf.value),
                          dc=np.zeros(nf.value), value0=np.zeros(nf.value), off=np.zeros(nf.value + 1, np.int32),
                          idx=np.zeros(nw.value, np.int32), w=np.zeros(nw.value))

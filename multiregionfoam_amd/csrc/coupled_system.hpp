@@ -99,7 +99,7 @@ struct PartitionedSide {
     std::vector<double> value0;       // initial patch values
     std::vector<int32_t> off, idx;    // per face: the other side's faces it gathers (GGI)
     std::vector<double> w;
-    int accType = 0;                  // 0 fixed relaxation, 1 Aitken
+    int accType = 0;                  // 0 fixed relaxation, 1 Aitken, 2 IQN-ILS
     double relax = 1.0;
     int nFaces() const { return (int)fc.size(); }
 };

@@ -81,6 +81,108 @@ __device__ void finishRelax(const DnaSide &S, double w, double *val, double *pre
     }
 }
 
+// IQN-ILS (IQNILS.C:70-170, IQNILSTemplates.C:37-125) on one side: val
+// holds this call's field before the update, res its residual.  The second
+// correction stores the references; from the third on the residual and
+// field differences are appended; with two columns or more the update is
+// val + sum_t W_t c_(cols-1-t), c from a modified Gram-Schmidt QR of the
+// columns (newest first) and a triangular solve that drops columns whose
+// pivot is below 1e-10 times the largest column sum; else fixed relaxation.
+__device__ void iqnils(const DnaSide &S, double *val, double *prev, const double *res, double *hist, int *cols,
+                       int *corr, int sideId)
+{
+    const int c = corr[sideId] == 0 ? 1 : corr[sideId];
+    int ncol = cols[sideId];
+    __syncthreads();   // the side's state is read before thread 0 updates it
+    const int n = S.end - S.begin, mc = S.maxCols;
+    double *V = hist + S.histBase, *W = V + (size_t)mc * n, *Q = W + (size_t)mc * n;
+    double *fRef = Q + (size_t)mc * n, *rRef = fRef + n, *R = rRef + n, *Cv = R + (size_t)mc * mc;
+    if (c == 2) {
+        for (int i = S.begin + threadIdx.x; i < S.end; i += kDnaBlock) {
+            fRef[i - S.begin] = val[i];
+            rRef[i - S.begin] = res[i];
+        }
+    } else if (c > 2 && ncol < mc) {
+        for (int i = S.begin + threadIdx.x; i < S.end; i += kDnaBlock) {
+            const int k = i - S.begin;
+            V[(size_t)ncol * n + k] = res[i] - rRef[k];
+            W[(size_t)ncol * n + k] = val[i] - fRef[k];
+        }
+        ++ncol;
+    }
+    if (ncol > 1) {
+        for (int i = S.begin + threadIdx.x; i < S.end; i += kDnaBlock) {
+            const int k = i - S.begin;
+            for (int t = 0; t < ncol; ++t) Q[(size_t)t * n + k] = V[(size_t)(ncol - 1 - t) * n + k];
+        }
+        for (int a = 0; a < ncol; ++a) {
+            double *Qa = Q + (size_t)a * n;
+            DD acc[1] = {DD{0.0, 0.0}};
+            for (int k = threadIdx.x; k < n; k += kDnaBlock) acc[0] = ddAdd(acc[0], DD{Qa[k] * Qa[k], 0.0});
+            double s[1];
+            blockSum<1>(acc, s);
+            const double raa = sqrt(s[0]);
+            if (threadIdx.x == 0) R[(size_t)a * mc + a] = raa;
+            for (int k = threadIdx.x; k < n; k += kDnaBlock) Qa[k] = Qa[k] / raa;
+            for (int b = a + 1; b < ncol; ++b) {
+                double *Qb = Q + (size_t)b * n;
+                DD d[1] = {DD{0.0, 0.0}};
+                for (int k = threadIdx.x; k < n; k += kDnaBlock) d[0] = ddAdd(d[0], DD{Qa[k] * Qb[k], 0.0});
+                double r[1];
+                blockSum<1>(d, r);
+                if (threadIdx.x == 0) R[(size_t)a * mc + b] = r[0];
+                for (int k = threadIdx.x; k < n; k += kDnaBlock) Qb[k] = Qb[k] - r[0] * Qa[k];
+            }
+            DD e[1] = {DD{0.0, 0.0}};
+            for (int k = threadIdx.x; k < n; k += kDnaBlock) e[0] = ddAdd(e[0], DD{Qa[k] * (-res[S.begin + k]), 0.0});
+            double cc[1];
+            blockSum<1>(e, cc);
+            if (threadIdx.x == 0) Cv[a] = cc[0];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double eps = 0.0;
+            for (int j = 0; j < ncol; ++j) {
+                double t = 0.0;
+                for (int i = 0; i < j + 1; ++i) t += fabs(R[(size_t)i * mc + j]);
+                if (j == 0 || t > eps) eps = t;
+            }
+            eps = 1.0e-10 * eps;
+            for (int i = 0; i < ncol; ++i)
+                if (fabs(R[(size_t)i * mc + i]) > eps) {
+                    for (int j = i + 1; j < ncol; ++j) R[(size_t)i * mc + j] = R[(size_t)i * mc + j] / R[(size_t)i * mc + i];
+                    Cv[i] = Cv[i] / R[(size_t)i * mc + i];
+                    R[(size_t)i * mc + i] = 1.0;
+                }
+            for (int j = ncol - 1; j >= 0; --j) {
+                if (fabs(R[(size_t)j * mc + j]) > eps) {
+                    for (int i = 0; i < j; ++i) Cv[i] = Cv[i] - Cv[j] * R[(size_t)i * mc + j];
+                } else {
+                    Cv[j] = 0.0;
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = S.begin + threadIdx.x; i < S.end; i += kDnaBlock) {
+            const int k = i - S.begin;
+            double out = val[i];
+            for (int t = 0; t < ncol; ++t) out = out + W[(size_t)t * n + k] * Cv[ncol - 1 - t];
+            val[i] = out;
+            prev[i] = out;
+        }
+    } else {
+        for (int i = S.begin + threadIdx.x; i < S.end; i += kDnaBlock) {
+            const double out = prev[i] + S.relax * res[i];
+            val[i] = out;
+            prev[i] = out;
+        }
+    }
+    if (threadIdx.x == 0) {
+        cols[sideId] = ncol;
+        corr[sideId] = c + 1;
+    }
+}
+
 // k snGrad T of Dirichlet face i: ((T_b - T_P) deltaCoeffs) k
 __device__ inline double dirFlux(const DnaDev &m, int i)
 {
@@ -98,8 +200,13 @@ __global__ __launch_bounds__(kDnaBlock) void k_dna_dirichlet(DnaDev m, const Dna
         const double cur = nbr + nbr * (S.K - 1.0);
         m.dPrevRes[i] = m.dRes[i];
         m.dRes[i] = cur - m.dPrev[i];
+        m.dT[i] = cur;
     }
     __syncthreads();
+    if (S.accType == 2) {
+        iqnils(S, m.dT, m.dPrev, m.dRes, m.dHist, m.dCols, m.dCorr, S.id);
+        return;
+    }
     const double w = relaxFactor(S, m.dRes, m.dPrevRes, m.dOmega, m.dCorr, S.id);
     finishRelax(S, w, m.dT, m.dPrev, m.dRes, m.dOmega, m.dCorr, S.id);
 }
@@ -117,8 +224,13 @@ __global__ __launch_bounds__(kDnaBlock) void k_dna_neumann(DnaDev m, const DnaSi
         const double cur = fl / m.nKb[j];
         m.nPrevRes[j] = m.nRes[j];
         m.nRes[j] = cur - m.nPrev[j];
+        m.nG[j] = cur;
     }
     __syncthreads();
+    if (S.accType == 2) {
+        iqnils(S, m.nG, m.nPrev, m.nRes, m.nHist, m.nCols, m.nCorr, S.id);
+        return;
+    }
     const double w = relaxFactor(S, m.nRes, m.nPrevRes, m.nOmega, m.nCorr, S.id);
     finishRelax(S, w, m.nG, m.nPrev, m.nRes, m.nOmega, m.nCorr, S.id);
 }

@@ -18,12 +18,22 @@ namespace mrfg {
 
 struct DnaSide {
     int begin, end;      // face range in its side set
-    int accType;         // 0 fixed relaxation, 1 Aitken
+    int accType;         // 0 fixed relaxation, 1 Aitken, 2 IQN-ILS
     int caseId;
-    int id, pad;         // index of the side's acceleration state
+    int id;              // index of the side's acceleration state
+    int maxCols;         // IQN-ILS: history columns held
     double relax;        // initial relaxation factor
     double K;            // jump coefficient of the pair (Dirichlet sides)
+    long long histBase;  // IQN-ILS: the side's slice of the history buffer
 };
+
+// IQN-ILS history of a side with n faces and m columns, from histBase:
+// V (m x n), W (m x n), Q (m x n), the reference field and residual (n
+// each), R (m x m, row-major) and the coefficients c (m).
+inline long long dnaHistSize(int n, int m)
+{
+    return 3LL * m * n + 2LL * n + (long long)m * m + m;
+}
 
 struct DnaPair {
     int dir, neu;        // side indices
@@ -41,9 +51,10 @@ struct DnaDev {
     const int32_t *nPos, *nOff, *nIdx;
     const double *nDc, *nKb, *nW;
     double *nG, *nPrev, *nRes, *nPrevRes, *nT;
-    // per side: Aitken factor and correction counter
+    // per side: Aitken factor, correction counter, IQN-ILS columns held
     double *dOmega, *nOmega;
-    int *dCorr, *nCorr;
+    int *dCorr, *nCorr, *dCols, *nCols;
+    double *dHist, *nHist;
 };
 
 // Source of one cell with partitioned faces: b[pos] = base, then

@@ -75,10 +75,13 @@ DnaRun::DnaRun(int device, const std::vector<const CoupledSystem *> &cases, bool
             checkSide(D, N.nFaces());
             checkSide(N, D.nFaces());
             const int sd = sysOf_[c][D.region], sn = sysOf_[c][N.region];
-            DnaSide ds{(int)dPos.size(), (int)dPos.size() + D.nFaces(), D.accType, (int)c, (int)dSides_.size(), 0,
-                       D.relax, P.K};
-            DnaSide ns{(int)nPos.size(), (int)nPos.size() + N.nFaces(), N.accType, (int)c, (int)nSides_.size(), 0,
-                       N.relax, P.K};
+            const int mc = std::max(1, M.maxCoupleIter);
+            DnaSide ds{(int)dPos.size(), (int)dPos.size() + D.nFaces(), D.accType, (int)c, (int)dSides_.size(), mc,
+                       D.relax, P.K, dHistSize_};
+            DnaSide ns{(int)nPos.size(), (int)nPos.size() + N.nFaces(), N.accType, (int)c, (int)nSides_.size(), mc,
+                       N.relax, P.K, nHistSize_};
+            if (D.accType == 2) dHistSize_ += dnaHistSize(D.nFaces(), mc);
+            if (N.accType == 2) nHistSize_ += dnaHistSize(N.nFaces(), mc);
             for (int i = 0; i < D.nFaces(); ++i) {
                 dPos.push_back(B_.position(sd, D.fc[i]));
                 dDc.push_back(D.dc[i]);
@@ -156,6 +159,10 @@ DnaRun::DnaRun(int device, const std::vector<const CoupledSystem *> &cases, bool
     nOmega_.alloc(std::max<size_t>(1, nSides_.size()));
     dCorr_.alloc(std::max<size_t>(1, dSides_.size()));
     nCorr_.alloc(std::max<size_t>(1, nSides_.size()));
+    dCols_.alloc(std::max<size_t>(1, dSides_.size()));
+    nCols_.alloc(std::max<size_t>(1, nSides_.size()));
+    dHist_.alloc((size_t)std::max<long long>(1, dHistSize_));
+    nHist_.alloc((size_t)std::max<long long>(1, nHistSize_));
     caseActive_.alloc(cases.size());
     resOut_.alloc(2 * std::max<size_t>(1, pairs_.size()));
     dSidesDev_.upload(dSides_, st);
@@ -191,6 +198,7 @@ DnaDev DnaRun::dev()
     m.nDc = nDc_.p; m.nKb = nKb_.p; m.nW = nW_.p;
     m.nG = nG_.p; m.nPrev = nPrev_.p; m.nRes = nRes_.p; m.nPrevRes = nPrevRes_.p; m.nT = nT_.p;
     m.dOmega = dOmega_.p; m.nOmega = nOmega_.p; m.dCorr = dCorr_.p; m.nCorr = nCorr_.p;
+    m.dCols = dCols_.p; m.nCols = nCols_.p; m.dHist = dHist_.p; m.nHist = nHist_.p;
     return m;
 }
 
@@ -216,6 +224,8 @@ void DnaRun::reset()
     std::vector<int32_t> cd(std::max<size_t>(1, dSides_.size()), 0), cn(std::max<size_t>(1, nSides_.size()), 0);
     HIP_OK(hipMemcpyAsync(dCorr_.p, cd.data(), cd.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(nCorr_.p, cn.data(), cn.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dCols_.p, cd.data(), cd.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(nCols_.p, cn.data(), cn.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     HIP_OK(hipStreamSynchronize(st));
     for (auto &h : history_) h.clear();
 }

@@ -49,7 +49,9 @@ private:
     std::vector<std::vector<DnaCell>> cellsByRegion_;
     std::vector<std::vector<std::array<double, 3>>> history_;
     // device
-    DevBuf<int32_t> dPos_, dOff_, dIdx_, nPos_, nOff_, nIdx_, termVal_, dCorr_, nCorr_, caseActive_;
+    DevBuf<int32_t> dPos_, dOff_, dIdx_, nPos_, nOff_, nIdx_, termVal_, dCorr_, nCorr_, dCols_, nCols_, caseActive_;
+    DevBuf<double> dHist_, nHist_;     // IQN-ILS histories
+    long long dHistSize_ = 0, nHistSize_ = 0;
     DevBuf<double> dDc_, dKb_, dW_, dT_, dPrev_, dRes_, dPrevRes_, nDc_, nKb_, nW_, nG_, nPrev_, nRes_, nPrevRes_,
         nT_, dOmega_, nOmega_, termCoeff_, resOut_;
     DevBuf<DnaSide> dSidesDev_, nSidesDev_;

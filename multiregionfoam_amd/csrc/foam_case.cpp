@@ -853,9 +853,9 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
         } else if (bc.type == "regionCoupledTemperatureJump" || bc.type == "regionCoupledHeatFlux") {
             R.bcT[p] = bc.type == "regionCoupledHeatFlux" ? BcT::PartNeumann : BcT::PartDirichlet;
             if ((int)bc.value.size() != P.size) throw FoamError(fdir + "T: patch '" + P.name + "' has no value");
-            if (bc.accType != "fixed" && bc.accType != "aitken")
+            if (bc.accType != "fixed" && bc.accType != "aitken" && bc.accType != "IQN-ILS")
                 throw FoamError(fdir + "T: patch '" + P.name + "': acceleration '" + bc.accType +
-                                "' is not supported (fixed, aitken)");
+                                "' is not supported (fixed, aitken, IQN-ILS)");
             R.Tb[p] = bc.value;
             R.accType[p] = bc.accType;
             R.relax[p] = bc.relax;
@@ -1222,7 +1222,7 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
                     S.off = g.off;
                     S.idx = g.idx;
                     S.w = g.w;
-                    S.accType = rc.accType[pa] == "aitken" ? 1 : 0;
+                    S.accType = rc.accType[pa] == "aitken" ? 1 : rc.accType[pa] == "IQN-ILS" ? 2 : 0;
                     S.relax = rc.relax[pa];
                 };
                 fill(P.dir, reg[qd], pat[qd], gD);

@@ -421,14 +421,58 @@ def _gather(side, vals):
     return out
 
 
+def _iqnils_update(V, W, res, cur):
+    """IQNILS<scalar>::updateIQNILS (IQNILSTemplates.C:37-125): QR of the
+    columns of V (newest first) by modified Gram-Schmidt, Q^T (-res), the
+    triangular solve with columns below 1e-10 * max column sum dropped, and
+    cur + sum_i W_i c_(cols-1-i)."""
+    cols = len(V)
+    R = [[0.0] * cols for _ in range(cols)]
+    C = [0.0] * cols
+    Q = [np.array(V[cols - 1 - i], dtype=np.float64) for i in range(cols)]
+    for i in range(cols):
+        R[i][i] = math.sqrt(_fsum(Q[i] * Q[i]))
+        Q[i] = Q[i] / R[i][i]
+        for j in range(i + 1, cols):
+            R[i][j] = _fsum(Q[i] * Q[j])
+            Q[j] = Q[j] - R[i][j] * Q[i]
+        C[i] = _fsum(Q[i] * (-res))
+    colsum = []
+    for j in range(cols):
+        t = 0.0
+        for i in range(j + 1):
+            t += abs(R[i][j])
+        colsum.append(t)
+    eps = 1.0e-10 * max(colsum)
+    for i in range(cols):
+        if abs(R[i][i]) > eps:
+            for j in range(i + 1, cols):
+                R[i][j] = R[i][j] / R[i][i]
+            C[i] = C[i] / R[i][i]
+            R[i][i] = 1.0
+    for j in range(cols - 1, -1, -1):
+        if abs(R[j][j]) > eps:
+            for i in range(j):
+                C[i] = C[i] - C[j] * R[i][j]
+        else:
+            C[j] = 0.0
+    out = np.array(cur, dtype=np.float64)
+    for i in range(cols):
+        out = out + W[i] * C[cols - 1 - i]
+    return out
+
+
 class _Acceleration:
-    """accelerationModel / fixedRelaxation / aitkenRelaxation
+    """accelerationModel / fixedRelaxation / aitkenRelaxation / IQNILS
     (acceleration/accelerationModel/accelerationModel.C:72-86,
     fixedRelaxation.C:76-106, aitkenRelaxation.C:83-110,
-    aitkenRelaxationTemplates.C:111-168): prevFld starts at the patch's
-    initial values (initialize(*this) of both patch types); the Aitken factor
-    keeps the initial relaxation for the first two corrections of a time
-    step."""
+    aitkenRelaxationTemplates.C:111-168, IQNILS.C:70-170): prevFld starts at
+    the patch's initial values (initialize(*this) of both patch types); the
+    Aitken factor keeps the initial relaxation for the first two corrections
+    of a time step; IQN-ILS stores the second correction's field and residual
+    as references, appends residual and field differences from the third
+    on, and takes the quasi-Newton step once it holds two columns (a steady
+    run is one time step, so couplingReuse drops nothing)."""
 
     def __init__(self, kind, relax, init):
         self.kind, self.init = kind, float(relax)
@@ -437,12 +481,26 @@ class _Acceleration:
         self.res = None
         self.prev_res = None
         self.corr = 0
+        self.V, self.W = [], []
 
     def relax(self, cur):
         if self.corr == 0:
             self.corr = 1
         self.prev_res = self.res
         self.res = cur - self.prev
+        if self.kind == "IQN-ILS":
+            if self.corr == 2:
+                self.fld_ref, self.res_ref = np.array(cur), np.array(self.res)
+            elif self.corr > 2:
+                self.V.append(self.res - self.res_ref)
+                self.W.append(cur - self.fld_ref)
+            if len(self.V) > 1:
+                out = _iqnils_update(self.V, self.W, self.res, cur)
+            else:
+                out = self.prev + self.init * self.res
+            self.prev = out
+            self.corr += 1
+            return out
         if self.kind == "aitken":
             if self.corr < 3:
                 self.omega = self.init if self.omega < self.init else self.omega

@@ -11,8 +11,9 @@ mathematics fixes, on the CPU:
   tends to the monolithic solution as the residual limit tightens (error
   below 1e-6 K at 1e-12, and smaller than at 1e-7);
 * Aitken relaxation needs fewer DNA iterations than fixed relaxation with
-  the same initial factor; without residual control, or with too few
-  iterations, the loop stops at maxCoupleIter, unconverged;
+  the same initial factor, and IQN-ILS fewer than Aitken on the test suite's
+  slowly converging cases; with too few iterations the loop stops at
+  maxCoupleIter, unconverged;
 * the reader's per-face coefficients are the closed forms k |S| / |d|
   (Dirichlet) and k |S| (Neumann), with |S| the same on both sides of the
   conformal interface.
@@ -67,7 +68,8 @@ def test_reader_partitioned_setup(tmp_path_factory):
     assert np.array_equal(src, R[0]["source"])
 
 
-@pytest.mark.parametrize("acc", [dict(accType="fixed", relax=1.0), dict(accType="aitken", relax=0.75)])
+@pytest.mark.parametrize("acc", [dict(accType="fixed", relax=1.0), dict(accType="aitken", relax=0.75),
+                                 dict(accType="IQN-ILS", relax=0.75)])
 def test_converges_to_the_monolithic_solution(tmp_path_factory, monolithic, acc):
     errs = []
     for lim in (1e-7, 1e-12):
@@ -90,6 +92,20 @@ def test_aitken_beats_fixed_relaxation(tmp_path_factory):
     assert its["aitken"] < its["fixed"]
 
 
+def test_iqnils_beats_aitken(tmp_path_factory):
+    """Pr 0.01, Re 10000, k 1 of the test suite, where Aitken needs 22 DNA
+    iterations."""
+    from test_suite_counterpart import suite_params
+    its = {}
+    for kind in ("aitken", "IQN-ILS"):
+        c = _read(tmp_path_factory, partitioned=dict(accType=kind, relax=0.75, maxCoupleIter=60),
+                  params=suite_params(0.01, 10000, 1))
+        out = ob.dna_solve(c.regions(), c.partitioned(), **SOLVE)
+        assert out["converged"]
+        its[kind] = out["iterations"]
+    assert its["IQN-ILS"] < its["aitken"]
+
+
 def test_iteration_limit(tmp_path_factory):
     c = _read(tmp_path_factory, partitioned=dict(maxCoupleIter=3))
     out = ob.dna_solve(c.regions(), c.partitioned(), **SOLVE)
@@ -99,8 +115,8 @@ def test_iteration_limit(tmp_path_factory):
 
 def test_reader_errors(tmp_path_factory):
     d = tmp_path_factory.mktemp("bad")
-    ff.write_fixture_case(str(d), partitioned=dict(accType="IQN-ILS", relax=0.75))
-    with pytest.raises(mrf.MrfgError, match="IQN-ILS"):
+    ff.write_fixture_case(str(d), partitioned=dict(accType="broyden", relax=0.75))
+    with pytest.raises(mrf.MrfgError, match="broyden"):
         mrf.Case.from_foam(str(d), ddt="steadyState")
     d = tmp_path_factory.mktemp("transient")
     ff.write_fixture_case(str(d), partitioned=dict())
@@ -126,9 +142,12 @@ def _assert_dna_equal(A, s, case, out, dna):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("acc", [dict(accType="fixed", relax=1.0), dict(accType="aitken", relax=0.75)])
-def test_gpu_dna_matches_oracle(gpu, tmp_path_factory, acc):
-    c = _read(tmp_path_factory, partitioned=acc)
+@pytest.mark.parametrize("acc,params", [
+    (dict(accType="fixed", relax=1.0), None), (dict(accType="aitken", relax=0.75), None),
+    (dict(accType="IQN-ILS", relax=0.75), None),
+    (dict(accType="IQN-ILS", relax=0.75, maxCoupleIter=60), dict(kf=100.0, cp=10000.0))])
+def test_gpu_dna_matches_oracle(gpu, tmp_path_factory, acc, params):
+    c = _read(tmp_path_factory, partitioned=acc, params=params)
     A = gpu.CoupledFvMatrix(0)
     A.add_case(c)
     A.commit()
