@@ -232,12 +232,28 @@ def run_suite(m, args):
     t_commit = time.perf_counter() - t1
     B.close()
 
+    # --copies N: N systems cycling through the nine parameter sets (the
+    # suite itself is nine)
+    n_sys = max(1, args.copies)
     A = m.CoupledFvMatrix(0)
-    for _ in suite:
+    for _ in range(n_sys):
         A.add_case(base)
     A.commit()
     n_fluid = base.regions()[0]["nCells"]
-    fields = [(np.full(n_fluid, params(*c)["kf"]), params(*c)["cp"]) for c in suite]
+    fields = [(np.full(n_fluid, params(*suite[s % len(suite)])["kf"]), params(*suite[s % len(suite)])["cp"])
+              for s in range(n_sys)]
+    if args.kernel_probe:
+        # only the assembly passes, replayed, then a plain copy of known size
+        # (for rocprofv3 --pmc runs)
+        for s, (kf, cp) in enumerate(fields):
+            A.set_field(s, 0, "k", kf)
+            A.set_field(s, 0, "rhoCp", cp)
+        A.assemble()
+        for which, name in ((7, "fv_assemble"), (8, "fv_place")):
+            print("probe %s %.4f ms" % (name, A.time_kernel(which, args.kernel_reps)), file=sys.stderr, flush=True)
+        ms = A.copy_probe(256 * 2**20, args.kernel_reps)
+        print("probe copy 268435456 bytes read + written per dispatch: %.4f ms" % ms, file=sys.stderr, flush=True)
+        return
 
     def step(tim):
         t1 = time.perf_counter()
@@ -267,6 +283,18 @@ def run_suite(m, args):
     passes = {"assemble": dict(ms=ms7, bytes=nb["assemble"], GBs=nb["assemble"] / (ms7 * 1e-3) / 1e9),
               "place": dict(ms=ms8, bytes=nb["place"], GBs=nb["place"] / (ms8 * 1e-3) / 1e9)}
     dom = max(passes, key=lambda k: passes[k]["ms"])
+    # HBM bytes per pass from the committed rocprofv3 PMC summary of this
+    # workload at the same number of systems (None otherwise)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "suite_assembly_hbm_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        if tj.get("systems") == n_sys:
+            names = {"assemble": ("k_fv_faces", "k_fv_sides", "k_fv_cells"), "place": ("k_fv_place",)}
+            tk = tj.get("kernels", {})
+            if all(k in tk for k in names[dom]):
+                traffic = sum(tk[k]["total_bytes"] for k in names[dom])
     cpu = None
     if not args.no_cpu_baseline:
         import oracle_bind
@@ -283,12 +311,12 @@ def run_suite(m, args):
     mean = lambda i: 1e3 * sum(t[i] for t in tim) / len(tim)
     out = {
         "metric": "test-suite cases/s (fields set, assembled on the device, solved to steady state)",
-        "value": len(suite) * args.steps / total, "unit": "cases/s", "n_gpus": 1, "steps": args.steps,
+        "value": n_sys * args.steps / total, "unit": "cases/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "reference tutorial mesh (fixture) with the test suite's nine parameter sets",
-        "config": {"workload": "flowOverHeatedPlate_testSuite: 9 cases (Pr, Re, k) of genCases.py as one batch, "
-                               "BiCGStab/DILU tol 1e-12", "systems": len(suite),
+        "config": {"workload": "flowOverHeatedPlate_testSuite: %d cases (the nine (Pr, Re, k) sets of genCases.py, "
+                               "cycled) as one batch, BiCGStab/DILU tol 1e-12" % n_sys, "systems": n_sys,
                    "cells_per_system": info["cells"], "internal_faces_per_system": info["faces"]},
         "iterations": [p.n_iterations for p in perf], "all_converged": all(p.converged for p in perf),
         "set_fields_ms": mean(0), "assemble_ms": mean(1), "solve_ms": mean(2),
@@ -296,7 +324,7 @@ def run_suite(m, args):
         "assembly_passes": passes,
         "roofline": {"kernel": "fv_" + dom, "bound": "hbm", "achieved": passes[dom]["GBs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": passes[dom]["GBs"] / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -477,6 +505,8 @@ def main():
                          "set and assembled on the device, then solved; suite_partitioned: the test suite's "
                          "partitioned_Aitken cases, Dirichlet-Neumann coupling of the nine as one batch")
     ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
+    ap.add_argument("--copies", type=int, default=9,
+                    help="suite: systems in the batch, cycling through the nine parameter sets (default 9)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
                          "device copy of known size (for rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE "
