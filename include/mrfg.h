@@ -174,7 +174,20 @@ typedef struct {
    receives one entry per system; the coupled fields become the systems'
    fields (mrfg_get_psi). */
 int mrfg_dna_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_dna_perf *perf);
-/* Per DNA iteration of the last mrfg_dna_solve for system `sys`: the jump
+/* n_steps time steps of the committed partitioned cases (regions read with
+   a ddt scheme), each a DNA loop as in mrfg_dna_solve: the ddt terms are
+   rebuilt on the device from the current and previous fields (Euler, or
+   "backward" with a first Euler step, as mrfg_time_steps), the interface
+   cells take the step's source plus their partitioned terms, and the
+   acceleration models start a new time value (correction count reset;
+   IQN-ILS keeps columns within the step, couplingReuse 0) while keeping
+   their previous field and Aitken factor (accelerationModel.C,
+   aitkenRelaxation.C:83-110, IQNILS.C:70-140).  Runs from the committed
+   initial state; perf receives n_steps x n_systems entries (step-major). */
+int mrfg_dna_time_steps(mrfg_ctx *ctx, const mrfg_controls *controls, const char *ddt_scheme, int n_steps,
+                        mrfg_dna_perf *perf);
+/* Per DNA iteration of the last mrfg_dna_solve (or of the last step of
+   mrfg_dna_time_steps) for system `sys`: the jump
    and flux residuals and the relaxation factor of its first Dirichlet side
    (at most max_iter values each; *n receives the number of iterations). */
 int mrfg_dna_history(mrfg_ctx *ctx, int sys, int max_iter, double *jump, double *flux, double *omega, int *n);

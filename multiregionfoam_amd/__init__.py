@@ -91,6 +91,7 @@ def lib():
         "mrfg_time_steps": ([vp, C.POINTER(Controls), C.c_char_p, C.c_int, C.POINTER(Perf)], C.c_int),
         "mrfg_dna_solve": ([vp, C.POINTER(Controls), C.POINTER(DnaPerf)], C.c_int),
         "mrfg_dna_history": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, C.POINTER(C.c_int)], C.c_int),
+        "mrfg_dna_time_steps": ([vp, C.POINTER(Controls), C.c_char_p, C.c_int, C.POINTER(DnaPerf)], C.c_int),
         "mrfg_parse_controls": ([C.c_char_p, C.c_char_p, C.POINTER(Controls), C.c_char_p, C.c_int], C.c_int),
         "mrfg_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_reset_psi": ([vp], C.c_int),
@@ -146,7 +147,7 @@ def lib():
 EXPORTED = (
     "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region mrfg_set_region_ddt "
     "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_time_steps mrfg_parse_controls "
-    "mrfg_dna_solve mrfg_dna_history "
+    "mrfg_dna_solve mrfg_dna_history mrfg_dna_time_steps "
     "mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_read_foam mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes "
     "mrfg_case_region mrfg_case_region_ddt mrfg_case_dna_controls mrfg_case_dna_side_sizes mrfg_case_dna_side "
@@ -525,6 +526,21 @@ class CoupledFvMatrix:
         return [dict(iterations=p.iterations, converged=bool(p.converged), region_solves=p.region_solves,
                      solver_iterations=p.solver_iterations, jump=p.jump_residual, flux=p.flux_residual,
                      seconds=p.seconds) for p in perf[:self.n_systems]]
+
+    def dna_time_steps(self, n_steps, ddt="backward", ctrl=None, fv_solution=None, entry="T", **kw):
+        """``n_steps`` time steps of the committed partitioned cases, each a
+        DNA loop; returns, per step, one dict per system."""
+        if fv_solution is not None:
+            ctrl = parse_controls(fv_solution, entry)
+        elif ctrl is None:
+            ctrl = controls(**kw)
+        n = max(1, self.n_systems)
+        perf = (DnaPerf * (n * n_steps))()
+        self._check(lib().mrfg_dna_time_steps(self._h, C.byref(ctrl), _text(ddt), int(n_steps), perf))
+        conv = lambda p: dict(iterations=p.iterations, converged=bool(p.converged), region_solves=p.region_solves,
+                              solver_iterations=p.solver_iterations, jump=p.jump_residual, flux=p.flux_residual,
+                              seconds=p.seconds)
+        return [[conv(p) for p in perf[k * n:(k + 1) * n]] for k in range(n_steps)]
 
     def dna_history(self, sys=0, max_iter=1000):
         """Per DNA iteration of the last dna_solve: (jump, flux, omega) arrays."""

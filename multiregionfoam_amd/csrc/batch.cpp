@@ -837,15 +837,22 @@ void Batch::timeSteps(const SolverControls &c, bool backward, int nSteps, std::v
     if (nSteps < 1) throw std::runtime_error("time stepping needs nSteps >= 1");
     perf.clear();
     for (int n = 0; n < nSteps; ++n) {
-        const bool bwd = backward && stepsDone_ >= 1;
-        setDdtDiag(bwd ? 1.5 : 1.0);
-        HIP_OK(launchDdtSource(blocks_.p, nBlk_, bwd, srcBase_.p, ddtA_.p, x64_.p, xPrev_.p, b64_.p, stream_));
-        HIP_OK(hipMemcpyAsync(xPrev_.p, x64_.p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        beginTimeStep(backward);
         std::vector<SolvePerf> p;
         solve(c, p);
         perf.insert(perf.end(), p.begin(), p.end());
-        ++stepsDone_;
     }
+}
+
+void Batch::beginTimeStep(bool backward)
+{
+    HIP_OK(hipSetDevice(device_));
+    if (!transient_) throw std::runtime_error("time stepping needs a transient system (ddt coefficients)");
+    const bool bwd = backward && stepsDone_ >= 1;
+    setDdtDiag(bwd ? 1.5 : 1.0);
+    HIP_OK(launchDdtSource(blocks_.p, nBlk_, bwd, srcBase_.p, ddtA_.p, x64_.p, xPrev_.p, b64_.p, stream_));
+    HIP_OK(hipMemcpyAsync(xPrev_.p, x64_.p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    ++stepsDone_;
 }
 
 void Batch::setPsi(int sys, int region, const double *psi)

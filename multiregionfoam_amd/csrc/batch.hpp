@@ -115,6 +115,10 @@ public:
     // perf gets nSteps x nSystems entries.  FP64 only.
     void timeSteps(const SolverControls &c, bool backward, int nSteps, std::vector<SolvePerf> &perf);
     bool transient() const { return transient_; }
+    // The ddt terms of the next time step, rebuilt on the device from the
+    // current and previous fields (diagonal, source b, previous field); the
+    // systems are then solved by the caller.
+    void beginTimeStep(bool backward);
 
     // Reset every system's field to the committed initial guess (device
     // copy); a following time-step run starts anew.
@@ -149,6 +153,7 @@ public:
     // global cell `cell` of system `sys`
     const double *fieldDevice() const { return x64_.p; }
     double *rhsDevice() { return b64_.p; }
+    size_t nPositions() const { return (size_t)nPos_; }
     int position(int sys, int cell) const { return posBegin(sys) + lay_.at(sys)->posOfCell.at(cell); }
     hipStream_t stream() const { return stream_; }
     // Canonical algorithmic bytes of one iteration / one SpMV / one DILU

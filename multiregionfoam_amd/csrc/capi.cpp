@@ -334,6 +334,43 @@ int mrfg_dna_solve(mrfg_ctx *ctx, const mrfg_controls *controls, mrfg_dna_perf *
     });
 }
 
+int mrfg_dna_time_steps(mrfg_ctx *ctx, const mrfg_controls *controls, const char *ddt_scheme, int n_steps,
+                        mrfg_dna_perf *perf)
+{
+    if (!ctx || !controls || !ddt_scheme || !perf || n_steps < 1) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        const std::string sch = ddt_scheme;
+        if (sch != "Euler" && sch != "backward") throw ArgError("ddt scheme must be Euler or backward");
+        mrfg::Batch &B = committedBatch(ctx);
+        std::vector<const mrfg::CoupledSystem *> cases;
+        for (auto &s : ctx->systems) {
+            if (!s->part) throw ArgError("system " + std::to_string(cases.size()) + " is not a partitioned case");
+            cases.push_back(s.get());
+        }
+        if (!ctx->dna) ctx->dna.reset(new mrfg::DnaRun(ctx->device, cases, ctx->preferLevels));
+        std::vector<mrfg::DnaPerf> out;
+        ctx->dna->timeSteps(fromControls(controls), sch == "backward", n_steps, out);
+        for (size_t q = 0; q < out.size(); ++q) {
+            mrfg_dna_perf &p = perf[q];
+            p.iterations = out[q].iterations;
+            p.converged = out[q].converged;
+            p.region_solves = out[q].regionSolves;
+            p.solver_iterations = out[q].solverIterations;
+            p.jump_residual = out[q].jumpResidual;
+            p.flux_residual = out[q].fluxResidual;
+            p.seconds = out[q].seconds;
+        }
+        for (size_t q = 0; q < ctx->systems.size(); ++q) {
+            const mrfg::CoupledSystem &S = *ctx->systems[q];
+            for (size_t r = 0; r < S.regions.size(); ++r) {
+                std::vector<double> psi(S.regions[r].nCells);
+                ctx->dna->getPsi((int)q, (int)r, psi.data());
+                B.setPsi((int)q, (int)r, psi.data());
+            }
+        }
+    });
+}
+
 int mrfg_dna_history(mrfg_ctx *ctx, int sys, int max_iter, double *jump, double *flux, double *omega, int *n)
 {
     if (!ctx || !n || max_iter < 0) return MRFG_ERR_ARG;

@@ -101,6 +101,7 @@ struct PartitionedSide {
     std::vector<double> w;
     int accType = 0;                  // 0 fixed relaxation, 1 Aitken, 2 IQN-ILS
     double relax = 1.0;
+    int reuse = 0;                    // IQN-ILS couplingReuse (time steps of columns kept)
     int nFaces() const { return (int)fc.size(); }
 };
 

@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kDnaBlock) void k_dna_source(DnaDev m, const DnaCel
     if (c >= nCells) return;
     const DnaCell C = cells[c];
     if (!m.caseActive[C.caseId]) return;
-    double s = C.base;
+    double s = m.base ? m.base[C.pos] : C.base;
     for (int t = C.tBegin; t < C.tEnd; ++t) {
         const int v = val[t];
         s = s + coeff[t] * (v >= 0 ? m.dT[v] : m.nG[-v - 1]);

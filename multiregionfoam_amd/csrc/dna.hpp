@@ -42,6 +42,7 @@ struct DnaPair {
 
 struct DnaDev {
     const double *x;                               // batch field (positions)
+    const double *base;                            // per-position source base (time steps), or null
     const int *caseActive;
     // Dirichlet faces: cell position, deltaCoeffs, face k, GGI gather of Neumann faces
     const int32_t *dPos, *dOff, *dIdx;
@@ -57,7 +58,8 @@ struct DnaDev {
     double *dHist, *nHist;
 };
 
-// Source of one cell with partitioned faces: b[pos] = base, then
+// Source of one cell with partitioned faces: b[pos] = base (DnaDev::base[pos]
+// when set, the step's sources with their ddt terms), then
 // b += coeff * value for its terms in order (value: Dirichlet face v >= 0,
 // Neumann face -v-1).
 struct DnaCell {

@@ -190,6 +190,7 @@ DnaDev DnaRun::dev()
 {
     DnaDev m;
     m.x = B_.fieldDevice();
+    m.base = nullptr;
     m.caseActive = caseActive_.p;
     m.dPos = dPos_.p; m.dOff = dOff_.p; m.dIdx = dIdx_.p;
     m.dDc = dDc_.p; m.dKb = dKb_.p; m.dW = dW_.p;
@@ -234,12 +235,48 @@ void DnaRun::run(const SolverControls &c, std::vector<DnaPerf> &out)
 {
     if (c.precision != SolverControls::FP64) throw std::runtime_error("partitioned coupling runs in precision fp64");
     reset();
+    loop(c, out, false);
+}
+
+void DnaRun::timeSteps(const SolverControls &c, bool backward, int nSteps, std::vector<DnaPerf> &out)
+{
+    if (c.precision != SolverControls::FP64) throw std::runtime_error("partitioned coupling runs in precision fp64");
+    if (!B_.transient()) throw std::runtime_error("time stepping needs a transient partitioned case (ddt Euler or backward)");
+    if (nSteps < 1) throw std::runtime_error("time stepping needs nSteps >= 1");
+    reset();
+    hipStream_t st = B_.stream();
+    out.clear();
+    const size_t nPosB = B_.nPositions();
+    bBase_.alloc(nPosB);
+    std::vector<int32_t> zd(std::max<size_t>(1, dSides_.size()), 0), zn(std::max<size_t>(1, nSides_.size()), 0);
+    for (int n = 0; n < nSteps; ++n) {
+        // the ddt terms of this step; the sources without the partitioned
+        // terms are kept for the interface cells
+        B_.beginTimeStep(backward);
+        HIP_OK(hipMemcpyAsync(bBase_.p, B_.rhsDevice(), nPosB * sizeof(double), hipMemcpyDeviceToDevice, st));
+        // a new time value: the acceleration models restart their correction
+        // count (IQN-ILS with couplingReuse 0 drops its columns)
+        HIP_OK(hipMemcpyAsync(dCorr_.p, zd.data(), zd.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(nCorr_.p, zn.data(), zn.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(dCols_.p, zd.data(), zd.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(nCols_.p, zn.data(), zn.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        HIP_OK(hipStreamSynchronize(st));
+        std::vector<DnaPerf> step;
+        loop(c, step, true);
+        out.insert(out.end(), step.begin(), step.end());
+    }
+}
+
+void DnaRun::loop(const SolverControls &c, std::vector<DnaPerf> &out, bool stepBase)
+{
     hipStream_t st = B_.stream();
     const int nC = (int)cases_.size();
     out.assign(nC, DnaPerf{});
+    for (auto &h : history_) h.clear();
     std::vector<int> corr(nC, 0), active(nC, 1);
     std::vector<double> jump(nC, 0.0), flux(nC, 0.0);
-    const DnaDev m = dev();
+    DnaDev m = dev();
+    m.base = stepBase ? bBase_.p : nullptr;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<double> res(2 * std::max<size_t>(1, pairs_.size()));
     while (true) {

@@ -26,6 +26,10 @@ public:
     DnaRun(int device, const std::vector<const CoupledSystem *> &cases, bool preferLevels);
     // Run every case's DNA loop from the initial state; one DnaPerf per case.
     void run(const SolverControls &c, std::vector<DnaPerf> &out);
+    // nSteps time steps from the initial state (Euler, or backward with a
+    // first Euler step), each a DNA loop per case; out gets nSteps x nCases
+    // entries (step-major).
+    void timeSteps(const SolverControls &c, bool backward, int nSteps, std::vector<DnaPerf> &out);
     void getPsi(int caseId, int region, double *psi);
     // per DNA iteration of a case: jump residual, flux residual, relaxation
     // factor of the case's first Dirichlet side
@@ -34,6 +38,7 @@ public:
 
 private:
     void reset();
+    void loop(const SolverControls &c, std::vector<DnaPerf> &out, bool stepBase);
 
     Batch B_;
     std::vector<const CoupledSystem *> cases_;
@@ -51,6 +56,7 @@ private:
     // device
     DevBuf<int32_t> dPos_, dOff_, dIdx_, nPos_, nOff_, nIdx_, termVal_, dCorr_, nCorr_, dCols_, nCols_, caseActive_;
     DevBuf<double> dHist_, nHist_;     // IQN-ILS histories
+    DevBuf<double> bBase_;             // per step: sources with the ddt terms, without the partitioned terms
     long long dHistSize_ = 0, nHistSize_ = 0;
     DevBuf<double> dDc_, dKb_, dW_, dT_, dPrev_, dRes_, dPrevRes_, nDc_, nKb_, nW_, nG_, nPrev_, nRes_, nPrevRes_,
         nT_, dOmega_, nOmega_, termCoeff_, resOut_;

@@ -363,6 +363,7 @@ struct PatchBC {
     std::vector<double> value;   // per face (scalar) or 3 per face (vector)
     std::string accType = "fixed";   // partitioned coupling: acceleration model
     double relax = 1.0;
+    int reuse = 0;                   // IQN-ILS couplingReuse
 };
 
 struct FieldFile {
@@ -428,6 +429,7 @@ FieldFile readField(const std::string &path, const PolyMesh &m, int nComp)
             bc.value = fieldValues(d, pd->lookup("value").tokens, p.size, nComp, path + " patch " + p.name);
         bc.accType = pd->wordOrDefault("accType", "fixed");
         bc.relax = pd->numberOrDefault("relax", 1.0);
+        bc.reuse = (int)pd->numberOrDefault("couplingReuse", 0);
         F.patches[p.name] = bc;
     }
     return F;
@@ -464,6 +466,7 @@ struct FoamRegion {
     std::vector<int8_t> bcU, kFixed;    // per patch: FvUBc, k given on the patch
     std::vector<std::string> accType;   // per patch (partitioned sides)
     std::vector<double> relax;
+    std::vector<int> reuse;
     std::vector<std::vector<double>> Ub;       // per patch: fixed U, 3 per face
     // coupled patches: face conductivity and delta coefficients, and the
     // boundary coefficients handed to the interface
@@ -837,6 +840,7 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
     R.kb.resize(nP);
     R.accType.assign(nP, "fixed");
     R.relax.assign(nP, 1.0);
+    R.reuse.assign(nP, 0);
     for (size_t p = 0; p < nP; ++p) {
         const Patch &P = m.patches[p];
         const PatchBC &bc = T.patches.at(P.name);
@@ -859,6 +863,7 @@ void readRegion(const std::string &dir, const std::string &name, const std::stri
             R.Tb[p] = bc.value;
             R.accType[p] = bc.accType;
             R.relax[p] = bc.relax;
+            R.reuse[p] = bc.reuse;
         } else {
             throw FoamError(fdir + "T: patch '" + P.name + "': unsupported type '" + bc.type + "'");
         }
@@ -1224,6 +1229,7 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
                     S.w = g.w;
                     S.accType = rc.accType[pa] == "aitken" ? 1 : rc.accType[pa] == "IQN-ILS" ? 2 : 0;
                     S.relax = rc.relax[pa];
+                    S.reuse = rc.reuse[pa];
                 };
                 fill(P.dir, reg[qd], pat[qd], gD);
                 fill(P.neu, reg[qn], pat[qn], gN);
@@ -1333,8 +1339,14 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
                 for (int i = 0; i < ps->nFaces(); ++i)
                     L.source[ps->fc[i]] += ps->coeff[i] * (ps == &pp.dir ? ps->value0[i] : 0.0);
             }
-        for (auto &L : sys.regions)
-            if (L.transient()) throw FoamError("partitioned coupling is supported for steady runs (ddt steadyState)");
+        // a transient run keeps IQN-ILS columns within one time step only
+        bool transient = false;
+        for (auto &L : sys.regions) transient = transient || L.transient();
+        for (auto &pp : partModel->pairs)
+            for (const PartitionedSide *ps : {&pp.dir, &pp.neu})
+                if (transient && ps->accType == 2 && ps->reuse > 0)
+                    throw FoamError("IQN-ILS with couplingReuse > 0 is not supported in transient runs "
+                                    "(couplingReuse 0 keeps the columns of the current time step)");
         sys.part = partModel;
         sys.fv = nullptr;   // device re-assembly covers monolithic set-ups
     } else {

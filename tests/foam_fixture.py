@@ -164,7 +164,7 @@ def write_fixture_case(dest, params=None, ddt=None, partitioned=None):
                 type="regionCoupledTemperatureJump", value=meta["fields"][ra]["T"]["internal"],
                 interfaceType="heatTransferInterface", k='"k"', accType=pt["accType"], relax=_num(pt["relax"]),
                 neighbourRegionName=rb, neighbourPatchName=pb, neighbourFieldName="T", secondOrder="false",
-                nonOrthCorr="true")
+                nonOrthCorr="true", **({"couplingReuse": int(pt["couplingReuse"])} if "couplingReuse" in pt else {}))
             meta["fields"][rb]["T"]["patches"][pb] = dict(
                 type="regionCoupledHeatFlux", value=meta["fields"][rb]["T"]["internal"],
                 interfaceType="heatTransferInterface", k='"k"', neighbourRegionName=ra, neighbourPatchName=pa,

@@ -543,6 +543,19 @@ def dna_solve(case_regions, part, **solve_kw):
     the limits, or after maxCoupleIter iterations.  Region sources are
     sourceBase + coeff * value per partitioned face, in pair order, Dirichlet
     side first.  Returns dict(psi, iterations, converged, history, solves)."""
+    return dna_time_steps(case_regions, part, None, 1, **solve_kw)[0]
+
+
+def dna_time_steps(case_regions, part, ddt, n_steps, scheme="backward", **solve_kw):
+    """n_steps time steps, each a DNA loop as in dna_solve, for regions with
+    ddt terms (ddt[r] = dict(ddtCoeff, diagBase, sourceBase), None for a
+    steady region; ddt None: one steady loop).  Per step the ddt terms are
+    rebuilt from the current and previous fields as in OracleSystem.time_steps
+    (Euler, or backward with a first Euler step) and the region sources become
+    sourceBase + a * t, then the partitioned terms.  The acceleration models
+    start a new time step (accelerationModel corr_ reset on a new time value;
+    IQN-ILS with couplingReuse 0 drops its columns) and keep their previous
+    field and Aitken factor.  Returns one dna_solve result per step."""
     nreg = len(case_regions)
     systems = [OracleSystem.from_arrays([R], []) for R in case_regions]
     x = [np.array(R["psi"], dtype=np.float64) for R in case_regions]
@@ -570,44 +583,66 @@ def dna_solve(case_regions, part, **solve_kw):
             fr = max(fr, _norm_res((fown + fnbr) - 0.0, fown, -1.0 * fnbr + 0.0))
         return jr, fr
 
-    control = part["maxJumpRes"] >= 0.0
-    history, solves, corr, converged = [], [], 0, False
-    res = (None, None)
-    while True:
-        corr += 1
-        if corr > 1 and control and res[0] <= part["maxJumpRes"] and res[1] <= part["maxFluxRes"]:
-            converged = True
-            break
-        if corr > part["maxCoupleIter"]:
-            break
-        for r in range(nreg):
-            for q, P in enumerate(pairs):
-                if P["dir"]["region"] == r:
-                    nbr = _gather(P["dir"], st[q]["TN"])
-                    cur = nbr + nbr * (P["K"] - 1.0)
-                    st[q]["TD"] = st[q]["accD"].relax(cur)
-                if P["neu"]["region"] == r:
-                    fl = _gather(P["neu"], flux_dir(q))
-                    fl = fl * -1.0
-                    fl = fl + 0.0
-                    st[q]["g"] = st[q]["accN"].relax(fl / P["neu"]["kb"])
-            src = np.array(part["sourceBase"][r], dtype=np.float64)
-            for q, P in enumerate(pairs):
-                for key, vals in (("dir", st[q]["TD"]), ("neu", st[q]["g"])):
-                    S = P[key]
-                    if S["region"] != r:
-                        continue
-                    for i, c in enumerate(S["fc"]):
-                        src[c] = src[c] + S["coeff"][i] * vals[i]
-            systems[r].set_values(0, source=src)
-            psi, perf = systems[r].solve(psi0=[x[r]], **solve_kw)
-            x[r] = psi[0]
-            solves.append(perf)
-            for q, P in enumerate(pairs):
-                N = P["neu"]
-                if N["region"] == r:
-                    st[q]["TN"] = x[r][N["fc"]] + st[q]["g"] / N["dc"]
-        res = residuals()
-        history.append(dict(jump=res[0], flux=res[1], omega=[s["accD"].omega for s in st]))
-    return dict(psi=x, iterations=corr - 1, converged=converged, history=history, solves=solves,
-                boundary=[dict(TD=s["TD"], g=s["g"], TN=s["TN"]) for s in st])
+    def loop(base):
+        control = part["maxJumpRes"] >= 0.0
+        history, solves, corr, converged = [], [], 0, False
+        res = (None, None)
+        while True:
+            corr += 1
+            if corr > 1 and control and res[0] <= part["maxJumpRes"] and res[1] <= part["maxFluxRes"]:
+                converged = True
+                break
+            if corr > part["maxCoupleIter"]:
+                break
+            for r in range(nreg):
+                for q, P in enumerate(pairs):
+                    if P["dir"]["region"] == r:
+                        nbr = _gather(P["dir"], st[q]["TN"])
+                        cur = nbr + nbr * (P["K"] - 1.0)
+                        st[q]["TD"] = st[q]["accD"].relax(cur)
+                    if P["neu"]["region"] == r:
+                        fl = _gather(P["neu"], flux_dir(q))
+                        fl = fl * -1.0
+                        fl = fl + 0.0
+                        st[q]["g"] = st[q]["accN"].relax(fl / P["neu"]["kb"])
+                src = np.array(base[r], dtype=np.float64)
+                for q, P in enumerate(pairs):
+                    for key, vals in (("dir", st[q]["TD"]), ("neu", st[q]["g"])):
+                        S = P[key]
+                        if S["region"] != r:
+                            continue
+                        for i, c in enumerate(S["fc"]):
+                            src[c] = src[c] + S["coeff"][i] * vals[i]
+                systems[r].set_values(0, source=src)
+                psi, perf = systems[r].solve(psi0=[x[r]], **solve_kw)
+                x[r] = psi[0]
+                solves.append(perf)
+                for q, P in enumerate(pairs):
+                    N = P["neu"]
+                    if N["region"] == r:
+                        st[q]["TN"] = x[r][N["fc"]] + st[q]["g"] / N["dc"]
+            res = residuals()
+            history.append(dict(jump=res[0], flux=res[1], omega=[s_["accD"].omega for s_ in st]))
+        return dict(psi=[a.copy() for a in x], iterations=corr - 1, converged=converged, history=history,
+                    solves=solves, boundary=[dict(TD=s_["TD"], g=s_["g"], TN=s_["TN"]) for s_ in st])
+
+    out = []
+    prev = None
+    for k in range(n_steps):
+        base = [np.array(b, dtype=np.float64) for b in part["sourceBase"]]
+        if ddt is not None:
+            bwd = scheme == "backward" and k >= 1
+            for r, D in enumerate(ddt):
+                if D is None:
+                    continue
+                a = D["ddtCoeff"]
+                t = 2.0 * x[r] - 0.5 * prev[r] if bwd else x[r]
+                systems[r].set_values(0, diag=D["diagBase"] + (1.5 if bwd else 1.0) * a)
+                base[r] = D["sourceBase"] + a * t
+            prev = [a_.copy() for a_ in x]
+            for s_ in st:
+                for acc in (s_["accD"], s_["accN"]):
+                    acc.corr = 0
+                    acc.V, acc.W = [], []
+        out.append(loop(base))
+    return out

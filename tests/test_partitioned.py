@@ -16,9 +16,13 @@ mathematics fixes, on the CPU:
   maxCoupleIter, unconverged;
 * the reader's per-face coefficients are the closed forms k |S| / |d|
   (Dirichlet) and k |S| (Neumann), with |S| the same on both sides of the
-  conformal interface.
-On the GPU, mrfg_dna_solve equals the oracle's loop bit for bit: DNA
-iterations, the residual and relaxation-factor history, and the fields.
+  conformal interface;
+* time steps (Euler, backward): each step's converged partitioned field
+  tends to the monolithic time step of the same discretisation as the
+  residual limit tightens.
+On the GPU, mrfg_dna_solve and mrfg_dna_time_steps equal the oracle's loop
+bit for bit: DNA iterations, the residual and relaxation-factor history, and
+the fields.
 """
 import numpy as np
 import pytest
@@ -31,10 +35,15 @@ SOLVE = dict(tol=1e-12, relTol=0.0, minIter=0, maxIter=1000)
 FV_SOLUTION = "solvers { T { solver BiCGStab; preconditioner DILU; tolerance 1e-12; relTol 0; maxIter 1000; } }"
 
 
-def _read(tmp_path_factory, partitioned=None, params=None):
+def _read(tmp_path_factory, partitioned=None, params=None, ddt="steadyState"):
     d = tmp_path_factory.mktemp("case")
     ff.write_fixture_case(str(d), params=params, partitioned=partitioned)
-    return mrf.Case.from_foam(str(d), ddt="steadyState")
+    return mrf.Case.from_foam(str(d), ddt=ddt)
+
+
+def _ddt(case):
+    return [({k: R[k] for k in ("ddtCoeff", "diagBase", "sourceBase")} if "ddtCoeff" in R else None)
+            for R in case.regions()]
 
 
 @pytest.fixture(scope="module")
@@ -113,15 +122,31 @@ def test_iteration_limit(tmp_path_factory):
     assert len(out["solves"]) == 6
 
 
+@pytest.mark.parametrize("scheme", ["Euler", "backward"])
+def test_time_steps_converge_to_the_monolithic_steps(tmp_path_factory, scheme):
+    cm = _read(tmp_path_factory, ddt=scheme)
+    mono = ob.OracleSystem.from_arrays(cm.regions(), cm.interfaces()).time_steps(
+        _ddt(cm), 3, scheme=scheme, tol=1e-15, maxIter=3000)
+    worst = []
+    for lim in (1e-7, 1e-12):
+        c = _read(tmp_path_factory, ddt=scheme, partitioned=dict(accType="aitken", relax=0.75, maxJumpRes=lim,
+                                                                 maxFluxRes=lim, maxCoupleIter=60))
+        out = ob.dna_time_steps(c.regions(), c.partitioned(), _ddt(c), 3, scheme=scheme, tol=1e-15, maxIter=3000)
+        assert all(o["converged"] for o in out)
+        worst.append(max(max(np.abs(a - b).max() for a, b in zip(o["psi"], m[0])) for o, m in zip(out, mono)))
+    assert worst[1] < 1e-7 and worst[1] < worst[0]
+
+
 def test_reader_errors(tmp_path_factory):
     d = tmp_path_factory.mktemp("bad")
     ff.write_fixture_case(str(d), partitioned=dict(accType="broyden", relax=0.75))
     with pytest.raises(mrf.MrfgError, match="broyden"):
         mrf.Case.from_foam(str(d), ddt="steadyState")
-    d = tmp_path_factory.mktemp("transient")
-    ff.write_fixture_case(str(d), partitioned=dict())
-    with pytest.raises(mrf.MrfgError, match="steady"):
-        mrf.Case.from_foam(str(d), ddt="Euler")
+    d = tmp_path_factory.mktemp("reuse")
+    ff.write_fixture_case(str(d), partitioned=dict(accType="IQN-ILS", relax=0.75, couplingReuse=2))
+    with pytest.raises(mrf.MrfgError, match="couplingReuse"):
+        mrf.Case.from_foam(str(d), ddt="backward")
+    mrf.Case.from_foam(str(d), ddt="steadyState")   # one time step: no columns to reuse
 
 
 # ---------------------------------------------------------------------------
@@ -158,6 +183,26 @@ def test_gpu_dna_matches_oracle(gpu, tmp_path_factory, acc, params):
     again = A.dna_solve(fv_solution=FV_SOLUTION, entry="T")[0]
     assert again["iterations"] == dna["iterations"]
     _assert_dna_equal(A, 0, c, out, again)
+    A.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme,acc", [("Euler", dict(accType="aitken", relax=0.75)),
+                                        ("backward", dict(accType="IQN-ILS", relax=0.75))])
+def test_gpu_dna_time_steps_match_oracle(gpu, tmp_path_factory, scheme, acc):
+    c = _read(tmp_path_factory, partitioned=acc, ddt=scheme)
+    A = gpu.CoupledFvMatrix(0)
+    A.add_case(c)
+    A.commit()
+    steps = A.dna_time_steps(3, ddt=scheme, fv_solution=FV_SOLUTION, entry="T")
+    out = ob.dna_time_steps(c.regions(), c.partitioned(), _ddt(c), 3, scheme=scheme, **SOLVE)
+    for n, (st, o) in enumerate(zip(steps, out)):
+        d = st[0]
+        assert (d["iterations"], d["converged"]) == (o["iterations"], o["converged"]), n
+        assert d["jump"] == o["history"][-1]["jump"] and d["flux"] == o["history"][-1]["flux"], n
+        assert d["region_solves"] == len(o["solves"]), n
+        assert d["solver_iterations"] == sum(p["nIterations"] for p in o["solves"]), n
+    _assert_dna_equal(A, 0, c, out[-1], steps[-1][0])
     A.close()
 
 
