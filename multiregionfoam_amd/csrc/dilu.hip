@@ -270,6 +270,7 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
 // consumer tile lag its producer by that many more steps.
 constexpr int kBoxAhead = 4;     // steps between a record load and its use
 constexpr int kBoxInAhead = 3;   // steps between an inbox load and its use (<= kBoxAhead)
+constexpr int kBoxRounds = 4;    // prefetch-ring rounds per unrolled group of steps (measured best)
 
 // KT: the batch's record width K when it is 3 (records of two FP64 / one
 // FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
@@ -357,12 +358,15 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     T yprev = T(0);
     // the loop runs whole groups of steps: a step past the end is invalid for
     // every lane, reads clamped addresses and stores nothing, so the only
-    // conditional code is the wave-uniform slow path and the stores (the body
-    // is unrolled over 4 rounds of the prefetch ring: the compiler then keeps
-    // the prefetches in flight across the loop's back edge)
-    for (int i0 = 0; i0 < nSteps; i0 += 4 * kBoxAhead) {
+    // conditional code is the wave-uniform slow path and the stores.  The
+    // body is unrolled over kBoxRounds rounds of the prefetch ring: within a
+    // group every record wait is exact, while the first step of a group waits
+    // for all loads in flight (the compiler does not carry the prefetch counts
+    // across the back edge).  Longer groups drain less often but run a longer
+    // body; 16 steps measured fastest (8 and 32 were slower)
+    for (int i0 = 0; i0 < nSteps; i0 += kBoxRounds * kBoxAhead) {
 #pragma unroll
-        for (int uu = 0; uu < 4 * kBoxAhead; ++uu) {
+        for (int uu = 0; uu < kBoxRounds * kBoxAhead; ++uu) {
             const int u = uu % kBoxAhead;
             const int i = i0 + uu;
             const bool inRange = i < nSteps;
