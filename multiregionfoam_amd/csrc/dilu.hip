@@ -146,16 +146,17 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
 
     if (active && nSteps > 0) {
         Step buf[kPrefetch];
+        // left to the scheduler, which reorders these loads so the loop's
+        // first use waits for all of them; issuing them in consumption order
+        // (as the box kernel does) made the waits exact here but the
+        // tutorial's solve 1.3 % slower
 #pragma unroll
         for (int u = 0; u < kPrefetch; ++u) load(u, buf[u]);
         // whole groups of steps: a step past the end reads the clamped last
         // slice and stores nothing (its LDS ring slot is never read), so the
         // group has no early exits and the record waits stay exact within it;
-        // the first step of a group waits for every load in flight (the
-        // compiler does not carry the prefetch counts across the back edge),
-        // so the group spans kWaveRounds rounds of the prefetch ring (16
-        // steps: 9 % faster than 8 on the tutorial's chained levels, 32 no
-        // faster)
+        // a group spans kWaveRounds rounds of the prefetch ring (16 steps: 9 %
+        // faster than 8 on the tutorial's chained levels, 32 no faster)
         for (int i0 = 0; i0 < nSteps; i0 += kWaveRounds * kPrefetch) {
 #pragma unroll
             for (int uu = 0; uu < kWaveRounds * kPrefetch; ++uu) {
@@ -366,19 +367,21 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     __builtin_amdgcn_wave_barrier();
     Step buf[kBoxAhead];
     U hbuf[kBoxAhead], cbuf[kBoxAhead];
+    // the first steps' loads in the order they are consumed (not reordered):
+    // the loop's first use then waits only for the loads it needs
 #pragma unroll
-    for (int u = 0; u < kBoxInAhead; ++u) loadIn(u, hbuf[u], cbuf[u]);
-#pragma unroll
-    for (int u = 0; u < kBoxAhead; ++u) load(u, buf[u]);
+    for (int u = 0; u < kBoxAhead; ++u) {
+        if (u < kBoxInAhead) loadIn(u, hbuf[u], cbuf[u]);
+        load(u, buf[u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     T yprev = T(0);
     // the loop runs whole groups of steps: a step past the end is invalid for
     // every lane, reads clamped addresses and stores nothing, so the only
     // conditional code is the wave-uniform slow path and the stores.  The
-    // body is unrolled over kBoxRounds rounds of the prefetch ring: within a
-    // group every record wait is exact, while the first step of a group waits
-    // for all loads in flight (the compiler does not carry the prefetch counts
-    // across the back edge).  Longer groups drain less often but run a longer
-    // body; 16 steps measured fastest (8 and 32 were slower)
+    // body is unrolled over kBoxRounds rounds of the prefetch ring, every
+    // record wait exact (the initial loads above are issued in order for
+    // that); 16 steps measured fastest (8 and 32 were slower)
     for (int i0 = 0; i0 < nSteps; i0 += kBoxRounds * kBoxAhead) {
 #pragma unroll
         for (int uu = 0; uu < kBoxRounds * kBoxAhead; ++uu) {
