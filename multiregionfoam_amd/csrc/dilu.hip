@@ -365,6 +365,9 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // coefficient multiplies whatever its lane holds
     xch[lane] = T(0);
     __builtin_amdgcn_wave_barrier();
+    // the neighbours' values for the next step, read as soon as they are
+    // written (below), so the LDS round trip overlaps the rest of the step
+    T xbN = xch[srcB], xcN = xch[srcC];
     Step buf[kBoxAhead];
     U hbuf[kBoxAhead], cbuf[kBoxAhead];
     // the first steps' loads in the order they are consumed (not reordered):
@@ -414,7 +417,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 }
                 __builtin_amdgcn_s_waitcnt(0);
             }
-            T xb = xch[srcB], xc = xch[srcC];
+            T xb = xbN, xc = xcN;
             // both exchange reads are issued by every lane (not sunk into the
             // select), so the step has no divergent code
             asm volatile("" : "+v"(xb), "+v"(xc));
@@ -449,6 +452,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             // of a wavefront execute in order) before any overwrites them
             xch[lane] = yz;
             __builtin_amdgcn_wave_barrier();
+            xbN = xch[srcB];
+            xcN = xch[srcC];
             yprev = yz;
             if (inRange) {
                 const int s = s0 + t;
