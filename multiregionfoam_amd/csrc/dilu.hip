@@ -152,6 +152,18 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
         // tutorial's solve 1.3 % slower
 #pragma unroll
         for (int u = 0; u < kPrefetch; ++u) load(u, buf[u]);
+        // ring values of a step's dependencies, read as soon as the previous
+        // step has written its slot (a dependency is at most kRing - 1 steps
+        // back, and the slot the step overwrites is never among them), so the
+        // LDS round trip overlaps the rest of the previous step.  A code
+        // without a ring slot reads slot 0, which the select below discards.
+        auto ringSlot = [&](const Step &S, int k) {
+            const int cd = (elem(S.c[k / 16], (k / 4) % 4) >> (8 * (k % 4))) & 0xFF;
+            return cd < 256 - 2 ? cd : 0;
+        };
+        T rv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) rv[k] = ring[ringSlot(buf[0], k)];
         // whole groups of steps: a step past the end reads the clamped last
         // slice and stores nothing (its LDS ring slot is never read), so the
         // group has no early exits and the record waits stay exact within it;
@@ -211,8 +223,7 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
                     else y = last;
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        const int slot = code[k] < 256 - 2 ? code[k] : 0;
-                        T v = ring[slot];
+                        T v = rv[k];
                         if (HMASK & (1 << k)) {
                             const U h = S.hv[__builtin_popcount(HMASK & ((1 << k) - 1))];
                             v = code[k] == kCodeInbox ? Bits<T>::from(h) : v;
@@ -227,6 +238,8 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
                     // drain the prefetch loads) is needed between steps
                     ring[(i % kRing) * 64 + lane] = y;
                     __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int k = 0; k < K; ++k) rv[k] = ring[ringSlot(buf[(u + 1) % kPrefetch], k)];
                     if (inRange) {
                         if (MODE == kDiluFactor) {
                             const double rd = V.cellOfPos[p] >= 0 ? 1.0 / (double)y : 0.0;
