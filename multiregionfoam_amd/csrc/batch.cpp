@@ -180,7 +180,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
 
     std::vector<int32_t> ellOff, ellCol, ifBase, ifRowFace, ifPairOff, ifPairCol;
     std::vector<unsigned long long> ifMask;
-    std::vector<int32_t> cellOfPos, fCode, bCode, fHalo, bHalo, waveSlice, waveSteps, waveSys;
+    std::vector<int32_t> cellOfPos, fCode, bCode, fHalo, bHalo, waveSlice, waveSteps, waveSys, waveOut;
     std::vector<double> diag, source, sumA, psi0, fProd;
     std::vector<std::array<int, 3>> waveKey;   // (key, system, global wave index)
     std::vector<uint8_t> waveBox;
@@ -370,6 +370,16 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
                 putOuts(fHalo[p], fOuts, fc);
                 putOuts(bHalo[p], bOuts, bc);
             }
+        // per wave: bit 0 (1) when a row of it delivers a forward (backward)
+        // value to another wave's inbox; a wave without skips the deliveries
+        waveOut.assign(waveSlice.size(), 0);
+        for (size_t w = 0; w < waveSlice.size(); ++w)
+            for (int s = waveSlice[w]; s < waveSlice[w] + waveSteps[w]; ++s)
+                for (int l = 0; l < kSlice; ++l)
+                    for (int j = 0; j < K_; ++j) {
+                        if (fc[intAt(s, NC + j, l)] >= 0) waveOut[w] |= 1;
+                        if (bc[intAt(s, NC + j, l)] >= 0) waveOut[w] |= 2;
+                    }
         nHaloF_ = nSl * KIF * 64;
         nHaloB_ = nSl * KIB * 64;
         auto asVec = [](const std::vector<int32_t> &a) {
@@ -421,6 +431,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     info_.upload(hInfo_, st);
     order_.upload(order, st);
     boxOrder_.upload(boxOrder, st);
+    waveOut_.upload(waveOut, st);
     boxMeta_.upload(boxMeta, st);
     waveSlice_.upload(waveSlice, st);
     waveSteps_.upload(waveSteps, st);
@@ -535,6 +546,7 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.boxMeta = boxMeta_.p;
     D.nBoxIn = nBoxIn_;
     D.fCodes = fCodes_.p;
+    D.waveOut = waveOut_.p;
     D.bCodes = bCodes_.p;
     D.fPack = m.fPack.p;
     D.bPack = m.bPack.p;

@@ -230,7 +230,7 @@ private:
     DevBuf<BlockEntry> blocks_;
     DevBuf<SysInfo> info_;
     // DILU schedule
-    DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_, boxOrder_, boxMeta_;
+    DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_, waveOut_, boxOrder_, boxMeta_;
     DevBuf<int4> fCodes_, bCodes_;
     DevBuf<double2> prodPack_;
     int haloMaskF_ = 0, haloMaskB_ = 0;

@@ -116,6 +116,8 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
     const bool active = MODE == kDiluFactor || st[sys].status == kRunning;
     const int s0 = V.waveSlice[w], nSteps = V.waveSteps[w];
     const int4 *codes = MODE == kDiluBwd ? V.bCodes : V.fCodes;
+    // whether any row of this wave delivers to another wave (wave-uniform)
+    const bool delivers = (V.waveOut[w] & (MODE == kDiluBwd ? 2 : 1)) != 0;
     const R *recs = MODE == kDiluFactor ? (const R *)V.prodPack : (const R *)(MODE == kDiluBwd ? V.bPack : V.fPack);
     const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
     unsigned long long nSlow = 0;
@@ -252,12 +254,14 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
                             out[p] = y;
                         }
                         // deliver the value to the inboxes of its consumers in other waves
+                        if (delivers) {
 #pragma unroll
-                        for (int j = 0; j < K; ++j) {
-                            const int o = elem(S.c[(NC + j) / 4], (NC + j) % 4);
-                            if (o >= 0)
-                                __hip_atomic_store(&inbox[o], Bits<T>::to(y), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
+                            for (int j = 0; j < K; ++j) {
+                                const int o = elem(S.c[(NC + j) / 4], (NC + j) % 4);
+                                if (o >= 0)
+                                    __hip_atomic_store(&inbox[o], Bits<T>::to(y), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                            }
                         }
                     }
                     load(i + kPrefetch, S);
@@ -550,6 +554,7 @@ static hipError_t launchK(const DiluView<T> &V, const SysState *st, const T *in,
                           typename Bits<T>::U *use, typename Bits<T>::U *clr, int nClr, hipStream_t s)
 {
     if (V.nWaves <= 0) return hipSuccess;
+    if (!V.waveOut) return hipErrorInvalidValue;
     const int mask = effectiveHaloMask(V.K, MODE == kDiluBwd ? V.haloMaskB : V.haloMaskF);
     switch (V.K) {
     case 3:

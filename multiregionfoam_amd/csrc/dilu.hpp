@@ -61,6 +61,7 @@ struct DiluView {
     const int32_t *waveSlice = nullptr, *waveSteps = nullptr, *waveSys = nullptr;
     int nWaves = 0, K = 3;
     const int4 *fCodes = nullptr, *bCodes = nullptr;
+    const int32_t *waveOut = nullptr;    // per wave: bit 0 / 1 when it delivers forward / backward values
     typename Pack<T>::V *fPack = nullptr, *bPack = nullptr;
     const double2 *prodPack = nullptr;   // factor records (FP64)
     double *rDout = nullptr;             // factor: plain rD per row
