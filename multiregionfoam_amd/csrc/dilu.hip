@@ -308,7 +308,9 @@ constexpr int kBoxRounds = 4;    // prefetch-ring rounds per unrolled group of s
 // KT: the batch's record width K when it is 3 (records of two FP64 / one
 // FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
 // rD in a chunk of its own).
-template <typename T, int KT, int MODE, int AX>
+// LONE: the tile has no neighbouring tile (alone in its region), so it has
+// no inbox to read and no values to deliver.
+template <typename T, int KT, int MODE, int AX, bool LONE>
 __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m, const T *__restrict__ in, T *out,
                                          typename Bits<T>::U *inbox, T *xch, unsigned long long &nSlow)
 {
@@ -387,11 +389,13 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     T xbN = xch[srcB], xcN = xch[srcC];
     Step buf[kBoxAhead];
     U hbuf[kBoxAhead], cbuf[kBoxAhead];
+#pragma unroll
+    for (int u = 0; u < kBoxAhead; ++u) hbuf[u] = cbuf[u] = kSent;   // a lone tile never loads them
     // the first steps' loads in the order they are consumed (not reordered):
     // the loop's first use then waits only for the loads it needs
 #pragma unroll
     for (int u = 0; u < kBoxAhead; ++u) {
-        if (u < kBoxInAhead) loadIn(u, hbuf[u], cbuf[u]);
+        if (!LONE && u < kBoxInAhead) loadIn(u, hbuf[u], cbuf[u]);
         load(u, buf[u]);
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -416,8 +420,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             const bool valid = inTile && (unsigned)ia < (unsigned)nA;
             const bool hasA = kFwd ? ia >= 1 : ia + 1 < nA;
             const bool needB = valid & useB, needC = valid & useC;
-            const bool miss = (needB & (hb == kSent)) | (needC & (hc == kSent));
-            if (__builtin_expect(__any(miss), 0)) {
+            const bool miss = !LONE && ((needB & (hb == kSent)) | (needC & (hc == kSent)));
+            if (!LONE && __builtin_expect(__any(miss), 0)) {
                 // rare slow path (wave-uniform): wait for the late values and
                 // reload the inbox values already requested for later steps
                 ++nSlow;
@@ -485,15 +489,16 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                     (out + (size_t)s * 64)[lane] = yz;
                 }
             }
-            if (valid && sendB)
+            if (!LONE && valid && sendB)
                 __hip_atomic_store(&(inbox + (ptrdiff_t)(t + dB) * TBC)[outB], Bits<T>::to(y), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-            if (valid && sendC)
+            if (!LONE && valid && sendC)
                 __hip_atomic_store(&(inbox + (ptrdiff_t)(t + dC) * TBC)[outC], Bits<T>::to(y), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             // the inbox values of a later step first, so that waiting for them
             // leaves the record loads issued after them in flight
-            loadIn(i + kBoxInAhead, hbuf[(u + kBoxInAhead) % kBoxAhead], cbuf[(u + kBoxInAhead) % kBoxAhead]);
+            if (!LONE)
+                loadIn(i + kBoxInAhead, hbuf[(u + kBoxInAhead) % kBoxAhead], cbuf[(u + kBoxInAhead) % kBoxAhead]);
             load(i + kBoxAhead, S);
             // keep each step's prefetches where they are (not sunk towards
             // their use)
@@ -514,10 +519,14 @@ __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *
     const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
     unsigned long long nSlow = 0;
     if (active && m[kBmSteps] > 0) {
-        switch (m[kBmAxis]) {
-        case 0: boxSweep<T, KT, MODE, 0>(V, m, in, out, inbox, xch, nSlow); break;
-        case 1: boxSweep<T, KT, MODE, 1>(V, m, in, out, inbox, xch, nSlow); break;
-        default: boxSweep<T, KT, MODE, 2>(V, m, in, out, inbox, xch, nSlow); break;
+        const bool lone = m[kBmPrevB] < 0 && m[kBmPrevC] < 0 && m[kBmNextB] < 0 && m[kBmNextC] < 0;
+        switch (m[kBmAxis] + (lone ? 3 : 0)) {
+        case 0: boxSweep<T, KT, MODE, 0, false>(V, m, in, out, inbox, xch, nSlow); break;
+        case 1: boxSweep<T, KT, MODE, 1, false>(V, m, in, out, inbox, xch, nSlow); break;
+        case 2: boxSweep<T, KT, MODE, 2, false>(V, m, in, out, inbox, xch, nSlow); break;
+        case 3: boxSweep<T, KT, MODE, 0, true>(V, m, in, out, inbox, xch, nSlow); break;
+        case 4: boxSweep<T, KT, MODE, 1, true>(V, m, in, out, inbox, xch, nSlow); break;
+        default: boxSweep<T, KT, MODE, 2, true>(V, m, in, out, inbox, xch, nSlow); break;
         }
     }
     if (V.trace && threadIdx.x == 0) {
