@@ -414,6 +414,10 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     nGenWaves_ = (int)order.size();
     nBoxWaves_ = (int)boxOrder.size();
     nBoxIn_ = boxIn;
+    boxLone_ = nBoxWaves_ > 0;
+    for (int w : boxOrder)
+        for (int k : {kBmPrevB, kBmPrevC, kBmNextB, kBmNextC})
+            if (boxMeta[(size_t)w * kBoxMeta + k] >= 0) boxLone_ = false;
     if (nGenWaves_ == 0) nHaloF_ = nHaloB_ = 0;
     if (order.empty()) order.push_back(0);
     if (boxOrder.empty()) boxOrder.push_back(0);
@@ -543,6 +547,7 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.K = K_;
     D.boxOrder = boxOrder_.p;
     D.nBoxWaves = nBoxWaves_;
+    D.boxLone = boxLone_;
     D.boxMeta = boxMeta_.p;
     D.nBoxIn = nBoxIn_;
     D.fCodes = fCodes_.p;

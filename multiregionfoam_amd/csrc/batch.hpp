@@ -221,6 +221,7 @@ private:
     std::vector<int> sliceBegin_;
     int nSlices_ = 0, nPos_ = 0, nBlk_ = 0, nWaves_ = 0, K_ = 3, nHaloF_ = 0, nHaloB_ = 0;
     int nGenWaves_ = 0, nBoxWaves_ = 0, nBoxIn_ = 0;
+    bool boxLone_ = false;   // every box tile alone in its region
     bool factorised_ = false, haveFloat_ = false;
 
     // structure (shared by both precisions)

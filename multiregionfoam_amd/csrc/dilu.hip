@@ -507,7 +507,9 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     }
 }
 
-template <typename T, int KT, int MODE>
+// LONE: every box tile of the launch is alone in its region (chosen by the
+// launcher, so a launch with neighbouring tiles runs no lone-tile code).
+template <typename T, int KT, int MODE, bool LONE>
 __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *st, const T *__restrict__ in, T *out,
                                                  typename Bits<T>::U *inbox, typename Bits<T>::U *inboxClear,
                                                  int nClear)
@@ -519,14 +521,10 @@ __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *
     const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
     unsigned long long nSlow = 0;
     if (active && m[kBmSteps] > 0) {
-        const bool lone = m[kBmPrevB] < 0 && m[kBmPrevC] < 0 && m[kBmNextB] < 0 && m[kBmNextC] < 0;
-        switch (m[kBmAxis] + (lone ? 3 : 0)) {
-        case 0: boxSweep<T, KT, MODE, 0, false>(V, m, in, out, inbox, xch, nSlow); break;
-        case 1: boxSweep<T, KT, MODE, 1, false>(V, m, in, out, inbox, xch, nSlow); break;
-        case 2: boxSweep<T, KT, MODE, 2, false>(V, m, in, out, inbox, xch, nSlow); break;
-        case 3: boxSweep<T, KT, MODE, 0, true>(V, m, in, out, inbox, xch, nSlow); break;
-        case 4: boxSweep<T, KT, MODE, 1, true>(V, m, in, out, inbox, xch, nSlow); break;
-        default: boxSweep<T, KT, MODE, 2, true>(V, m, in, out, inbox, xch, nSlow); break;
+        switch (m[kBmAxis]) {
+        case 0: boxSweep<T, KT, MODE, 0, LONE>(V, m, in, out, inbox, xch, nSlow); break;
+        case 1: boxSweep<T, KT, MODE, 1, LONE>(V, m, in, out, inbox, xch, nSlow); break;
+        default: boxSweep<T, KT, MODE, 2, LONE>(V, m, in, out, inbox, xch, nSlow); break;
         }
     }
     if (V.trace && threadIdx.x == 0) {
@@ -584,8 +582,13 @@ static hipError_t launchBox(const DiluView<T> &V, const SysState *st, const T *i
 {
     if (V.nBoxWaves <= 0) return hipSuccess;
     const int n = V.nBoxWaves;
-    if (V.K == 3) k_dilu_box<T, 3, MODE><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
-    else k_dilu_box<T, 0, MODE><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+    if (V.boxLone) {
+        if (V.K == 3) k_dilu_box<T, 3, MODE, true><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        else k_dilu_box<T, 0, MODE, true><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+    } else {
+        if (V.K == 3) k_dilu_box<T, 3, MODE, false><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        else k_dilu_box<T, 0, MODE, false><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+    }
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ struct DiluView {
     // (TB + TC) values per step per wave, one per direction
     const int32_t *boxOrder = nullptr;
     int nBoxWaves = 0;
+    bool boxLone = false;                // every box tile is alone in its region
     const int32_t *boxMeta = nullptr;
     void *boxInF = nullptr, *boxInB = nullptr, *boxInC = nullptr;
     int nBoxIn = 0;
