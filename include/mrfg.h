@@ -51,7 +51,9 @@ typedef struct {
     int n_iterations;          /* Krylov iterations (mixed: FP32 inner iterations plus the FP64
                                   continuation of a system whose refinement stalled) */
     int converged;
-    int singular;              /* breakdown (rho or (t,t) or (w,p) zero) */
+    int singular;              /* 0; 1 = breakdown (rho, (rw,v) or (w,p) zero, or s = 0 without
+                                  meeting the tolerance); 2 = diverged: a residual became NaN or Inf
+                                  (the system is stopped at once and releases its batch) */
     int n_outer;               /* mixed: FP64 refinement steps; fp64: 0 */
     double seconds;            /* wall time of the solve on the device */
     double bytes_moved;        /* algorithmic bytes of the solve */
@@ -116,6 +118,17 @@ int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int regi
                        int n_faces_b, const int32_t *fc_b, const double *coeff_b,
                        const int32_t *off_b, const int32_t *idx_b, const double *w_b);
 
+/* The two conductances of the monolithicTemperature wall value per face of
+   each side of interface `iface` (calcTemperature,
+   monolithicThermalDiffusivityFvPatchScalarField.C:300-380, radiation off):
+   k_own = forig / (1 - w) / |delta| of the face (monolithicBase::korig,
+   monolithicBase.C:97-159) and k_nei = the GGI-interpolated other side's
+   forig / w / |delta|; positive.  Needed by mrfg_interface_patch_values for
+   systems defined from LDU arrays (cases built or read by this library
+   carry them).  May be called before or after mrfg_commit. */
+int mrfg_set_interface_wall(mrfg_ctx *ctx, int sys, int iface, const double *k_own_a, const double *k_nei_a,
+                            const double *k_own_b, const double *k_nei_b);
+
 /* DILU schedule selection for the next mrfg_commit: MRFG_SCHEDULE_AUTO
    (structured hex regions as line tiles, others by levels) or
    MRFG_SCHEDULE_LEVELS (levels everywhere).  Both compute the same factor
@@ -125,6 +138,26 @@ int mrfg_set_schedule(mrfg_ctx *ctx, int schedule);
 
 /* Build the device layout of every system defined so far and upload it. */
 int mrfg_commit(mrfg_ctx *ctx);
+
+/* Value refresh of a committed system defined from LDU arrays: the next
+   time step's assembled values on the same mesh.  multiRegionFoam's time
+   loop (applications/solvers/multiRegionFoam/multiRegionFoam.C:94-117)
+   re-assembles every region's fvMatrix each step (regionType::setCoupledEqns,
+   multiRegionSystem.C:115) and hands it to coupledFvMatrix::solve
+   (multiRegionSystem.C:143-153); the addressing and the GGI weights stay
+   those of the mesh.  These calls replace the values only -- region: diag,
+   lower (NULL for a region committed symmetric), upper, source, in the
+   arrays' committed order; interface: the boundary coefficients of side a
+   and / or side b (a NULL side keeps its values) -- and the device layout,
+   DILU schedule and maps built by mrfg_commit are kept: the values are
+   staged on the device and put in place (SpMV values, DILU records,
+   diagonal, source, row sums, interface coefficients) before the next solve
+   or kernel call.  Commit once per mesh, refresh per time step.  Not for
+   systems read from disk (their values come from mrfg_set_field /
+   mrfg_assemble) or batches with ddt terms (MRFG_ERR_ARG). */
+int mrfg_update_region(mrfg_ctx *ctx, int sys, int region, const double *diag, const double *lower,
+                       const double *upper, const double *source);
+int mrfg_update_interface(mrfg_ctx *ctx, int sys, int iface, const double *coeff_a, const double *coeff_b);
 
 /* coupledFvMatrix<scalar>::solve(dict) for every committed system
    (multiRegionSystem.C:150-153).  perf receives one entry per system. */
@@ -215,6 +248,19 @@ int mrfg_set_psi(mrfg_ctx *ctx, int sys, int region, const double *psi);
    A conserving coupled solution gives q_a + q_b = 0. */
 int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b);
 
+/* The coupled patch values of side `side` (0 = a, 1 = b) of interface
+   `iface` from the system's current field, on the device in FP64, as
+   coupledFvMatrix::solve's correctBoundaryConditions leaves them on the
+   monolithicTemperature patches: t_neighbour = the patch neighbour field,
+   the GGI interpolate of the other side's face-cell values
+   (monolithicCouplingFvPatchField::patchNeighbourField,
+   monolithicCouplingFvPatchField.C:208-236), and t_wall = the patch value,
+   (k_own T_own + k_nei t_neighbour) / (k_own + k_nei)
+   (monolithicTemperatureFvPatchScalarField::initEvaluate / updateCoeffs ->
+   calcTemperature, monolithicTemperatureFvPatchScalarField.C:147-202).  One value per face
+   of that side.  MRFG_ERR_ARG if the interface has no wall conductances. */
+int mrfg_interface_patch_values(mrfg_ctx *ctx, int sys, int iface, int side, double *t_neighbour, double *t_wall);
+
 /* ---- finite-volume assembly on the device (cases read from disk) ------ */
 
 /* Replace one field of region `region` of committed system `sys`, a case
@@ -303,6 +349,11 @@ int mrfg_case_interface_sizes(const mrfg_case *c, int iface, int *region_a, int 
 int mrfg_case_interface(const mrfg_case *c, int iface,
                         int32_t *fc_a, double *coeff_a, int32_t *off_a, int32_t *idx_a, double *w_a,
                         int32_t *fc_b, double *coeff_b, int32_t *off_b, int32_t *idx_b, double *w_b);
+/* The wall conductances (see mrfg_set_interface_wall) of a built case's
+   interface, n_faces_a / n_faces_b values each; MRFG_ERR_STATE if the case
+   has none. */
+int mrfg_case_interface_wall(const mrfg_case *c, int iface, double *k_own_a, double *k_nei_a, double *k_own_b,
+                             double *k_nei_b);
 /* Read an OpenFOAM / foam-extend multi-region case directory and assemble
    it like mrfg_case_build: constant/multiRegionProperties (region types),
    constant/<region>/polyMesh (ASCII points, faces, owner, neighbour,

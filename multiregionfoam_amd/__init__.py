@@ -85,6 +85,8 @@ def lib():
                                 C.c_int, I32P, F64P, I32P, I32P, F64P], C.c_int),
         "mrfg_set_region_ddt": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
         "mrfg_commit": ([vp], C.c_int),
+        "mrfg_update_region": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, F64P], C.c_int),
+        "mrfg_update_interface": ([vp, C.c_int, C.c_int, F64P, F64P], C.c_int),
         "mrfg_set_schedule": ([vp, C.c_int], C.c_int),
         "mrfg_solve": ([vp, C.POINTER(Controls), C.POINTER(Perf)], C.c_int),
         "mrfg_solve_dict": ([vp, C.c_char_p, C.c_char_p, C.POINTER(Perf)], C.c_int),
@@ -112,6 +114,9 @@ def lib():
         "mrfg_case_interface": ([vp, C.c_int, I32P, F64P, I32P, I32P, F64P, I32P, F64P, I32P, I32P, F64P],
                                 C.c_int),
         "mrfg_add_case": ([vp, vp, C.POINTER(C.c_int)], C.c_int),
+        "mrfg_case_interface_wall": ([vp, C.c_int, F64P, F64P, F64P, F64P], C.c_int),
+        "mrfg_set_interface_wall": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, F64P], C.c_int),
+        "mrfg_interface_patch_values": ([vp, C.c_int, C.c_int, C.c_int, F64P, F64P], C.c_int),
         "mrfg_spmv": ([vp, C.c_int, F64P, F64P], C.c_int),
         "mrfg_dilu_factor": ([vp, C.c_int, F64P], C.c_int),
         "mrfg_precondition": ([vp, C.c_int, F64P, F64P], C.c_int),
@@ -146,13 +151,14 @@ def lib():
 
 EXPORTED = (
     "mrfg_version mrfg_create mrfg_destroy mrfg_last_error mrfg_begin_system mrfg_set_region mrfg_set_region_ddt "
-    "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_solve mrfg_solve_dict mrfg_time_steps mrfg_parse_controls "
+    "mrfg_set_interface mrfg_set_schedule mrfg_commit mrfg_update_region mrfg_update_interface mrfg_solve mrfg_solve_dict mrfg_time_steps mrfg_parse_controls "
     "mrfg_dna_solve mrfg_dna_history mrfg_dna_time_steps "
     "mrfg_get_psi mrfg_reset_psi "
     "mrfg_set_psi mrfg_case_build mrfg_case_read_foam mrfg_case_free mrfg_case_sizes mrfg_case_region_sizes "
     "mrfg_case_region mrfg_case_region_ddt mrfg_case_dna_controls mrfg_case_dna_side_sizes mrfg_case_dna_side "
     "mrfg_case_region_source_base "
-    "mrfg_case_interface_sizes mrfg_case_interface mrfg_add_case mrfg_spmv mrfg_dilu_factor "
+    "mrfg_case_interface_sizes mrfg_case_interface mrfg_case_interface_wall mrfg_set_interface_wall "
+    "mrfg_interface_patch_values mrfg_add_case mrfg_spmv mrfg_dilu_factor "
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
     "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
@@ -201,7 +207,9 @@ def controls(**kw):
 
 @dataclass
 class SolverPerformance:
-    """lduSolverPerformance of one coupled solve."""
+    """lduSolverPerformance of one coupled solve.  ``singular``: the Krylov
+    recurrence broke down; ``diverged``: a residual became NaN or Inf (the
+    system was stopped at once)."""
     initial_residual: float
     final_residual: float
     n_iterations: int
@@ -210,6 +218,12 @@ class SolverPerformance:
     n_outer: int
     seconds: float
     bytes_moved: float
+    diverged: bool = False
+
+    @classmethod
+    def from_c(cls, p):
+        return cls(p.initial_residual, p.final_residual, p.n_iterations, bool(p.converged), p.singular == 1,
+                   p.n_outer, p.seconds, p.bytes_moved, p.singular == 2)
 
     def line(self, solver="BiCGStab", field="T"):
         return ("%s:  Solving for %s, Initial residual = %.6g, Final residual = %.6g, No Iterations %d"
@@ -305,6 +319,11 @@ class Case:
                                   _p(d["offA"], C.c_int32), _p(d["idxA"], C.c_int32), _p(d["wA"], C.c_double),
                                   _p(d["fcB"], C.c_int32), _p(d["coeffB"], C.c_double),
                                   _p(d["offB"], C.c_int32), _p(d["idxB"], C.c_int32), _p(d["wB"], C.c_double))
+        # the wall conductances of the monolithicTemperature patch value
+        wall = dict(kOwnA=np.zeros(na), kNeiA=np.zeros(na), kOwnB=np.zeros(nb), kNeiB=np.zeros(nb))
+        if lib().mrfg_case_interface_wall(self._h, i, *[_p(wall[k], C.c_double)
+                                                        for k in ("kOwnA", "kNeiA", "kOwnB", "kNeiB")]) == MRFG_OK:
+            d.update(wall)
         return d
 
     def partitioned(self):
@@ -428,6 +447,10 @@ class CoupledFvMatrix:
                 _p(a["idxA"], C.c_int32), _p(a["wA"], C.c_double),
                 int(I["nB"]), _p(a["fcB"], C.c_int32), _p(a["coeffB"], C.c_double), _p(a["offB"], C.c_int32),
                 _p(a["idxB"], C.c_int32), _p(a["wB"], C.c_double)))
+            if I.get("kOwnA") is not None:
+                wl = [np.ascontiguousarray(I[k], dtype=np.float64) for k in ("kOwnA", "kNeiA", "kOwnB", "kNeiB")]
+                keep.append(wl)
+                self._check(lib().mrfg_set_interface_wall(self._h, s, i, *[_p(x, C.c_double) for x in wl]))
         self._sizes.append([int(R["nCells"]) for R in regions])
         self._faces.append([int(R["nFaces"]) for R in regions])
         self._ifaces.append([(int(I["nA"]), int(I["nB"])) for I in interfaces])
@@ -443,6 +466,28 @@ class CoupledFvMatrix:
     def n_systems(self):
         return len(self._sizes)
 
+    def update_region(self, sys, region, diag, upper, source, lower=None):
+        """New values of one committed region on its committed addressing
+        (the next time step's fvMatrix); ``lower`` None for a region
+        committed symmetric.  Placed on the device before the next solve."""
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        d, lo, up, src = f64(diag), f64(lower), f64(upper), f64(source)
+        if d.size != self._sizes[sys][region] or src.size != d.size or up.size != self._faces[sys][region] \
+                or (lo is not None and lo.size != up.size):
+            raise ValueError("value arrays do not match the committed region sizes")
+        self._check(lib().mrfg_update_region(self._h, sys, region, _p(d, C.c_double), _p(lo, C.c_double),
+                                             _p(up, C.c_double), _p(src, C.c_double)))
+
+    def update_interface(self, sys, iface, coeffA=None, coeffB=None):
+        """New boundary coefficients of one interface side or both (None:
+        unchanged); the GGI weights stay those committed."""
+        na, nb = self._ifaces[sys][iface]
+        a = None if coeffA is None else np.ascontiguousarray(coeffA, dtype=np.float64)
+        b = None if coeffB is None else np.ascontiguousarray(coeffB, dtype=np.float64)
+        if (a is not None and a.size != na) or (b is not None and b.size != nb):
+            raise ValueError("coefficient arrays do not match the interface sizes")
+        self._check(lib().mrfg_update_interface(self._h, sys, iface, _p(a, C.c_double), _p(b, C.c_double)))
+
     def solve(self, ctrl=None, fv_solution=None, entry="Tcoupled", **kw):
         """Solve every system.  ``ctrl``: Controls; or ``fv_solution`` text +
         ``entry``; or keyword controls (see :func:`controls`)."""
@@ -453,9 +498,7 @@ class CoupledFvMatrix:
             if ctrl is None:
                 ctrl = controls(**kw)
             self._check(lib().mrfg_solve(self._h, C.byref(ctrl), perf))
-        return [SolverPerformance(p.initial_residual, p.final_residual, p.n_iterations, bool(p.converged),
-                                  bool(p.singular), p.n_outer, p.seconds, p.bytes_moved)
-                for p in perf[:self.n_systems]]
+        return [SolverPerformance.from_c(p) for p in perf[:self.n_systems]]
 
     def time_steps(self, n_steps, ddt="backward", ctrl=None, fv_solution=None, entry="Tcoupled", **kw):
         """``n_steps`` time steps of the transient systems (the ddt terms
@@ -471,9 +514,7 @@ class CoupledFvMatrix:
         self._check(lib().mrfg_time_steps(self._h, C.byref(ctrl), _text(ddt), int(n_steps), perf))
         out = []
         for k in range(n_steps):
-            out.append([SolverPerformance(p.initial_residual, p.final_residual, p.n_iterations, bool(p.converged),
-                                          bool(p.singular), p.n_outer, p.seconds, p.bytes_moved)
-                        for p in perf[k * n:(k + 1) * n]])
+            out.append([SolverPerformance.from_c(p) for p in perf[k * n:(k + 1) * n]])
         return out
 
     # ---- finite-volume assembly on the device (cases read from disk) --------
@@ -601,6 +642,21 @@ class CoupledFvMatrix:
         self._check(lib().mrfg_interface_flux(self._h, sys, iface, C.byref(qa), C.byref(qb)))
         return qa.value, qb.value
 
+    def interface_patch_values(self, iface=0, side=0, sys=0):
+        """(t_neighbour, t_wall) of one interface side from the current FP64
+        field: the GGI-interpolated neighbour field and the
+        monolithicTemperature patch value, computed on the device."""
+        n = self._ifaces[sys][iface][side]
+        tn, tw = np.zeros(n), np.zeros(n)
+        self._check(lib().mrfg_interface_patch_values(self._h, sys, iface, side, _p(tn, C.c_double),
+                                                      _p(tw, C.c_double)))
+        return tn, tw
+
+    def set_interface_wall(self, sys, iface, kOwnA, kNeiA, kOwnB, kNeiB):
+        """Wall conductances of the patch values (per face of each side)."""
+        wl = [np.ascontiguousarray(x, dtype=np.float64) for x in (kOwnA, kNeiA, kOwnB, kNeiB)]
+        self._check(lib().mrfg_set_interface_wall(self._h, sys, iface, *[_p(x, C.c_double) for x in wl]))
+
     def copy_probe(self, nbytes, reps=10):
         """Average ms of a plain device copy of nbytes (reads + writes nbytes)."""
         ms = C.c_double()
@@ -700,8 +756,7 @@ class Server:
         rc = lib().mrfg_server_result(self._h, rid, C.byref(perf), _p(t, C.c_double), C.byref(bs))
         if rc != MRFG_OK:
             raise MrfgError(rc, lib().mrfg_server_last_error(self._h).decode())
-        sp = SolverPerformance(perf.initial_residual, perf.final_residual, perf.n_iterations, bool(perf.converged),
-                               bool(perf.singular), perf.n_outer, perf.seconds, perf.bytes_moved)
+        sp = SolverPerformance.from_c(perf)
         return ServeResult(sp, float(t[0]), float(t[1]), float(t[2]), bs.value)
 
     def psi(self, rid):

@@ -72,6 +72,8 @@ __global__ __launch_bounds__(kAsmBlock) void k_fv_sides(FvDev m)
     const double delta = 1.0 / magDelta;
     const double kOwn = fOwn / (1.0 - wP) / magDelta;
     const double kNei = fNei / wP / magDelta;
+    m.sKOwn[i] = kOwn;
+    m.sKNei[i] = kNei;
     double kk = kOwn * kNei;
     kk /= delta * (kOwn + kNei);
     const double kHarm = 1.0 / (wP / fOwn + (1.0 - wP) / fNei);
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(kAsmBlock) void k_fv_cells(FvDev m)
 // the interface coefficients in interface order)
 __global__ __launch_bounds__(kAsmBlock) void k_fv_place(FvPlace q)
 {
-    const int p = blockIdx.x * kAsmBlock + threadIdx.x;
+    const int p = q.pos0 + blockIdx.x * kAsmBlock + threadIdx.x;
     if (p >= q.nPos) return;
     const int c = q.cellOfPos[p];
     if (c < 0) return;
@@ -241,7 +243,7 @@ hipError_t launchFvAssemble(const FvDev &m, hipStream_t st)
 
 hipError_t launchFvPlace(const FvPlace &q, hipStream_t st)
 {
-    if (q.nPos > 0) k_fv_place<<<gridFor(q.nPos), kAsmBlock, 0, st>>>(q);
+    if (q.nPos > q.pos0) k_fv_place<<<gridFor(q.nPos - q.pos0), kAsmBlock, 0, st>>>(q);
     return hipGetLastError();
 }
 

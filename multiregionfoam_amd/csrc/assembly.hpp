@@ -45,6 +45,7 @@ struct FvDev {
     double *lapU, *aLo, *aUp;        // per internal face
     double *faceVal;                 // 2 per internal face: lower, upper
     double *sK, *sDelta;             // per side face: face conductivity, 1/|d|
+    double *sKOwn, *sKNei;           // per side face: the wall value's two conductances
     double *ifCoeff;                 // per side face
     double *diag, *source, *ddtA, *diagBase, *srcBase;   // per cell
 };
@@ -52,6 +53,7 @@ struct FvDev {
 // Where the assembled values go, per batch position.
 struct FvPlace {
     int nPos, K, G;
+    int pos0;                        // positions [pos0, nPos) are placed
     const int32_t *cellOfPos;        // batch cell of the model, -1 (padding / no model)
     const int32_t *ellOff, *ellSrc;  // ELL columns per slice; source face value or -1
     const int32_t *fSrc, *bSrc;      // per DILU slot (nSlices*K*64)

@@ -480,6 +480,12 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     haveFloat_ = false;   // FP32 copies are rebuilt (in the buffers kept) on demand
     hostStale_ = false;
     buildFv();
+    rf_.reset();
+    // the interface face lists (flux check, patch values)
+    ifDev_.clear();
+    for (int q = 0; q < nS; ++q)
+        for (int i = 0; i < (int)systems[q]->interfaces.size(); ++i)
+            for (int sd = 0; sd < 2; ++sd) (void)ifaceSlot(q, i, sd);
     HIP_OK(hipStreamSynchronize(st));
 }
 
@@ -733,6 +739,22 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
         }
         for (int q = 0; q < nS; ++q)
             if (hi[q].status != kIdle) innerIters[q] += hi[q].nIter;
+        // an inner solve that broke down or left the FP32 range (singular,
+        // diverged) gives no correction: the system continues in FP64 from
+        // its current field
+        {
+            bool bad = false;
+            for (int q = 0; q < nS; ++q)
+                if (hs[q].status == kRunning && (hi[q].status == kSingular || hi[q].status == kDiverged)) {
+                    fallback[q] = 1;
+                    hs[q].status = kIdle;
+                    bad = true;
+                }
+            if (bad) {
+                HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+            }
+        }
         HIP_OK(launchRefineUpdate(K64, K32, stream_));
         HIP_OK(launchRefineResidual(A64, K64, stream_));
         readState(hs);
@@ -792,6 +814,7 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const s
 {
     if (active && active->size() != sys_.size()) throw std::runtime_error("active mask does not match the batch");
     HIP_OK(hipSetDevice(device_));
+    applyRefresh();
     if (c.precision == SolverControls::Mixed) ensureFloat();
     HIP_OK(hipStreamSynchronize(stream_));
     auto t0 = std::chrono::steady_clock::now();
@@ -814,7 +837,7 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const s
         p.initialResidual = hs[q].initRes;
         p.finalResidual = hs[q].finalRes;
         p.converged = hs[q].status == kConverged;
-        p.singular = hs[q].status == kSingular;
+        p.singular = hs[q].status == kSingular ? 1 : hs[q].status == kDiverged ? 2 : 0;
         p.seconds = secs;
         if (c.precision == SolverControls::Mixed) {
             p.nIterations = inner[q] + fp64More[q];
@@ -897,6 +920,7 @@ void Batch::getPsi(int sys, int region, double *psi)
 
 void Batch::spmv(const double *xHost, double *yHost, int sys)
 {
+    applyRefresh();
     const SystemLayout &L = *lay_.at(sys);
     std::vector<double> x(nPos_, 0.0), y(nPos_);
     for (long long c = 0; c < L.nCells; ++c) x[posBegin(sys) + L.posOfCell[c]] = xHost[c];
@@ -911,6 +935,7 @@ void Batch::spmv(const double *xHost, double *yHost, int sys)
 
 void Batch::diluFactor(int sys, double *rDHost)
 {
+    applyRefresh();
     factorised_ = false;
     factorise();
     const SystemLayout &L = *lay_.at(sys);
@@ -923,6 +948,7 @@ void Batch::diluFactor(int sys, double *rDHost)
 
 void Batch::precondition(const double *rHost, double *wHost, int sys)
 {
+    applyRefresh();
     factorise();
     const SystemLayout &L = *lay_.at(sys);
     std::vector<double> r(nPos_, 0.0), w(nPos_);
@@ -942,56 +968,133 @@ void Batch::precondition(const double *rHost, double *wHost, int sys)
     for (long long c = 0; c < L.nCells; ++c) wHost[c] = w[posBegin(sys) + L.posOfCell[c]];
 }
 
-void Batch::interfaceFlux(int sys, int iface, double *qa, double *qb)
+Batch::IfaceDev &Batch::ifaceSlot(int sys, int iface, int side)
 {
-    HIP_OK(hipSetDevice(device_));
     const CoupledSystem &S = *sys_.at(sys);
     const SystemLayout &L = *lay_.at(sys);
     const CoupledInterface &I = S.interfaces.at(iface);
     if (ifDev_.size() != sys_.size()) ifDev_.resize(sys_.size());
     if (ifDev_[sys].size() != S.interfaces.size()) ifDev_[sys].resize(S.interfaces.size());
-    const InterfaceSide *sides[2] = {&I.a, &I.b};
-    double out[2] = {0.0, 0.0};
-    DevBuf<double> dOut;
-    dOut.alloc(2);
-    for (int q = 0; q < 2; ++q) {
-        auto &slot = ifDev_[sys][iface][q];
-        const InterfaceSide &O = *sides[q], &Nb = *sides[1 - q];
-        if (!slot) {
-            slot.reset(new IfaceDev);
-            const int p0 = posBegin(sys);
-            const int oO = L.regionOffset[O.region], oN = L.regionOffset[Nb.region];
-            std::vector<int32_t> fcPos(O.nFaces()), pairPos(O.idx.size());
-            for (int f = 0; f < O.nFaces(); ++f) fcPos[f] = p0 + L.posOfCell[oO + O.fc[f]];
-            for (size_t k = 0; k < O.idx.size(); ++k) pairPos[k] = p0 + L.posOfCell[oN + Nb.fc[O.idx[k]]];
-            slot->nFaces = O.nFaces();
-            slot->fcPos.upload(fcPos, stream_);
-            slot->pairPos.upload(pairPos.empty() ? std::vector<int32_t>{0} : pairPos, stream_);
-            slot->off.upload(O.off, stream_);
-            slot->coeff.upload(O.coeff.empty() ? std::vector<double>{0.0} : O.coeff, stream_);
-            slot->w.upload(O.w.empty() ? std::vector<double>{0.0} : O.w, stream_);
-        }
-        if (hasModel(sys) && fv_->assembled && slot->nFaces > 0) {
-            // the coefficients assembled on the device
-            size_t off = fv_->sideBase[sys];
-            for (int i = 0; i < iface; ++i) off += S.interfaces[i].a.nFaces() + S.interfaces[i].b.nFaces();
-            if (q) off += I.a.nFaces();
-            HIP_OK(hipMemcpyAsync(slot->coeff.p, fv_->ifCoeff.p + off, slot->nFaces * sizeof(double),
-                                  hipMemcpyDeviceToDevice, stream_));
-        }
-        const int nBlk = (slot->nFaces + 255) / 256;
-        if ((size_t)nBlk * 8 > partials_.n) throw std::runtime_error("interface larger than the reduction buffer");
-        HIP_OK(launchInterfaceFlux(slot->nFaces, slot->fcPos.p, slot->coeff.p, slot->off.p, slot->pairPos.p,
-                                   slot->w.p, x64_.p, partials_.p, dOut.p + q, stream_));
+    auto &slot = ifDev_[sys][iface][side];
+    if (!slot) {
+        const InterfaceSide &O = side ? I.b : I.a, &Nb = side ? I.a : I.b;
+        slot.reset(new IfaceDev);
+        const int p0 = posBegin(sys);
+        const int oO = L.regionOffset[O.region], oN = L.regionOffset[Nb.region];
+        std::vector<int32_t> fcPos(O.nFaces()), pairPos(O.idx.size());
+        for (int f = 0; f < O.nFaces(); ++f) fcPos[f] = p0 + L.posOfCell[oO + O.fc[f]];
+        for (size_t k = 0; k < O.idx.size(); ++k) pairPos[k] = p0 + L.posOfCell[oN + Nb.fc[O.idx[k]]];
+        slot->nFaces = O.nFaces();
+        slot->fcPos.upload(fcPos.empty() ? std::vector<int32_t>{0} : fcPos, stream_);
+        slot->pairPos.upload(pairPos.empty() ? std::vector<int32_t>{0} : pairPos, stream_);
+        slot->off.upload(O.off, stream_);
+        slot->coeff.upload(O.coeff.empty() ? std::vector<double>{0.0} : O.coeff, stream_);
+        slot->w.upload(O.w.empty() ? std::vector<double>{0.0} : O.w, stream_);
+        slot->wall = !O.kOwn.empty() || O.nFaces() == 0;
+        slot->kOwn.upload(O.kOwn.empty() ? std::vector<double>{0.0} : O.kOwn, stream_);
+        slot->kNei.upload(O.kNei.empty() ? std::vector<double>{0.0} : O.kNei, stream_);
+        HIP_OK(hipStreamSynchronize(stream_));
     }
-    HIP_OK(hipMemcpyAsync(out, dOut.p, sizeof(out), hipMemcpyDeviceToHost, stream_));
+    return *slot;
+}
+
+void Batch::ifaceSync(int sys, int iface, int side, IfaceDev &slot)
+{
+    if (slot.nFaces == 0) return;
+    const CoupledSystem &S = *sys_.at(sys);
+    const bool fvNew = hasModel(sys) && fv_->assembled;
+    const bool rfNew = !fvNew && rf_ && rf_->changed[sys];
+    if (!fvNew && !rfNew) return;
+    size_t off = fvNew ? fv_->sideBase[sys] : rf_->sideBase[sys];
+    for (int i = 0; i < iface; ++i) off += S.interfaces[i].a.nFaces() + S.interfaces[i].b.nFaces();
+    if (side) off += S.interfaces[iface].a.nFaces();
+    const size_t n = slot.nFaces * sizeof(double);
+    HIP_OK(hipMemcpyAsync(slot.coeff.p, (fvNew ? fv_->ifCoeff.p : rf_->sideCoeff.p) + off, n,
+                          hipMemcpyDeviceToDevice, stream_));
+    if (fvNew) {   // the wall conductances follow k
+        HIP_OK(hipMemcpyAsync(slot.kOwn.p, fv_->sKOwn.p + off, n, hipMemcpyDeviceToDevice, stream_));
+        HIP_OK(hipMemcpyAsync(slot.kNei.p, fv_->sKNei.p + off, n, hipMemcpyDeviceToDevice, stream_));
+    }
+}
+
+void Batch::interfaceFlux(int sys, int iface, double *qa, double *qb)
+{
+    HIP_OK(hipSetDevice(device_));
+    (void)ifaceSlot(sys, iface, 0);
+    (void)ifaceSlot(sys, iface, 1);
+    // reduction partials of up to one per 256 positions of the batch, then
+    // the two results
+    fluxBuf_.alloc(partials_.n + 2);
+    interfaceFluxScratch(sys, iface, qa, qb, fluxBuf_.p, fluxBuf_.n);
+}
+
+void Batch::interfaceFluxScratch(int sys, int iface, double *qa, double *qb, double *scratch, size_t nScratch)
+{
+    HIP_OK(hipSetDevice(device_));
+    applyRefresh();
+    const CoupledSystem &S = *sys_.at(sys);
+    if (iface < 0 || iface >= (int)S.interfaces.size()) throw std::out_of_range("interface index");
+    if (ifDev_.size() != sys_.size() || ifDev_[sys].size() != S.interfaces.size() || !ifDev_[sys][iface][0] ||
+        !ifDev_[sys][iface][1])
+        throw std::runtime_error("interface face lists are not resident");
+    // the two results sit after the partials
+    if (nScratch < 2) throw std::runtime_error("flux scratch too small");
+    double *out = scratch + nScratch - 2;
+    for (int q = 0; q < 2; ++q) {
+        IfaceDev &slot = *ifDev_[sys][iface][q];
+        ifaceSync(sys, iface, q, slot);
+        const size_t nBlk = ((size_t)slot.nFaces + 255) / 256;
+        if (nBlk * 8 > nScratch - 2)
+            throw std::runtime_error("interface side of " + std::to_string(slot.nFaces) +
+                                     " faces exceeds the reduction scratch (at most " +
+                                     std::to_string((nScratch - 2) / 8 * 256) + " faces)");
+        HIP_OK(launchInterfaceFlux(slot.nFaces, slot.fcPos.p, slot.coeff.p, slot.off.p, slot.pairPos.p, slot.w.p,
+                                   x64_.p, scratch, out + q, stream_));
+    }
+    double h[2] = {0.0, 0.0};
+    HIP_OK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    *qa = out[0];
-    *qb = out[1];
+    *qa = h[0];
+    *qb = h[1];
+}
+
+void Batch::interfacePatchValues(int sys, int iface, int side, double *tn, double *tw)
+{
+    HIP_OK(hipSetDevice(device_));
+    applyRefresh();
+    if (side < 0 || side > 1) throw std::out_of_range("interface side");
+    IfaceDev &slot = ifaceSlot(sys, iface, side);
+    if (!slot.wall)
+        throw std::runtime_error("interface " + std::to_string(iface) + " has no wall conductances "
+                                 "(mrfg_set_interface_wall)");
+    ifaceSync(sys, iface, side, slot);
+    if (slot.nFaces == 0) return;
+    DevBuf<double> out;
+    out.alloc(2 * (size_t)slot.nFaces);
+    HIP_OK(launchInterfacePatch(slot.nFaces, slot.fcPos.p, slot.off.p, slot.pairPos.p, slot.w.p, slot.kOwn.p,
+                                slot.kNei.p, x64_.p, out.p, out.p + slot.nFaces, stream_));
+    HIP_OK(hipMemcpyAsync(tn, out.p, slot.nFaces * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipMemcpyAsync(tw, out.p + slot.nFaces, slot.nFaces * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Batch::setInterfaceWall(int sys, int iface, int side, const double *kOwn, const double *kNei)
+{
+    HIP_OK(hipSetDevice(device_));
+    if (side < 0 || side > 1) throw std::out_of_range("interface side");
+    IfaceDev &slot = ifaceSlot(sys, iface, side);
+    if (!kOwn || !kNei) throw std::invalid_argument("null wall conductances");
+    if (slot.nFaces == 0) return;
+    std::vector<double> a(kOwn, kOwn + slot.nFaces), b(kNei, kNei + slot.nFaces);
+    slot.kOwn.upload(a, stream_);
+    slot.kNei.upload(b, stream_);
+    HIP_OK(hipStreamSynchronize(stream_));
+    slot.wall = true;
 }
 
 void Batch::precondition32(const float *rHost, float *wHost, int sys)
 {
+    applyRefresh();
     ensureFloat();
     factorise();
     const SystemLayout &L = *lay_.at(sys);
@@ -1016,6 +1119,7 @@ void Batch::precondition32(const float *rHost, float *wHost, int sys)
 double Batch::timeKernel(int which, int reps)
 {
     HIP_OK(hipSetDevice(device_));
+    applyRefresh();
     if (which == 7 || which == 8) return timeAssembly(which, reps);
     factorise();
     markAllRunning();
@@ -1081,6 +1185,7 @@ double Batch::timeKernel(int which, int reps)
 int Batch::traceDilu(long long *out, int maxWaves)
 {
     HIP_OK(hipSetDevice(device_));
+    applyRefresh();
     factorise();
     markAllRunning();
     Krylov<double> K;

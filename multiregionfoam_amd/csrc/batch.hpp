@@ -88,9 +88,22 @@ struct FvBatch {
     DevBuf<int32_t> cellRegion, faceRegion, own, nei, cfOff, cf, cbOff, bSide, sCell, sOff, sPairCell;
     DevBuf<int8_t> bT, bU, bKFixed;
     DevBuf<double> w, dc, magSf, Sf, V, k, U, T, bNDist, bMagSf, bSf, bTb, bUb, bKb, sNDist, sW, sPairNDist;
-    DevBuf<double> lapU, aLo, aUp, faceVal, sK, sDelta, ifCoeff, diag, source, ddtA, diagBase, srcBase;
+    DevBuf<double> lapU, aLo, aUp, faceVal, sK, sDelta, sKOwn, sKNei, ifCoeff, diag, source, ddtA, diagBase, srcBase;
     DevBuf<int32_t> cellOfPos, ellSrc, fSrc, bSrc, ifSrc;
     bool assembled = false;
+};
+
+// Value refresh of systems defined from LDU arrays: the values a drop-in
+// caller hands over per solve (diag, lower / upper, source per region;
+// interface coefficients per side), numbered batch-wide, and the maps the
+// placement pass gathers them through (built on the first refresh).
+struct RefreshMaps {
+    std::vector<int> cellBase, faceBase, sideBase;   // per system
+    int nCells = 0, nFaces = 0, nSides = 0;
+    DevBuf<int32_t> cellOfPos, ellSrc, fSrc, bSrc, ifSrc;
+    DevBuf<double> faceVal, sideCoeff, diag, source;
+    std::vector<char> dirty;     // per system: staged values not yet placed
+    std::vector<char> changed;   // per system: values differ from the committed ones
 };
 
 class Batch {
@@ -136,6 +149,19 @@ public:
     // current FP64 field, on the device: Q = sum coeff*(T_neighbour - T_own)
     // per side (a conserving interface gives qa + qb = 0).
     void interfaceFlux(int sys, int iface, double *qa, double *qb);
+    // The same on a caller-given device scratch (nScratch doubles: the
+    // reduction partials, then the two results); allocates nothing.  The
+    // interface's face lists must be resident (setSystems uploads them).
+    void interfaceFluxScratch(int sys, int iface, double *qa, double *qb, double *scratch, size_t nScratch);
+    // monolithicTemperature's patch values of one side (0 = a, 1 = b) from
+    // the current FP64 field, on the device: the neighbour field tn (the GGI
+    // interpolate of the other side's face-cell values,
+    // monolithicCouplingFvPatchField::patchNeighbourField) and the wall value
+    // tw = (kOwn To + kNei tn) / (kOwn + kNei) (calcTemperature, radiation
+    // off).  Needs the side's wall conductances.
+    void interfacePatchValues(int sys, int iface, int side, double *tn, double *tw);
+    // Replace the wall conductances of one interface side (after commit)
+    void setInterfaceWall(int sys, int iface, int side, const double *kOwn, const double *kNei);
 
     // Time `reps` repetitions of one kernel on resident data; returns the
     // average device time per launch (ms), measured with HIP events on the
@@ -161,6 +187,18 @@ public:
     double bytesIter(int sys, int sv) const;
     double bytesSpmv(int sys, int sv) const;
     double bytesDilu(int sys, int sv) const;
+
+    // Value refresh on the committed structure (systems defined from LDU
+    // arrays, steady): new values of one region (lower == nullptr: symmetric,
+    // the upper values) or of one interface's coefficients (a null side is
+    // left unchanged).  They are staged on the device and put in place --
+    // SpMV values, DILU records and products, diagonal, source, row sums,
+    // interface coefficients -- before the next solve or kernel call; no
+    // layout is rebuilt.  applyRefresh() places what is staged.
+    void updateRegion(int sys, int region, const double *diag, const double *lower, const double *upper,
+                      const double *source);
+    void updateInterface(int sys, int iface, const double *coeffA, const double *coeffB);
+    void applyRefresh();
 
     // Force the level schedule for every region (general-mesh path); applies
     // at the next setSystems.
@@ -250,11 +288,19 @@ private:
     // per system, per interface, per side: the face lists in batch positions
     struct IfaceDev {
         int nFaces = 0;
+        bool wall = false;   // kOwn / kNei known
         DevBuf<int32_t> fcPos, off, pairPos;
-        DevBuf<double> coeff, w;
+        DevBuf<double> coeff, w, kOwn, kNei;
     };
     std::vector<std::vector<std::array<std::unique_ptr<IfaceDev>, 2>>> ifDev_;
+    // the face lists of one interface side in batch positions, uploaded on
+    // first use (every side is uploaded at setSystems)
+    IfaceDev &ifaceSlot(int sys, int iface, int side);
+    // copy coefficients (and wall conductances) the device changed since
+    // the commit -- assembled from fields, or refreshed -- into the slot
+    void ifaceSync(int sys, int iface, int side, IfaceDev &slot);
     DevBuf<double> partials_;
+    DevBuf<double> fluxBuf_;   // interface-flux check: partials and results
     // time stepping (position space): ddt coefficient, diagonal and source
     // without the ddt term, previous field
     DevBuf<double> ddtA_, diagBase_, srcBase_, xPrev_;
@@ -263,6 +309,8 @@ private:
     double diagCoefft_ = 1.0;
     std::vector<SysInfo> hInfo_;
     std::unique_ptr<FvBatch> fv_;
+    std::unique_ptr<RefreshMaps> rf_;
+    void buildRefresh();
     bool hostStale_ = false;
 };
 

@@ -257,7 +257,7 @@ void Batch::buildFv()
         pr.first->upload(*pr.second, st);
     for (auto *b : {&F->lapU, &F->aLo, &F->aUp}) b->alloc(std::max(1, F->nFaces));
     F->faceVal.alloc(2 * (size_t)std::max(1, F->nFaces));
-    for (auto *b : {&F->sK, &F->sDelta, &F->ifCoeff}) b->alloc(std::max(1, F->nSideFaces));
+    for (auto *b : {&F->sK, &F->sDelta, &F->sKOwn, &F->sKNei, &F->ifCoeff}) b->alloc(std::max(1, F->nSideFaces));
     for (auto *b : {&F->diag, &F->source, &F->ddtA, &F->diagBase, &F->srcBase}) b->alloc(std::max(1, F->nCells));
     F->cellOfPos.upload(cellOfPos, st);
     F->ellSrc.upload(ellSrc.empty() ? std::vector<int32_t>{-1} : ellSrc, st);
@@ -360,6 +360,8 @@ FvDev Batch::fvDevView() const
     m.faceVal = F.faceVal.p;
     m.sK = F.sK.p;
     m.sDelta = F.sDelta.p;
+    m.sKOwn = F.sKOwn.p;
+    m.sKNei = F.sKNei.p;
     m.ifCoeff = F.ifCoeff.p;
     m.diag = F.diag.p;
     m.source = F.source.p;
@@ -374,6 +376,7 @@ FvPlace Batch::fvPlaceView()
     const FvBatch &F = *fv_;
     FvPlace q;
     q.nPos = nPos_;
+    q.pos0 = 0;
     q.K = K_;
     q.G = packChunks(K_ + 1, 2);
     q.cellOfPos = F.cellOfPos.p;

@@ -5,6 +5,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "batch.hpp"
@@ -253,6 +254,8 @@ int mrfg_set_interface(mrfg_ctx *ctx, int sys, int iface, int region_a, int regi
             s.fc.assign(fc, fc + n);
             s.coeff.assign(coeff, coeff + n);
             s.magSf.assign(n, 0.0);
+            s.kOwn.clear();   // wall conductances: mrfg_set_interface_wall
+            s.kNei.clear();
             s.off.assign(off, off + n + 1);
             int np = s.off.back();
             if (np < 0 || s.off.front() != 0) throw ArgError("bad GGI offsets");
@@ -296,6 +299,41 @@ int mrfg_commit(mrfg_ctx *ctx)
         ctx->dna.reset();
         ctx->batch->setSystems(ptrs);
         ctx->committed = true;
+    });
+}
+
+int mrfg_update_region(mrfg_ctx *ctx, int sys, int region, const double *diag, const double *lower,
+                       const double *upper, const double *source)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::Batch &B = committedBatch(ctx);
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (region < 0 || region >= (int)S.regions.size()) throw ArgError("region index out of range");
+        B.updateRegion(sys, region, diag, lower, upper, source);
+        // the definition follows, so a later commit keeps these values
+        mrfg::LduRegion &R = S.regions[region];
+        R.diag.assign(diag, diag + R.nCells);
+        R.source.assign(source, source + R.nCells);
+        if (R.nFaces > 0) {
+            R.upper.assign(upper, upper + R.nFaces);
+            if (lower) R.lower.assign(lower, lower + R.nFaces);
+            else R.lower = R.upper;
+        }
+    });
+}
+
+int mrfg_update_interface(mrfg_ctx *ctx, int sys, int iface, const double *coeff_a, const double *coeff_b)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::Batch &B = committedBatch(ctx);
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (iface < 0 || iface >= (int)S.interfaces.size()) throw ArgError("interface index out of range");
+        B.updateInterface(sys, iface, coeff_a, coeff_b);
+        mrfg::CoupledInterface &I = S.interfaces[iface];
+        if (coeff_a) I.a.coeff.assign(coeff_a, coeff_a + I.a.nFaces());
+        if (coeff_b) I.b.coeff.assign(coeff_b, coeff_b + I.b.nFaces());
     });
 }
 
@@ -631,6 +669,19 @@ int mrfg_case_interface(const mrfg_case *c, int iface, int32_t *fc_a, double *co
     return MRFG_OK;
 }
 
+int mrfg_case_interface_wall(const mrfg_case *c, int iface, double *k_own_a, double *k_nei_a, double *k_own_b,
+                             double *k_nei_b)
+{
+    if (!c || iface < 0 || iface >= (int)c->sys.interfaces.size()) return MRFG_ERR_ARG;
+    const auto &I = c->sys.interfaces[iface];
+    if (I.a.kOwn.empty() && I.a.nFaces() > 0) return MRFG_ERR_STATE;
+    auto cp = [](double *dst, const std::vector<double> &src) {
+        if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(double));
+    };
+    cp(k_own_a, I.a.kOwn); cp(k_nei_a, I.a.kNei); cp(k_own_b, I.b.kOwn); cp(k_nei_b, I.b.kNei);
+    return MRFG_OK;
+}
+
 int mrfg_add_case(mrfg_ctx *ctx, const mrfg_case *c, int *sys_id)
 {
     if (!ctx || !c) return MRFG_ERR_ARG;
@@ -679,6 +730,47 @@ int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *
         if (iface < 0 || iface >= (int)ctx->systems[sys]->interfaces.size())
             throw std::invalid_argument("interface index out of range");
         B.interfaceFlux(sys, iface, q_a, q_b);
+    });
+}
+
+int mrfg_set_interface_wall(mrfg_ctx *ctx, int sys, int iface, const double *k_own_a, const double *k_nei_a,
+                            const double *k_own_b, const double *k_nei_b)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        mrfg::CoupledSystem &S = systemOf(ctx, sys);
+        if (iface < 0 || iface >= (int)S.interfaces.size()) throw ArgError("interface index out of range");
+        mrfg::CoupledInterface &I = S.interfaces[iface];
+        if (I.a.region < 0) throw StateError("set the interface before its wall conductances");
+        const bool na = I.a.nFaces() > 0, nb = I.b.nFaces() > 0;
+        if ((na && (!k_own_a || !k_nei_a)) || (nb && (!k_own_b || !k_nei_b)))
+            throw ArgError("null wall conductance array");
+        for (auto side : {std::make_tuple(&I.a, k_own_a, k_nei_a), std::make_tuple(&I.b, k_own_b, k_nei_b)}) {
+            mrfg::InterfaceSide &IS = *std::get<0>(side);
+            const int n = IS.nFaces();
+            for (int f = 0; f < n; ++f)
+                if (!(std::get<1>(side)[f] > 0.0) || !(std::get<2>(side)[f] > 0.0))
+                    throw ArgError("wall conductances must be positive");
+            IS.kOwn.assign(std::get<1>(side), std::get<1>(side) + n);
+            IS.kNei.assign(std::get<2>(side), std::get<2>(side) + n);
+        }
+        if (ctx->committed && ctx->batch) {
+            ctx->batch->setInterfaceWall(sys, iface, 0, I.a.kOwn.data(), I.a.kNei.data());
+            ctx->batch->setInterfaceWall(sys, iface, 1, I.b.kOwn.data(), I.b.kNei.data());
+        }
+    });
+}
+
+int mrfg_interface_patch_values(mrfg_ctx *ctx, int sys, int iface, int side, double *t_neighbour, double *t_wall)
+{
+    if (!ctx || !t_neighbour || !t_wall) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        mrfg::Batch &B = committedBatch(ctx);
+        if (iface < 0 || iface >= (int)ctx->systems[sys]->interfaces.size())
+            throw ArgError("interface index out of range");
+        if (side < 0 || side > 1) throw ArgError("side must be 0 (a) or 1 (b)");
+        B.interfacePatchValues(sys, iface, side, t_neighbour, t_wall);
     });
 }
 

@@ -461,6 +461,8 @@ void calcThermalDiffusivity(const RegionCase &own, int sOwn, const RegionCase &n
     kface.resize(nf);
     S.fc.resize(nf);
     S.magSf.assign(nf, mo.area[a]);
+    S.kOwn.resize(nf);
+    S.kNei.resize(nf);
     int idx[3];
     for (int f = 0; f < nf; ++f) {
         int cO = mo.sideFaceCell(sOwn, f, idx);
@@ -471,6 +473,8 @@ void calcThermalDiffusivity(const RegionCase &own, int sOwn, const RegionCase &n
             fNei += S.w[p] * nei.k[mn.sideFaceCell(sNei, S.idx[p], idx)];
         double kOwn = fOwn / (1.0 - wP) / magDelta;
         double kNei = fNei / wP / magDelta;
+        S.kOwn[f] = kOwn;
+        S.kNei[f] = kNei;
         double k = kOwn * kNei;
         k /= delta * (kOwn + kNei);
         double kHarm = 1.0 / (wP / fOwn + (1.0 - wP) / fNei);
@@ -601,6 +605,8 @@ void CoupledSystem::check() const
             if ((int)S.coeff.size() != S.nFaces() || (int)S.off.size() != S.nFaces() + 1 ||
                 S.idx.size() != S.w.size() || S.off.back() != (int32_t)S.w.size())
                 throw std::runtime_error("interface '" + I.name + "': inconsistent GGI addressing");
+            if (S.kOwn.size() != S.kNei.size() || (!S.kOwn.empty() && (int)S.kOwn.size() != S.nFaces()))
+                throw std::runtime_error("interface '" + I.name + "': wall conductances do not match the faces");
             for (int32_t c : S.fc)
                 if (c < 0 || c >= nC) throw std::runtime_error("interface '" + I.name + "': face cell out of range");
             for (int32_t j : S.idx)

@@ -37,6 +37,12 @@ struct InterfaceSide {
     std::vector<double> magSf;
     std::vector<int32_t> off, idx;
     std::vector<double> w;
+    // per face, the two conductances of monolithicTemperature's wall value
+    // (calcTemperature, monolithicThermalDiffusivityFvPatchScalarField.C:
+    // 300-380, radiation off): kOwn = forig / (1 - w) / |delta| of this
+    // face (monolithicBase::korig) and kNei = the GGI-interpolated other
+    // side's forig / w / |delta|; empty when the caller gave none
+    std::vector<double> kOwn, kNei;
     int nFaces() const { return (int)fc.size(); }
 };
 

@@ -15,10 +15,13 @@ struct SysState {
     double tol, relTol;
     double red0, red1, red2, red3;   // last finalised reduction values (diagnostics)
     int nIter, minIter, maxIter, status;
-    int nCells, pad0, pad1, pad2;
+    int nCells;
+    int sZero;   // BiCGStab: s = r - alpha*v vanished in the last iteration (omega taken as 0)
+    int pad1, pad2;
 };
 
-enum SysStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kSingular = 3, kIdle = 4 };
+// kDiverged: a residual (or the normFactor it is scaled by) is not finite.
+enum SysStatus { kRunning = 0, kConverged = 1, kMaxIter = 2, kSingular = 3, kIdle = 4, kDiverged = 5 };
 
 // Block of up to 4 slices of one system (256 threads).
 struct BlockEntry {

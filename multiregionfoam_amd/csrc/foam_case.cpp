@@ -781,6 +781,8 @@ void calcThermalDiffusivity(FoamRegion &own, int pOwn, const FoamRegion &nei, in
     delta.resize(Po.size);
     S.fc.resize(Po.size);
     S.magSf.resize(Po.size);
+    S.kOwn.resize(Po.size);
+    S.kNei.resize(Po.size);
     for (int i = 0; i < Po.size; ++i) {
         const int f = Po.start + i;
         const int cO = mo.owner[f];
@@ -801,6 +803,8 @@ void calcThermalDiffusivity(FoamRegion &own, int pOwn, const FoamRegion &nei, in
         delta[i] = 1.0 / magDelta;           // patch deltaCoeffs
         double kOwn = fOwn / (1.0 - wP) / magDelta;
         double kNei = fNei / wP / magDelta;
+        S.kOwn[i] = kOwn;
+        S.kNei[i] = kNei;
         double k = kOwn * kNei;
         k /= delta[i] * (kOwn + kNei);
         double kHarm = 1.0 / (wP / fOwn + (1.0 - wP) / fNei);

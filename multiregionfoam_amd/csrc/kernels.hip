@@ -50,9 +50,12 @@ __device__ void blockPartial(double (&v)[NV], double *partials)
         }
 }
 
-// lduSolver::stop with SolverPerformance::checkConvergence
+// lduSolver::stop with SolverPerformance::checkConvergence.  A residual
+// that is not finite (NaN or Inf in the system, the field or the Krylov
+// scalars) stops the system at once as diverged, so it releases its batch.
 __device__ inline int stopStatus(const SysState &s)
 {
+    if (!isfinite(s.finalRes)) return kDiverged;
     if (s.nIter < s.minIter) return kRunning;
     bool conv = s.finalRes < s.tol || (s.relTol > kSmall && s.finalRes < s.relTol * s.initRes);
     if (conv) return kConverged;
@@ -98,13 +101,18 @@ __device__ inline void applyStage(int stage, SysState &S, const double *out)
     case kFinOmega:
         S.red0 = out[0];
         S.red1 = out[1];
-        if (out[1] == 0.0) S.status = kSingular;
-        else S.omega = out[0] / out[1];
+        // t . t == 0: t = A M^-1 s vanished, so s = r - alpha*v did and
+        // x + alpha*ph solves the system.  The step is taken with omega = 0
+        // (x += alpha*ph, r = s) and the stop test decides; if it does not
+        // stop, the system is singular (omega = 0 cannot continue).
+        S.sZero = out[1] == 0.0;
+        S.omega = S.sZero ? 0.0 : out[0] / out[1];
         break;
     case kFinResBicg:
         S.finalRes = out[0] / S.normFactor;
         S.nIter++;
         S.status = stopStatus(S);
+        if (S.status == kRunning && S.sZero) S.status = kSingular;
         S.rhoOld = S.rho;
         S.rho = out[1];
         S.beta = S.rho / S.rhoOld * (S.alpha / S.omega);
@@ -192,6 +200,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(int stage, const SysInfo
         G.red1 = S.red1;
         G.nIter = S.nIter;
         G.status = S.status;
+        G.sZero = S.sZero;
     }
 }
 
@@ -728,6 +737,36 @@ hipError_t launchInterfaceFlux(int nFaces, const int32_t *fcPos, const double *c
     if (nBlk > 0)
         k_iface_flux<<<nBlk, kBlock, 0, st>>>(nFaces, fcPos, coeff, off, pairPos, w, x, partials);
     k_iface_flux_sum<<<1, kBlock, 0, st>>>(nBlk, partials, out);
+    MRFG_CHECK_LAUNCH();
+}
+
+// monolithicTemperature's patch values on one side of an interface: the
+// neighbour field tn = sum_p w_p x[pos_p] (patchNeighbourField: the GGI
+// interpolate of the other side's face-cell values,
+// monolithicCouplingFvPatchField.C:208-236) and the wall value
+// tw = (kOwn To + kNei tn) / (kOwn + kNei) (calcTemperature with Qr = 0,
+// monolithicThermalDiffusivityFvPatchScalarField.C:300-380), one thread per
+// face.
+__global__ __launch_bounds__(kBlock) void k_iface_patch(int nFaces, const int32_t *fcPos, const int32_t *off,
+                                                      const int32_t *pairPos, const double *w, const double *kOwn,
+                                                      const double *kNei, const double *x, double *tn, double *tw)
+{
+    const int f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nFaces) return;
+    double t = 0.0;
+    for (int q = off[f]; q < off[f + 1]; ++q) t += w[q] * x[pairPos[q]];
+    const double to = x[fcPos[f]];
+    tn[f] = t;
+    tw[f] = (kOwn[f] * to + kNei[f] * t) / (kOwn[f] + kNei[f]);
+}
+
+hipError_t launchInterfacePatch(int nFaces, const int32_t *fcPos, const int32_t *off, const int32_t *pairPos,
+                                const double *w, const double *kOwn, const double *kNei, const double *x,
+                                double *tn, double *tw, hipStream_t st)
+{
+    if (nFaces > 0)
+        k_iface_patch<<<(nFaces + kBlock - 1) / kBlock, kBlock, 0, st>>>(nFaces, fcPos, off, pairPos, w, kOwn, kNei,
+                                                                         x, tn, tw);
     MRFG_CHECK_LAUNCH();
 }
 

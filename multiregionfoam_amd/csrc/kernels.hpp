@@ -49,6 +49,12 @@ hipError_t launchInterfaceFlux(int nFaces, const int32_t *fcPos, const double *c
                                const int32_t *pairPos, const double *w, const double *x, double *partials,
                                double *out, hipStream_t st);
 
+// Patch values of one interface side (neighbour field and monolithicTemperature
+// wall value), one per face.
+hipError_t launchInterfacePatch(int nFaces, const int32_t *fcPos, const int32_t *off, const int32_t *pairPos,
+                                const double *w, const double *kOwn, const double *kNei, const double *x,
+                                double *tn, double *tw, hipStream_t st);
+
 // Average time (ms) of a plain device-to-device copy of `bytes` (reads and
 // writes `bytes` per launch), for calibrating HBM traffic counters.
 hipError_t copyProbe(int device, size_t bytes, int reps, double *ms);

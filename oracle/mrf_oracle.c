@@ -731,6 +731,51 @@ void orc_interface_flux(const orc_system *sys, int i, double *const *psi,
     }
 }
 
+/* Coupled patch values of one side of interface i from the field psi, as
+   correctBoundaryConditions leaves them on the monolithicTemperature
+   patches (box-built systems: k and cell sizes known).  tn: the patch
+   neighbour field, the GGI interpolate of the other side's face-cell values
+   (monolithicCouplingFvPatchField.C:208-236).  tw: the patch value
+   calcTemperature gives with radiation off
+   (monolithicThermalDiffusivityFvPatchScalarField.C:300-380, called from
+   monolithicTemperatureFvPatchScalarField.C:147-202):
+   Tw = (kOwn*TcOwn + kNei*TcNei)/(kOwn + kNei), kOwn = forig/(1-w)/|d|,
+   kNei = interpolate(forig_nei)/w/|d|. */
+void orc_interface_patch_values(const orc_system *sys, int i, int side, double *const *psi,
+                                double *tn, double *tw)
+{
+    const orc_interface *I = &sys->interfaces[i];
+    int n = side ? I->nB : I->nA;
+    const int32_t *off = side ? I->offB : I->offA, *idx = side ? I->idxB : I->idxA;
+    const double *w = side ? I->wB : I->wA;
+    const int32_t *fc = side ? I->fcB : I->fcA, *fcN = side ? I->fcA : I->fcB;
+    const double *magSf = side ? I->magSfB : I->magSfA;
+    const orc_ldu *LO = &sys->regions[side ? I->regionB : I->regionA];
+    const orc_ldu *LN = &sys->regions[side ? I->regionA : I->regionB];
+    const double *xO = psi[side ? I->regionB : I->regionA];
+    const double *xN = psi[side ? I->regionA : I->regionB];
+    /* the normal axis, as in orc_interface_flux */
+    int a = 0;
+    for (int t = 0; t < 3; ++t) {
+        double ar = 1.0;
+        for (int u = 0; u < 3; ++u) if (u != t) ar *= LO->cellSize[u];
+        if (n > 0 && fabs(ar - magSf[0]) <= 1e-12 * magSf[0]) a = t;
+    }
+    double dOwn = 0.5 * LO->cellSize[a], dNei = 0.5 * LN->cellSize[a];
+    double magDelta = dOwn + dNei, wP = dNei / magDelta;
+    for (int f = 0; f < n; ++f) {
+        double Tn = 0.0, kn = 0.0;
+        for (int p = off[f]; p < off[f + 1]; ++p) {
+            Tn += w[p] * xN[fcN[idx[p]]];
+            kn += w[p] * LN->k[fcN[idx[p]]];
+        }
+        double kOwn = LO->k[fc[f]] / (1.0 - wP) / magDelta;
+        double kNei = kn / wP / magDelta;
+        tn[f] = Tn;
+        tw[f] = (kOwn * xO[fc[f]] + kNei * Tn) / (kOwn + kNei);
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Krylov solvers                                                             */
 /* ------------------------------------------------------------------------- */
@@ -777,9 +822,17 @@ static double ff_summag(const ffield *a)
     return csum_val(s);
 }
 
-/* lduSolver::stop + SolverPerformance::checkConvergence */
+/* lduSolver::stop + SolverPerformance::checkConvergence.  A residual that
+   is not finite (NaN / Inf in the system, the field or the Krylov scalars)
+   stops the solve at once, flagged diverged (singular = 2); the reference
+   has no such test and would iterate to maxIter. */
 static int solver_stop(orc_perf *p, double tol, double relTol, int minIter, int maxIter)
 {
+    if (!isfinite(p->finalResidual)) {
+        p->converged = 0;
+        p->singular = 2;
+        return 1;
+    }
     if (p->nIterations < minIter) return 0;
     int conv = p->finalResidual < tol
             || (relTol > 1e-20 && p->finalResidual < relTol * p->initialResidual);
@@ -830,8 +883,12 @@ static void bicgstab(orc_system *sys, const ffield *x, const ffield *b, double *
             orc_dilu_precondition(sys, rD, sh.v, s.v);
             orc_amul(sys, sh.v, t.v);
             double ts = ff_sumprod(&t, &s), tt = ff_sumprod(&t, &t);
-            if (tt == 0.0) { perf->singular = 1; break; }
-            omega = ts / tt;
+            /* t.t == 0: t = A M^-1 s vanished, so s did and x + alpha*ph
+               solves the system.  The step is taken with omega = 0
+               (x += alpha*ph, r = s) and the stop test decides; if it does
+               not stop, the solve cannot continue (singular). */
+            int sZero = tt == 0.0;
+            omega = sZero ? 0.0 : ts / tt;
             for (int g = 0; g < r.nR; ++g)
                 for (int c = 0; c < r.n[g]; ++c) {
                     x->v[g][c] = x->v[g][c] + alpha * ph.v[g][c] + omega * sh.v[g][c];
@@ -839,7 +896,9 @@ static void bicgstab(orc_system *sys, const ffield *x, const ffield *b, double *
                 }
             perf->finalResidual = ff_summag(&r) / nf;
             perf->nIterations++;
-        } while (!solver_stop(perf, tol, relTol, minIter, maxIter));
+            if (solver_stop(perf, tol, relTol, minIter, maxIter)) break;
+            if (sZero) { perf->singular = 1; break; }
+        } while (1);
         ff_free(&ph); ff_free(&v); ff_free(&s); ff_free(&sh); ff_free(&t); ff_free(&rw);
     }
     ff_free(&p); ff_free(&r); ff_free(&tmp);

@@ -139,6 +139,11 @@ void orc_dilu_precondition(const orc_system *sys, double *const *rD,
 void orc_interface_flux(const orc_system *sys, int i, double *const *psi,
                         double *QA, double *QB, double *srcA, double *srcB);
 
+/* Patch values of one side (0 = A, 1 = B) of interface i: the neighbour
+   field tn and the monolithicTemperature wall value tw, per face. */
+void orc_interface_patch_values(const orc_system *sys, int i, int side, double *const *psi,
+                                double *tn, double *tw);
+
 /* Solve A psi = source for all regions; psi is in/out (sys->regions[r].psi). */
 int orc_solve(orc_system *sys, int solver, double tol, double relTol,
               int minIter, int maxIter, orc_perf *perf);

@@ -233,6 +233,11 @@ static void mt_precondition(const orc_system *sys, const mt_region *M, double *c
 
 static int mt_stop(orc_perf *p, double tol, double relTol, int minIter, int maxIter)
 {
+    if (!isfinite(p->finalResidual)) {   /* diverged, as the sequential solve */
+        p->converged = 0;
+        p->singular = 2;
+        return 1;
+    }
     if (p->nIterations < minIter) return 0;
     int conv = p->finalResidual < tol || (relTol > 1e-20 && p->finalResidual < relTol * p->initialResidual);
     p->converged = conv;
@@ -316,11 +321,10 @@ int orc_solve_mt(orc_system *sys, double tol, double relTol, int minIter, int ma
             double ts, tt;
             MT_SUM(ts, t[g][c] * s[g][c]);
             MT_SUM(tt, t[g][c] * t[g][c]);
-            if (tt == 0.0) {
-                perf->singular = 1;
-                break;
-            }
-            omega = ts / tt;
+            /* s vanished: take the step with omega = 0, as the sequential
+               solve does */
+            const int sZero = tt == 0.0;
+            omega = sZero ? 0.0 : ts / tt;
             for (int g = 0; g < nR; ++g)
 #pragma omp parallel for schedule(static)
                 for (int c = 0; c < M[g].n; ++c) {
@@ -330,7 +334,12 @@ int orc_solve_mt(orc_system *sys, double tol, double relTol, int minIter, int ma
             MT_SUM(sr, fabs(r[g][c]));
             perf->finalResidual = sr / nf;
             perf->nIterations++;
-        } while (!mt_stop(perf, tol, relTol, minIter, maxIter));
+            if (mt_stop(perf, tol, relTol, minIter, maxIter)) break;
+            if (sZero) {
+                perf->singular = 1;
+                break;
+            }
+        } while (1);
     }
     vec_free(sys, p); vec_free(sys, r); vec_free(sys, tmp); vec_free(sys, ph); vec_free(sys, v);
     vec_free(sys, s); vec_free(sys, sh); vec_free(sys, t); vec_free(sys, rw); vec_free(sys, rD);

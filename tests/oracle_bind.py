@@ -99,6 +99,8 @@ def lib():
         L.orc_dilu_factor.argtypes = [C.POINTER(System), PP]
         L.orc_dilu_precondition.argtypes = [C.POINTER(System), PP, PP, PP]
         L.orc_interface_flux.argtypes = [C.POINTER(System), C.c_int, PP] + [C.POINTER(C.c_double)] * 4
+        L.orc_interface_patch_values.argtypes = [C.POINTER(System), C.c_int, C.c_int, PP,
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_solve_mt.argtypes = [C.POINTER(System), C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(Perf)]
         L.orc_threads.restype = C.c_int
         L.orc_u01.argtypes = [C.c_uint64, C.c_uint64]
@@ -288,6 +290,16 @@ class OracleSystem:
         lib().orc_interface_flux(C.byref(self.sys), i, pp, *[C.byref(o) for o in out])
         return tuple(o.value for o in out)
 
+    def interface_patch_values(self, i, side, psi):
+        """(tn, tw) of side 0 (A) / 1 (B) of interface i: the GGI neighbour
+        field and the monolithicTemperature wall value per face."""
+        pa, pp = self._pp(psi)
+        n = self.interfaces[i]["nB" if side else "nA"]
+        tn, tw = np.zeros(n), np.zeros(n)
+        lib().orc_interface_patch_values(C.byref(self.sys), i, side, pp, tn.ctypes.data_as(C.POINTER(C.c_double)),
+                                         tw.ctypes.data_as(C.POINTER(C.c_double)))
+        return tn, tw
+
     def solve(self, solver="BiCGStab", tol=1e-10, relTol=0.0, minIter=0, maxIter=1000, psi0=None):
         """Solve in place from psi0 (default: the case's T0); returns
         (psi list, perf dict)."""
@@ -302,7 +314,7 @@ class OracleSystem:
         psi = [_arr(s.regions[r].psi, s.regions[r].nCells, np.float64) for r in range(s.nRegions)]
         return psi, dict(initialResidual=perf.initialResidual, finalResidual=perf.finalResidual,
                          nIterations=perf.nIterations, converged=bool(perf.converged),
-                         singular=bool(perf.singular))
+                         singular=perf.singular == 1, diverged=perf.singular == 2)
 
     def set_values(self, r, diag=None, source=None):
         """Overwrite region r's diagonal and / or source (in the arrays the
@@ -352,7 +364,7 @@ class OracleSystem:
         psi = [_arr(s.regions[r].psi, s.regions[r].nCells, np.float64) for r in range(s.nRegions)]
         return psi, dict(initialResidual=perf.initialResidual, finalResidual=perf.finalResidual,
                          nIterations=perf.nIterations, converged=bool(perf.converged),
-                         singular=bool(perf.singular)), lib().orc_threads()
+                         singular=perf.singular == 1, diverged=perf.singular == 2), lib().orc_threads()
 
     # ---- global (concatenated) sparse matrix, for brute-force checks ----
     def offsets(self):
