@@ -124,7 +124,7 @@ def run_tutorial(m, args):
     try:
         meta = foam_fixture.write_fixture_case(d)
         t0 = time.perf_counter()
-        case = m.Case.from_foam(d)
+        case = m.Case.from_foam(d, div="upwind")
         t_read = time.perf_counter() - t0
     finally:
         shutil.rmtree(d, ignore_errors=True)
@@ -212,7 +212,7 @@ def run_suite(m, args):
     d = tempfile.mkdtemp(prefix="testSuite_")
     try:
         foam_fixture.write_fixture_case(d)
-        base = m.Case.from_foam(d, ddt="steadyState")
+        base = m.Case.from_foam(d, ddt="steadyState", div="upwind")
         # the host path this replaces: read and assemble each case, then
         # build the batch layout and upload the matrices
         host_cases, t_read = [], 0.0
@@ -220,7 +220,7 @@ def run_suite(m, args):
             shutil.rmtree(d, ignore_errors=True)
             foam_fixture.write_fixture_case(d, params=params(*c))   # writing the files is not timed
             t1 = time.perf_counter()
-            host_cases.append(m.Case.from_foam(d, ddt="steadyState"))
+            host_cases.append(m.Case.from_foam(d, ddt="steadyState", div="upwind"))
             t_read += time.perf_counter() - t1
     finally:
         shutil.rmtree(d, ignore_errors=True)
@@ -355,7 +355,7 @@ def run_suite_partitioned(m, args):
         d = tempfile.mkdtemp(prefix="testSuite_p_")
         try:
             foam_fixture.write_fixture_case(d, params=params(*c), partitioned=dict(accType="aitken", relax=0.75))
-            cs.append(m.Case.from_foam(d, ddt="steadyState"))
+            cs.append(m.Case.from_foam(d, ddt="steadyState", div="upwind"))
         finally:
             shutil.rmtree(d, ignore_errors=True)
     A = m.CoupledFvMatrix(0)
@@ -401,10 +401,36 @@ def run_suite_partitioned(m, args):
     print(json.dumps(out))
 
 
+def _inside_solve_kernels(A, reps):
+    """In-solve launch times (ms) of the three hot kernels, from HIP events
+    around one launch in each of `reps` BiCGStab iterations."""
+    return {"dilu_fwd": A.time_kernel(4, reps), "dilu_bwd": A.time_kernel(5, reps),
+            "spmv_alpha": A.time_kernel(6, reps)}
+
+
+def _traffic(name, kernel, systems=None):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    tfile = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(tfile):
+        return None
+    with open(tfile) as f:
+        tj = json.load(f)
+    if systems is not None and tj.get("systems") not in (None, systems):
+        return None
+    k = tj.get("kernels", {}).get(kernel)
+    return None if k is None else k["total_bytes"]
+
+
 def run_batch(m, cases, args):
     """64 independent two-region systems of seeded uneven size (cells per
     region log-uniform in [10k, 500k], conductivity ratio log-uniform in
-    [1, 1000]), all solved by the same launches."""
+    [1, 1000]), all solved by the same launches: each check interval's
+    BiCGStab iterations replayed as one captured HIP graph, the batch
+    kernels' launch configuration picked by the autotuner on the first
+    solve.  One step = one solve of all 64 systems from their initial
+    fields; value = systems solved per second (FP64); the mixed FP32/FP64
+    refinement is reported beside it."""
     import numpy as np
     progress("batch64: building 64 systems")
     cs = cases.batch(64)
@@ -412,27 +438,80 @@ def run_batch(m, cases, args):
     for c in cs:
         A.add_case(c)
     A.commit()
-    cells = sum(A.layout_info(q)["cells"] for q in range(len(cs)))
+    n = len(cs)
+    cells = [A.layout_info(q)["cells"] for q in range(n)]
+    if args.kernel_probe:
+        A.solve(m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000))   # autotune
+        kernel_probe(A, args.kernel_reps)
+        return
     res = {}
     for prec in ("fp64", "mixed"):
-        ctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
-                          innerRelTol=1e-4)
+        if prec == "mixed" and args.no_mixed:
+            continue
+        ctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=1e-4)
         total, lat, perfs = run_steps(A, ctrl, args.warmup, args.steps, all_perf=True, verbose=True)
-        res[prec] = dict(systems_s=len(cs) * args.steps / total, ms_per_batch=1e3 * total / args.steps,
+        res[prec] = dict(systems_s=n * args.steps / total, ms_per_batch=1e3 * total / args.steps,
                          converged=sum(p.converged for p in perfs), max_iterations=max(p.n_iterations for p in perfs),
-                         max_outer=max(p.n_outer for p in perfs),
+                         max_outer=max(p.n_outer for p in perfs), iterations=[p.n_iterations for p in perfs],
                          p50_ms=1e3 * pct(lat, 0.5), p99_ms=1e3 * pct(lat, 0.99))
+        res[prec + "_psi"] = [np.concatenate(A.psi(q)) for q in range(n)]
         progress("batch64 %s: %.1f ms per batch" % (prec, res[prec]["ms_per_batch"]))
+    if "mixed" in res:
+        res["mixed"]["max_rel_l2_vs_fp64"] = max(
+            float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(res["mixed_psi"], res["fp64_psi"]))
+    # kernels inside the solve's iterations (the batch's tuned configuration)
+    A.reset_psi()
+    ks = _inside_solve_kernels(A, args.kernel_reps)
+    dilu_bytes = sum(0.5 * A.bytes_model(q, 8)["dilu"] for q in range(n))
+    spmv_bytes = sum(A.bytes_model(q, 8)["spmv"] + 8 * cells[q] for q in range(n))
+    per_iter = {"dilu_fwd": 2 * ks["dilu_fwd"], "dilu_bwd": 2 * ks["dilu_bwd"], "spmv_alpha": 2 * ks["spmv_alpha"]}
+    dom = max(per_iter, key=per_iter.get)
+    launch_bytes = spmv_bytes if dom == "spmv_alpha" else dilu_bytes
+    names = {"dilu_fwd": "k_dilu_box<double, 3, 0, false>", "dilu_bwd": "k_dilu_box<double, 3, 1, false>",
+             "spmv_alpha": "k_spmv_alpha<double>"}
+    achieved = launch_bytes / (ks[dom] * 1e-3) / 1e9
+    kernels = {k: dict(ms=v, GBs=(spmv_bytes if k == "spmv_alpha" else dilu_bytes) / (v * 1e-3) / 1e9,
+                       frac=(spmv_bytes if k == "spmv_alpha" else dilu_bytes) / (v * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       traffic=_traffic("batch64_hbm_traffic.json", names[k], n))
+               for k, v in ks.items()}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # a bounded sample: the oracle's solves of every 8th system on one
+        # host core, scaled to the 64 by cells x iterations
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind
+        sample = list(range(0, n, 8))
+        t_s, work_s = 0.0, 0.0
+        for q in sample:
+            O = oracle_bind.OracleSystem(cs[q])
+            t0 = time.perf_counter()
+            _, pr = O.solve(tol=1e-10, maxIter=3000)
+            t_s += time.perf_counter() - t0
+            work_s += cells[q] * pr["nIterations"]
+        work_all = sum(c * it for c, it in zip(cells, res["fp64"]["iterations"]))
+        t_all = t_s * work_all / work_s
+        cpu = dict(value=n / t_all, unit="systems/s", cores=1, kind="port",
+                   sample="oracle FP64 solves of systems %s on one host core (%.1f s), scaled to all 64 by "
+                          "cells x iterations" % (sample, t_s))
+    t = A.tuning()
     out = {
-        "metric": "coupled solves/s, 64 uneven systems per launch; mixed vs FP64",
-        "value": res["mixed"]["systems_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": res["mixed"]["ms_per_batch"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 (mixed refinement)",
-        "data": "synthetic (seeded case generator)",
-        "config": {"workload": "batch64: 64 two-region box systems, 10k-500k cells per region", "systems": len(cs),
-                   "cells": cells},
-        "mixed": res["mixed"], "fp64": res["fp64"],
-        "mixed_speedup_vs_fp64": res["mixed"]["systems_s"] / res["fp64"]["systems_s"],
+        "metric": "coupled solves/s, 64 uneven systems per launch (FP64); mixed vs FP64",
+        "value": res["fp64"]["systems_s"], "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fp64"]["ms_per_batch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded case generator)",
+        "config": {"workload": "batch64: 64 two-region box systems, 10k-500k cells per region, k ratio 1-1000, "
+                               "DILU-BiCGStab tol 1e-10, HIP graph per check interval, autotuned launches",
+                   "systems": n, "cells": int(sum(cells))},
+        "fp64": res["fp64"], "mixed": res.get("mixed"),
+        "mixed_speedup_vs_fp64": (res["mixed"]["systems_s"] / res["fp64"]["systems_s"]) if "mixed" in res else None,
+        "tuning": t, "kernels_in_solve": kernels,
+        "hbm_gbs_algorithmic": sum(it * A.bytes_model(q, 8)["iter"] for q, it in enumerate(res["fp64"]["iterations"]))
+        / (res["fp64"]["ms_per_batch"] * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": _traffic("batch64_hbm_traffic.json", names[dom], n),
+                     "bytes_per_launch": launch_bytes},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -554,6 +633,34 @@ def main():
     solves_s = args.steps / total
     gbs = perf.n_iterations * bm64["iter"] / (total / args.steps) / 1e9
 
+    # the drop-in time step: the caller hands over the assembled values of
+    # every region and interface (mrfg_update_region / _interface: upload and
+    # placement on the committed structure), then the solve (factor +
+    # iterations), from the same initial field as above
+    C = m.Case(case)
+    regs, ifs = C.regions(), C.interfaces()
+    A.reset_psi()
+    A.solve(fp64)     # graphs and tuning are already in place; warm once
+    dlat = []
+    for _ in range(max(3, args.steps // 2)):
+        d0 = time.perf_counter()
+        A.reset_psi()
+        for r, R in enumerate(regs):
+            A.update_region(0, r, R["diag"], R["upper"], R["source"], lower=None if R["symmetric"] else R["lower"])
+        for i, I in enumerate(ifs):
+            A.update_interface(0, i, I["coeffA"], I["coeffB"])
+        dperf = A.solve(fp64)[0]
+        dlat.append(time.perf_counter() - d0)
+    upload = sum(8 * (2 * R["nCells"] + (1 if R["symmetric"] else 2) * R["nFaces"]) for R in regs) + \
+        sum(8 * (I["nA"] + I["nB"]) for I in ifs)
+    drop_in = dict(ms_per_step=1e3 * sum(dlat) / len(dlat), solves_s=len(dlat) / sum(dlat),
+                   iterations=dperf.n_iterations, same_as_resident=dperf.final_residual == perf.final_residual,
+                   bytes_uploaded=upload, steps=len(dlat),
+                   note="reset field, upload every region's diag/lower/upper/source and the interface "
+                        "coefficients (pageable host memory), place on the device, factor, solve")
+    psi_drop = np.concatenate(A.psi(0))
+    drop_in["field_equals_resident"] = bool(np.array_equal(psi_drop, psi64))
+
     mixed = None
     if not args.no_mixed:
         mctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
@@ -650,6 +757,8 @@ def main():
         "p50_ms": 1e3 * pct(lat, 0.5),
         "p99_ms": 1e3 * pct(lat, 0.99),
         "rel_l2_vs_fp64_oracle": err64,
+        "drop_in_step": drop_in,
+        "tuning": A.tuning(),
         "mixed": mixed,
         "kernels_ms": {"back_to_back": {"spmv": k_spmv, "dilu_fwd": k_fwd, "dilu_bwd": k_bwd},
                        "in_solve": {"spmv_alpha": s_spmv, "dilu_fwd": s_fwd, "dilu_bwd": s_bwd}},

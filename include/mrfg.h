@@ -76,7 +76,18 @@ typedef struct {
     int inner_max_iter;
     int max_outer;
     int check_interval;    /* iterations between host convergence checks */
+    int launch_mode;       /* MRFG_LAUNCH_GRAPH (0): each check interval's iterations replayed as one
+                              captured HIP graph; MRFG_LAUNCH_KERNELS (1): kernels launched one by
+                              one (fvSolution key graphs on|off) */
+    int tuning;            /* MRFG_TUNE_AUTO (0): the first solve after a commit times the SpMV in
+                              every launch configuration (blocks of 4, 8 or 16 slices of 64 rows;
+                              ELL chunk 2, 4, 8 or 16) on the batch and keeps the fastest;
+                              MRFG_TUNE_FIXED (1): 4 slices, chunk 8 (fvSolution key
+                              autotune on|off).  Results do not depend on the choice. */
 } mrfg_controls;
+
+enum { MRFG_LAUNCH_GRAPH = 0, MRFG_LAUNCH_KERNELS = 1 };
+enum { MRFG_TUNE_AUTO = 0, MRFG_TUNE_FIXED = 1 };
 
 /* ---- context --------------------------------------------------------- */
 
@@ -402,6 +413,17 @@ int mrfg_time_kernel(mrfg_ctx *ctx, int which, int reps, double *ms);
    end ns, slow-path waits, steps], forward sweep then backward sweep
    (8*min(n_waves, max_waves) values).  n_waves receives the wave count. */
 int mrfg_trace_dilu(mrfg_ctx *ctx, long long *out, int max_waves, int *n_waves);
+/* The launch configuration the autotuner chose for the committed batch
+   (profiling facility; no counterpart in the reference): threads per block
+   of the batch kernels, the SpMV's ELL chunk, and the SpMV time (ms) with
+   it and with the fixed default (256 threads, chunk 8).  *tuned = 0 before
+   the first autotuned solve. */
+int mrfg_tuning(mrfg_ctx *ctx, int *tuned, int *block_threads, int *ell_chunk, double *spmv_ms,
+                double *spmv_ms_default);
+/* Force a launch configuration (block_threads 256, 512 or 1024; ell_chunk
+   2, 4, 8 or 16) for the following solves of the committed batch, as if the
+   autotuner had chosen it (testing and profiling facility). */
+int mrfg_set_tuning(mrfg_ctx *ctx, int block_threads, int ell_chunk);
 /* info[0..5] = cells, internal faces, GGI pairs, rows incl. padding,
    64-row slices, DILU waves. */
 int mrfg_layout_info(mrfg_ctx *ctx, int sys, long long *info);
@@ -415,18 +437,23 @@ int mrfg_copy_probe(mrfg_ctx *ctx, long long bytes, int reps, double *ms);
 
 /* ---- serving (continuous batching; no counterpart in the reference) ------
 
-   A server admits coupled solves as they arrive: worker threads, each with
-   its own device-resident batch and HIP stream, take the oldest request,
-   assemble it, and start it once its device footprint fits the HBM budget
-   left by the requests in flight; requests already waiting join the same
-   batch (up to max_batch systems).  Different workers' batches run
-   concurrently on the device.  A result does not depend on the batch or the
-   worker that produced it.  Every request is solved with the controls of
-   solver entry `entry` of `fv_solution`; keep_fields = 1 keeps each
-   result's field for mrfg_server_get_psi. */
+   A server admits coupled solves as they arrive.  n_prep host threads
+   (0: the host core share -- OMP_NUM_THREADS when set -- less n_workers)
+   assemble each request and build its device layout, in arrival order.
+   n_workers device slots, each a resident batch on its own HIP stream,
+   take the oldest prepared requests whose device footprint fits the HBM
+   budget left by the requests in flight (up to max_batch per slot) and
+   solve them with one set of launches; slots run concurrently on the
+   device, so a request arriving while others run starts as soon as a slot
+   and memory are free.  Each request is retired at the convergence check
+   that finds it finished, while the rest of its batch continues.  A result
+   does not depend on the batch, slot or time it ran in.  Every request is
+   solved with the controls of solver entry `entry` of `fv_solution`;
+   keep_fields = 1 keeps each result's field for mrfg_server_get_psi.
+   Results may be polled while requests run (MRFG_ERR_STATE until done). */
 typedef struct mrfg_server mrfg_server;
 int mrfg_server_create(mrfg_server **srv, int device, long long hbm_budget_bytes, int n_workers, int max_batch,
-                       const char *fv_solution, const char *entry, int keep_fields);
+                       int n_prep, const char *fv_solution, const char *entry, int keep_fields);
 void mrfg_server_destroy(mrfg_server *srv);
 /* Queue one request (a case dictionary, as mrfg_case_build reads). */
 int mrfg_server_submit(mrfg_server *srv, const char *case_dict, int *request_id);

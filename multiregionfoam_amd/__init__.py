@@ -52,7 +52,8 @@ class Controls(C.Structure):
     _fields_ = [("solver", C.c_int), ("precision", C.c_int), ("tolerance", C.c_double),
                 ("rel_tol", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
                 ("inner_rel_tol", C.c_double), ("inner_max_iter", C.c_int),
-                ("max_outer", C.c_int), ("check_interval", C.c_int)]
+                ("max_outer", C.c_int), ("check_interval", C.c_int), ("launch_mode", C.c_int),
+                ("tuning", C.c_int)]
 
 
 _lib = None
@@ -123,6 +124,8 @@ def lib():
         "mrfg_precondition_fp32": ([vp, C.c_int, F32P, F32P], C.c_int),
         "mrfg_time_kernel": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_layout_info": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
+        "mrfg_tuning": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), F64P, F64P], C.c_int),
+        "mrfg_set_tuning": ([vp, C.c_int, C.c_int], C.c_int),
         "mrfg_trace_dilu": ([vp, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)], C.c_int),
         "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
         "mrfg_copy_probe": ([vp, C.c_longlong, C.c_int, F64P], C.c_int),
@@ -132,8 +135,8 @@ def lib():
         "mrfg_assembly_bytes": ([vp, F64P, F64P], C.c_int),
         "mrfg_get_region_matrix": ([vp, C.c_int, C.c_int, F64P, F64P, F64P, F64P], C.c_int),
         "mrfg_get_interface_coeff": ([vp, C.c_int, C.c_int, C.c_int, F64P], C.c_int),
-        "mrfg_server_create": ([C.POINTER(vp), C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_char_p, C.c_char_p,
-                                C.c_int], C.c_int),
+        "mrfg_server_create": ([C.POINTER(vp), C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                C.c_char_p, C.c_int], C.c_int),
         "mrfg_server_destroy": ([vp], None),
         "mrfg_server_submit": ([vp, C.c_char_p, C.POINTER(C.c_int)], C.c_int),
         "mrfg_server_wait_all": ([vp], C.c_int),
@@ -159,7 +162,7 @@ EXPORTED = (
     "mrfg_case_region_source_base "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_case_interface_wall mrfg_set_interface_wall "
     "mrfg_interface_patch_values mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
+    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_tuning mrfg_set_tuning mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
     "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
     "mrfg_server_get_psi mrfg_server_last_error").split()
@@ -191,13 +194,15 @@ def parse_controls(fv_solution: str, entry: str) -> Controls:
 
 
 def _controls_text(solver="BiCGStab", tolerance=1e-10, relTol=0.0, minIter=0, maxIter=1000,
-                   precision="fp64", innerRelTol=1e-4, innerMaxIter=1000, maxOuter=50, checkInterval=8):
+                   precision="fp64", innerRelTol=1e-4, innerMaxIter=1000, maxOuter=50, checkInterval=8,
+                   graphs=True, autotune=True):
     """fvSolution text with one solver entry "X" holding the given keys."""
     return ("solvers { X { solver %s; preconditioner { preconditioner Cholesky; } tolerance %r; "
-            "relTol %r; minIter %d; maxIter %d; precision %s; checkInterval %d; "
+            "relTol %r; minIter %d; maxIter %d; precision %s; checkInterval %d; graphs %s; autotune %s; "
             "refinement { innerRelTol %r; innerMaxIter %d; maxOuter %d; } } }"
             % (solver, float(tolerance), float(relTol), minIter, maxIter, precision, checkInterval,
-               float(innerRelTol), innerMaxIter, maxOuter))
+               "on" if graphs else "off", "on" if autotune else "off", float(innerRelTol), innerMaxIter,
+               maxOuter))
 
 
 def controls(**kw):
@@ -251,13 +256,15 @@ class Case:
         self._h = h
 
     @classmethod
-    def from_foam(cls, case_dir, fieldDir=None, interfaceProperties=None, ddt=None, deltaT=None):
+    def from_foam(cls, case_dir, fieldDir=None, interfaceProperties=None, ddt=None, deltaT=None, div=None):
         """Read and assemble a multi-region case directory (polyMesh, 0/,
         constant/, system/), as multiRegionFoam sets up its coupled T
-        equation; keyword options override what the case files select."""
+        equation; keyword options override what the case files select
+        (``div="upwind"``: assemble upwind convection where the case names a
+        div(phi,T) scheme the reader does not assemble, e.g. linearUpwind)."""
         opts = []
         for k, v in (("fieldDir", fieldDir), ("interfaceProperties", interfaceProperties), ("ddt", ddt),
-                     ("deltaT", deltaT)):
+                     ("deltaT", deltaT), ("div", div)):
             if v is not None:
                 opts.append("%s %s;" % (k, repr(float(v)) if k == "deltaT" else v))
         h = C.c_void_p()
@@ -684,6 +691,18 @@ class CoupledFvMatrix:
         keys = ("cells", "faces", "pairs", "rows", "slices", "waves")
         return dict(zip(keys, [int(v) for v in info]))
 
+    def tuning(self):
+        """The autotuner's launch configuration for the batch (after a solve)."""
+        t, th, ch = C.c_int(), C.c_int(), C.c_int()
+        a, b = C.c_double(), C.c_double()
+        self._check(lib().mrfg_tuning(self._h, C.byref(t), C.byref(th), C.byref(ch), C.byref(a), C.byref(b)))
+        return dict(tuned=bool(t.value), block_threads=th.value, ell_chunk=ch.value, spmv_ms=a.value,
+                    spmv_ms_default=b.value)
+
+    def set_tuning(self, block_threads, ell_chunk):
+        """Force a launch configuration for the following solves."""
+        self._check(lib().mrfg_set_tuning(self._h, int(block_threads), int(ell_chunk)))
+
     def bytes_model(self, sys=0, value_size=8):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         self._check(lib().mrfg_bytes(self._h, sys, value_size, C.byref(a), C.byref(b), C.byref(c)))
@@ -708,18 +727,20 @@ class ServeResult:
 
 
 class Server:
-    """Request loop over coupled solves: HBM-budget admission, worker threads
-    with their own device-resident batches, batching of waiting requests."""
+    """Request loop over coupled solves with continuous batching: host
+    preparation threads (assembly, layout), device slots with their own
+    resident batches and streams, HBM-budget admission in arrival order,
+    each request retired as soon as it has converged."""
 
     def __init__(self, device=0, hbm_budget=64 << 30, workers=4, max_batch=8, fv_solution=None,
-                 entry="Tcoupled", keep_fields=False, **controls_kw):
+                 entry="Tcoupled", keep_fields=False, prep_threads=0, **controls_kw):
         if fv_solution is None:
             kw = dict(solver="BiCGStab", tolerance=1e-10, maxIter=1000)
             kw.update(controls_kw)
             fv_solution = _controls_text(**kw)
             entry = "X"
         h = C.c_void_p()
-        rc = lib().mrfg_server_create(C.byref(h), device, int(hbm_budget), workers, max_batch,
+        rc = lib().mrfg_server_create(C.byref(h), device, int(hbm_budget), workers, max_batch, int(prep_threads),
                                       _text(fv_solution), _text(entry), 1 if keep_fields else 0)
         if rc != MRFG_OK:
             raise MrfgError(rc, "cannot create the server")

@@ -28,6 +28,12 @@ namespace mrfg {
 
 // Buffers keep their capacity: a batch re-filled with systems of at most
 // the same size allocates (and frees, which synchronises the device) nothing.
+std::recursive_mutex &deviceMutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 template <typename T>
 void DevBuf<T>::alloc(size_t count)
 {
@@ -35,6 +41,7 @@ void DevBuf<T>::alloc(size_t count)
         n = count;
         return;
     }
+    std::lock_guard<std::recursive_mutex> lk(deviceMutex());
     release();
     n = count;
     cap = std::max<size_t>(count, 1);
@@ -95,6 +102,18 @@ SolverControls parseControls(const Dict &d)
         c.maxOuter = (int)r.numberOrDefault("maxOuter", c.maxOuter);
     }
     c.checkInterval = std::max(1, (int)d.numberOrDefault("checkInterval", c.checkInterval));
+    if (d.found("autotune")) {
+        const std::string g = d.word("autotune");
+        if (g == "on" || g == "yes" || g == "true") c.autotune = true;
+        else if (g == "off" || g == "no" || g == "false") c.autotune = false;
+        else throw DictError("autotune must be on or off, not '" + g + "'");
+    }
+    if (d.found("graphs")) {
+        const std::string g = d.word("graphs");
+        if (g == "on" || g == "yes" || g == "true") c.graphs = true;
+        else if (g == "off" || g == "no" || g == "false") c.graphs = false;
+        else throw DictError("graphs must be on or off, not '" + g + "'");
+    }
     return c;
 }
 
@@ -104,6 +123,7 @@ SolverControls parseControls(const Dict &d)
 
 Batch::Batch(int device) : device_(device)
 {
+    std::lock_guard<std::recursive_mutex> lk(deviceMutex());
     HIP_OK(hipSetDevice(device_));
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIP_OK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
@@ -113,7 +133,9 @@ Batch::Batch(int device) : device_(device)
 
 Batch::~Batch()
 {
+    std::lock_guard<std::recursive_mutex> lk(deviceMutex());
     (void)hipSetDevice(device_);
+    dropGraphs();
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
@@ -145,17 +167,77 @@ int roundK(int k)
 
 }  // namespace
 
-void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
+void Batch::dropGraphs()
+{
+    std::lock_guard<std::recursive_mutex> lk(deviceMutex());
+    for (auto &g : graphs_) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    graphs_.clear();
+}
+
+template <typename T>
+void Batch::runChunk(bool pcg, int chunk, bool useGraph, const MatView<T> &A, const DiluView<T> &D,
+                     const Krylov<T> &K)
+{
+    auto launch = [&]() -> hipError_t {
+        for (int it = 0; it < chunk; ++it) {
+            hipError_t e = pcg ? launchPcgIteration<T>(A, D, K, stream_) : launchBicgIteration<T>(A, D, K, stream_);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    if (!useGraph) {
+        HIP_OK(launch());
+        return;
+    }
+    const int kind = (pcg ? 1 : 0) + (sizeof(T) == 4 ? 2 : 0);
+    IterGraph *g = nullptr;
+    for (auto &x : graphs_)
+        if (x.kind == kind && x.chunk == chunk) g = &x;
+    if (!g) {
+        std::lock_guard<std::recursive_mutex> lk(deviceMutex());
+        IterGraph n;
+        n.kind = kind;
+        n.chunk = chunk;
+        HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch();
+        hipError_t e2 = hipStreamEndCapture(stream_, &n.graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (n.graph) (void)hipGraphDestroy(n.graph);
+            HIP_OK(e != hipSuccess ? e : e2);
+        }
+        e = hipGraphInstantiate(&n.exec, n.graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(n.graph);
+            HIP_OK(e);
+        }
+        graphs_.push_back(n);
+        g = &graphs_.back();
+    }
+    HIP_OK(hipGraphLaunch(g->exec, stream_));
+}
+
+void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
+                       std::vector<std::unique_ptr<SystemLayout>> *layouts)
 {
     HIP_OK(hipSetDevice(device_));
     HIP_OK(hipStreamSynchronize(stream_));
+    dropGraphs();   // the captured launches name the buffers about to change
     sys_ = systems;
     lay_.clear();
     ifDev_.clear();
     // layouts of several systems are built concurrently (each depends only
     // on its own system)
     lay_.resize(systems.size());
-    {
+    if (layouts) {
+        if (layouts->size() != systems.size()) throw std::runtime_error("one layout per system is needed");
+        for (size_t q = 0; q < systems.size(); ++q) {
+            if (!(*layouts)[q]) throw std::runtime_error("missing layout");
+            lay_[q] = std::move((*layouts)[q]);
+        }
+    } else {
         std::vector<std::thread> th;
         std::vector<std::string> errs(systems.size());
         for (size_t q = 0; q < systems.size(); ++q) {
@@ -433,6 +515,22 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems)
     cellOfPos_.upload(cellOfPos, st);
     blocks_.upload(blocks, st);
     info_.upload(hInfo_, st);
+    for (int k = 0; k < 2; ++k) {
+        const int S = 8 << k;
+        std::vector<BlockEntry> bl;
+        std::vector<SysInfo> inf = hInfo_;
+        for (int q = 0; q < nS; ++q) {
+            inf[q].blkBegin = (int)bl.size();
+            for (int s0 = 0; s0 < hInfo_[q].nSlices; s0 += S)
+                bl.push_back(BlockEntry{q, hInfo_[q].sliceBegin + s0,
+                                        hInfo_[q].sliceBegin + std::min(hInfo_[q].nSlices, s0 + S), 0});
+            inf[q].nBlk = (int)bl.size() - inf[q].blkBegin;
+        }
+        bcfg_[k].nBlk = (int)bl.size();
+        bcfg_[k].blocks.upload(bl.empty() ? std::vector<BlockEntry>{BlockEntry{0, 0, 0, 0}} : bl, st);
+        bcfg_[k].info.upload(inf, st);
+    }
+    tune_ = Tuning{};
     order_.upload(order, st);
     boxOrder_.upload(boxOrder, st);
     waveOut_.upload(waveOut, st);
@@ -493,6 +591,7 @@ void Batch::ensureFloat()
 {
     if (haveFloat_) return;
     if (hostStale_) refreshHostCopies();
+    dropGraphs();   // FP32 buffers may be allocated anew
     auto toF = [](const std::vector<double> &a) {
         std::vector<float> b(a.size());
         for (size_t i = 0; i < a.size(); ++i) b[i] = (float)a[i];
@@ -538,6 +637,7 @@ MatView<T> Batch::view(const DevMatrix<T> &m) const
     A.ifPairOff = ifPairOff_.p;
     A.ifPairCol = ifPairCol_.p;
     A.ifPairW = m.ifPairW.p;
+    A.ellChunk = tune_.ellChunk;
     return A;
 }
 
@@ -581,13 +681,68 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     return D;
 }
 
+void Batch::launchCfg(int slices, const BlockEntry *&blocks, int &nBlk, const SysInfo *&info, int &threads) const
+{
+    const int k = slices == 16 ? 1 : 0;
+    blocks = slices == 4 ? blocks_.p : bcfg_[k].blocks.p;
+    nBlk = slices == 4 ? nBlk_ : bcfg_[k].nBlk;
+    info = slices == 4 ? info_.p : bcfg_[k].info.p;
+    threads = 64 * slices;
+}
+
+void Batch::setTuning(int slices, int ellChunk)
+{
+    if (tune_.slices == slices && tune_.ellChunk == ellChunk) return;
+    tune_.slices = slices;
+    tune_.ellChunk = ellChunk;
+    dropGraphs();   // the captured launches use the previous configuration
+}
+
+void Batch::autotune()
+{
+    HIP_OK(hipSetDevice(device_));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    const int reps = 5;
+    double best = 1e300, def = 0.0;
+    int bs = 4, bc = 8;
+    for (int slices : {4, 8, 16})
+        for (int ch : {2, 4, 8, 16}) {
+            MatView<double> A = view(m64_);
+            A.ellChunk = ch;
+            const BlockEntry *bl;
+            const SysInfo *inf;
+            int nb, th;
+            launchCfg(slices, bl, nb, inf, th);
+            HIP_OK(launchSpmvPlain<double>(A, bl, nb, th, x64_.p, v64_[0].p, stream_));   // warm-up
+            HIP_OK(hipEventRecord(e0, stream_));
+            for (int r = 0; r < reps; ++r) HIP_OK(launchSpmvPlain<double>(A, bl, nb, th, x64_.p, v64_[0].p, stream_));
+            HIP_OK(hipEventRecord(e1, stream_));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            const double t = (double)ms / reps;
+            if (slices == 4 && ch == 8) def = t;
+            if (t < best) {
+                best = t;
+                bs = slices;
+                bc = ch;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    setTuning(bs, bc);
+    tune_.tuned = true;
+    tune_.bestMs = best;
+    tune_.defaultMs = def;
+}
+
 template <>
 void Batch::bindKrylov<double>(Krylov<double> &K)
 {
-    K.blocks = blocks_.p;
-    K.nBlk = nBlk_;
+    launchCfg(tune_.slices, K.blocks, K.nBlk, K.info, K.threads);
     K.nSys = (int)sys_.size();
-    K.info = info_.p;
     K.state = st64_.p;
     K.partials = partials_.p;
     K.sumA = sumA_.p;
@@ -601,10 +756,8 @@ void Batch::bindKrylov<double>(Krylov<double> &K)
 template <>
 void Batch::bindKrylov<float>(Krylov<float> &K)
 {
-    K.blocks = blocks_.p;
-    K.nBlk = nBlk_;
+    launchCfg(tune_.slices, K.blocks, K.nBlk, K.info, K.threads);
     K.nSys = (int)sys_.size();
-    K.info = info_.p;
     K.state = st32_.p;
     K.partials = partials_.p;
     K.sumA = sumA_.p;
@@ -677,6 +830,31 @@ static bool anyRunning(const std::vector<SysState> &s)
     return false;
 }
 
+SolvePerf Batch::perfFp64(int q, const SysState &s) const
+{
+    SolvePerf p;
+    p.initialResidual = s.initRes;
+    p.finalResidual = s.finalRes;
+    p.nIterations = s.nIter;
+    p.converged = s.status == kConverged;
+    p.singular = s.status == kSingular ? 1 : s.status == kDiverged ? 2 : 0;
+    p.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - solveStart_).count();
+    p.bytes = p.nIterations * bytesIter(q, 8);
+    return p;
+}
+
+std::vector<int> Batch::newlyFinished(const std::vector<SysState> &hs)
+{
+    std::vector<int> out;
+    if (!retire_) return out;
+    for (size_t q = 0; q < hs.size(); ++q)
+        if (!retired_[q] && hs[q].status != kRunning && hs[q].status != kIdle) {
+            retired_[q] = 1;
+            out.push_back((int)q);
+        }
+    return out;
+}
+
 void Batch::runFp64(const SolverControls &c)
 {
     Krylov<double> K;
@@ -687,16 +865,36 @@ void Batch::runFp64(const SolverControls &c)
     HIP_OK(launchSetup<double>(A, K, pcg, stream_));
     std::vector<SysState> hs;
     readState(hs);
+    // systems that stopped at the last check are handed to the retire
+    // callback after the next chunk is launched, so their results are
+    // copied out while the others iterate
+    std::vector<int> fin = newlyFinished(hs);
+    std::vector<SysState> finState = hs;
+    auto retireAll = [&]() {
+        for (int q : fin) (*retire_)(q, perfFp64(q, finState[q]));
+        fin.clear();
+    };
     int done = 0;
     while (anyRunning(hs) && done <= c.maxIter) {
         int chunk = std::min(c.checkInterval, c.maxIter - done + 1);
-        for (int it = 0; it < chunk; ++it) {
-            if (pcg) HIP_OK(launchPcgIteration<double>(A, D, K, stream_));
-            else HIP_OK(launchBicgIteration<double>(A, D, K, stream_));
-        }
+        runChunk<double>(pcg, chunk, c.graphs, A, D, K);
+        retireAll();
         done += chunk;
         readState(hs);
+        fin = newlyFinished(hs);
+        finState = hs;
     }
+    retireAll();
+}
+
+void Batch::copyPsiNow(int sys, int region, double *psi)
+{
+    const SystemLayout &L = *lay_.at(sys);
+    const CoupledSystem &S = *sys_.at(sys);
+    const int n = S.regions.at(region).nCells, o = L.regionOffset[region];
+    std::vector<double> x(L.nPos);
+    HIP_OK(hipMemcpy(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < n; ++c) psi[c] = x[L.posOfCell[o + c]];
 }
 
 void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std::vector<int> &fp64Iters)
@@ -726,15 +924,34 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
         HIP_OK(launchRefineRhs(K64, K32, stream_));
         HIP_OK(launchSetup<float>(A32, K32, pcg, stream_));
         int done = 0;
+        // an inner solve whose best residual over the last kStallWindow
+        // checks is not 10 % below its best before them has stagnated (the
+        // FP32 system resolves no further): it stops there, and the outer
+        // step takes the correction it has
+        std::vector<std::vector<double>> hist(nS);
         for (;;) {
             HIP_OK(hipMemcpyAsync(hi.data(), st32_.p, nS * sizeof(SysState), hipMemcpyDeviceToHost, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
+            bool stopped = false;
+            for (int q = 0; q < nS; ++q) {
+                if (hi[q].status != kRunning) continue;
+                std::vector<double> &h = hist[q];
+                h.push_back(hi[q].finalRes);
+                if ((int)h.size() <= kStallWindow) continue;
+                const double recent = *std::min_element(h.end() - kStallWindow, h.end());
+                const double prior = *std::min_element(h.begin(), h.end() - kStallWindow);
+                if (!(recent < 0.9 * prior)) {
+                    hi[q].status = kMaxIter;
+                    stopped = true;
+                }
+            }
+            if (stopped) {
+                HIP_OK(hipMemcpyAsync(st32_.p, hi.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+                HIP_OK(hipStreamSynchronize(stream_));
+            }
             if (!anyRunning(hi) || done > c.innerMaxIter) break;
             int chunk = std::min(c.checkInterval, c.innerMaxIter - done + 1);
-            for (int it = 0; it < chunk; ++it) {
-                if (pcg) HIP_OK(launchPcgIteration<float>(A32, D32, K32, stream_));
-                else HIP_OK(launchBicgIteration<float>(A32, D32, K32, stream_));
-            }
+            runChunk<float>(pcg, chunk, c.graphs, A32, D32, K32);
             done += chunk;
         }
         for (int q = 0; q < nS; ++q)
@@ -773,6 +990,14 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
             HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
             HIP_OK(hipStreamSynchronize(stream_));
         }
+        // systems whose refinement has ended (not those continuing in FP64)
+        for (int q : newlyFinished(hs)) {
+            SolvePerf p = perfFp64(q, hs[q]);
+            p.nIterations = innerIters[q];
+            p.nOuter = hs[q].nIter;
+            p.bytes = p.nIterations * bytesIter(q, 4) + p.nOuter * bytesSpmv(q, 8);
+            (*retire_)(q, p);
+        }
     }
     for (int q = 0; q < nS; ++q)
         if (hs[q].status == kRunning) fallback[q] = 1;   // outer steps exhausted
@@ -795,7 +1020,10 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
     HIP_OK(hipStreamSynchronize(stream_));
     SolverControls c64 = c;
     c64.precision = SolverControls::FP64;
+    const RetireFn *keep = retire_;
+    retire_ = nullptr;   // the continuation's records are composed below
     runFp64(c64);
+    retire_ = keep;
     readState(h);
     for (int q = 0; q < nS; ++q) {
         if (fallback[q]) {
@@ -810,14 +1038,26 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
     HIP_OK(hipStreamSynchronize(stream_));
 }
 
-void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const std::vector<char> *active)
+void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const std::vector<char> *active,
+                  const RetireFn *retire)
 {
     if (active && active->size() != sys_.size()) throw std::runtime_error("active mask does not match the batch");
     HIP_OK(hipSetDevice(device_));
+    struct ClearRetire {
+        const RetireFn *&r;
+        ~ClearRetire() { r = nullptr; }
+    } clearRetire{retire_};
+    retire_ = retire;
+    retired_.assign(sys_.size(), 0);
+    if (active)
+        for (size_t q = 0; q < sys_.size(); ++q) retired_[q] = (*active)[q] ? 0 : 1;
     applyRefresh();
     if (c.precision == SolverControls::Mixed) ensureFloat();
+    if (c.autotune && !tune_.tuned) autotune();
+    if (!c.autotune) setTuning(4, 8);
     HIP_OK(hipStreamSynchronize(stream_));
     auto t0 = std::chrono::steady_clock::now();
+    solveStart_ = t0;
     // the preconditioner is built per solve call, as the coupled solver
     // object is constructed per coupledFvMatrix::solve call
     factorised_ = false;
@@ -848,6 +1088,12 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const s
             p.bytes = p.nIterations * bytesIter((int)q, 8);
         }
     }
+    if (retire_)
+        for (size_t q = 0; q < sys_.size(); ++q)
+            if (!retired_[q]) {
+                retired_[q] = 1;
+                (*retire_)((int)q, perf[q]);
+            }
 }
 
 void Batch::resetPsi()
@@ -927,7 +1173,7 @@ void Batch::spmv(const double *xHost, double *yHost, int sys)
     DevBuf<double> dx, dy;
     dx.upload(x, stream_);
     dy.alloc(nPos_);
-    HIP_OK(launchSpmvPlain<double>(view(m64_), blocks_.p, nBlk_, dx.p, dy.p, stream_));
+    HIP_OK(launchSpmvPlain<double>(view(m64_), blocks_.p, nBlk_, 256, dx.p, dy.p, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     HIP_OK(hipMemcpy(y.data(), dy.p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
     for (long long c = 0; c < L.nCells; ++c) yHost[c] = y[posBegin(sys) + L.posOfCell[c]];
@@ -1136,7 +1382,7 @@ double Batch::timeKernel(int which, int reps)
         hipEvent_t a = ev[2 * i], b = ev[2 * i + 1];
         if (which == 0) {
             HIP_OK(hipEventRecord(a, stream_));
-            HIP_OK(launchSpmvPlain<double>(A, blocks_.p, nBlk_, x64_.p, v64_[0].p, stream_));
+            HIP_OK(launchSpmvPlain<double>(A, K.blocks, K.nBlk, K.threads, x64_.p, v64_[0].p, stream_));
             HIP_OK(hipEventRecord(b, stream_));
         } else if (which == 3) {
             HIP_OK(hipEventRecord(a, stream_));

@@ -10,7 +10,10 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <chrono>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -38,6 +41,19 @@ struct SolverControls {
     int innerMaxIter = 1000;
     int maxOuter = 50;
     int checkInterval = 8;    // iterations between host convergence checks
+    bool graphs = true;       // replay each check interval's iterations as one captured HIP graph
+    bool autotune = true;     // time the SpMV launch configurations on the batch once, use the fastest
+};
+
+// The batch kernels' launch configuration chosen by the autotuner: blocks of
+// 4, 8 or 16 slices (256, 512 or 1024 threads) and the SpMV's ELL chunk;
+// the SpMV times (ms) of the choice and of the fixed default (4 slices,
+// chunk 8).  Results do not depend on it (row sums in entry order, exact
+// reductions).
+struct Tuning {
+    bool tuned = false;
+    int slices = 4, ellChunk = 8;
+    double bestMs = 0.0, defaultMs = 0.0;
 };
 
 SolverControls parseControls(const Dict &d);
@@ -48,6 +64,12 @@ struct SolvePerf {
     double seconds = 0;       // device time of the solve (setup + iterations)
     double bytes = 0;         // algorithmic bytes moved (canonical model)
 };
+
+// Held around every device allocation and free, stream creation and
+// destruction, and each HIP-graph capture of the library: a free (an
+// implicit device synchronisation) or a stream change in one thread while
+// another thread captures would invalidate that capture.
+std::recursive_mutex &deviceMutex();
 
 template <typename T>
 struct DevBuf {
@@ -60,6 +82,7 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void release()
     {
+        std::lock_guard<std::recursive_mutex> lk(deviceMutex());
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
@@ -111,14 +134,26 @@ public:
     explicit Batch(int device);
     ~Batch();
 
-    // Replace the batch content.  Builds layouts, uploads FP64 (and, on
-    // demand, FP32) matrices and the initial guesses.
-    void setSystems(const std::vector<const CoupledSystem *> &systems);
+    // Replace the batch content.  Builds layouts (or takes the given ones,
+    // one per system, built by buildLayout without preferLevels), uploads
+    // FP64 (and, on demand, FP32) matrices and the initial guesses.
+    void setSystems(const std::vector<const CoupledSystem *> &systems,
+                    std::vector<std::unique_ptr<SystemLayout>> *layouts = nullptr);
     int nSystems() const { return (int)sys_.size(); }
+
+    // Called for each system as soon as it has stopped iterating for good
+    // (at the convergence check that finds it, while the others continue),
+    // with its performance record; copyPsiNow may read its field then.
+    using RetireFn = std::function<void(int sys, const SolvePerf &perf)>;
 
     // Solve every system (or those with active[q] != 0; the others keep
     // their fields); perf gets one entry per system.
-    void solve(const SolverControls &c, std::vector<SolvePerf> &perf, const std::vector<char> *active = nullptr);
+    void solve(const SolverControls &c, std::vector<SolvePerf> &perf, const std::vector<char> *active = nullptr,
+               const RetireFn *retire = nullptr);
+    // The field of one region of a system that no longer iterates, copied
+    // without waiting for the batch's stream (the other systems' kernels
+    // may be running; this system's part of the field is final).
+    void copyPsiNow(int sys, int region, double *psi);
 
     // Time stepping of the transient systems (regions with ddt coefficients;
     // steady systems are re-solved from their current field): per step the
@@ -175,6 +210,15 @@ public:
     int traceDilu(long long *out, int maxWaves);
 
     const SystemLayout &layout(int sys) const { return *lay_[sys]; }
+    // Time k_spmv_plain in every launch configuration on this batch and keep
+    // the fastest (run by solve once per setSystems when controls ask for it)
+    void autotune();
+    const Tuning &tuning() const { return tune_; }
+    void forceTuning(int slices, int ellChunk)
+    {
+        setTuning(slices, ellChunk);
+        tune_.tuned = true;
+    }
     // device field and right-hand side (batch positions) and the position of
     // global cell `cell` of system `sys`
     const double *fieldDevice() const { return x64_.p; }
@@ -229,9 +273,30 @@ private:
     void initState(const SolverControls &c, const std::vector<char> *active = nullptr);
     void readState(std::vector<SysState> &out);
     void runFp64(const SolverControls &c);
+    // early retirement of finished systems during a solve
+    const RetireFn *retire_ = nullptr;
+    std::vector<char> retired_;
+    std::chrono::steady_clock::time_point solveStart_;
+    SolvePerf perfFp64(int q, const SysState &s) const;
+    std::vector<int> newlyFinished(const std::vector<SysState> &hs);
+    // `chunk` BiCGStab / PCG iterations on the library stream: one launch of
+    // a HIP graph captured from the iteration's kernel chain (kept per
+    // kind and chunk length until the batch's buffers change), or the
+    // kernels launched one by one
+    template <typename T>
+    void runChunk(bool pcg, int chunk, bool useGraph, const MatView<T> &A, const DiluView<T> &D, const Krylov<T> &K);
+    struct IterGraph {
+        int kind = -1, chunk = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<IterGraph> graphs_;
+    void dropGraphs();
     // Mixed FP32/FP64 iterative refinement; a system whose outer residual
     // falls by less than kStallFactor in an outer step continues in FP64.
     static constexpr double kStallFactor = 0.5;
+    // host checks of an inner FP32 solve without progress before it stops
+    static constexpr int kStallWindow = 3;
     void runMixed(const SolverControls &c, std::vector<int> &innerIters, std::vector<int> &fp64Iters);
     void checkDeviceError();
     void markAllRunning();
@@ -268,6 +333,17 @@ private:
     DevBuf<int32_t> cellOfPos_;
     DevBuf<BlockEntry> blocks_;
     DevBuf<SysInfo> info_;
+    // blocks of 8 and 16 slices (blocks_ / info_ hold those of 4), with the
+    // systems' ranges of block partials in each
+    struct BlockCfg {
+        int nBlk = 0;
+        DevBuf<BlockEntry> blocks;
+        DevBuf<SysInfo> info;
+    };
+    std::array<BlockCfg, 2> bcfg_;
+    Tuning tune_;
+    void launchCfg(int slices, const BlockEntry *&blocks, int &nBlk, const SysInfo *&info, int &threads) const;
+    void setTuning(int slices, int ellChunk);
     // DILU schedule
     DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_, waveOut_, boxOrder_, boxMeta_;
     DevBuf<int4> fCodes_, bCodes_;

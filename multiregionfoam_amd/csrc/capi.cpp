@@ -99,6 +99,8 @@ void toControls(const mrfg::SolverControls &s, mrfg_controls *c)
     c->inner_max_iter = s.innerMaxIter;
     c->max_outer = s.maxOuter;
     c->check_interval = s.checkInterval;
+    c->launch_mode = s.graphs ? MRFG_LAUNCH_GRAPH : MRFG_LAUNCH_KERNELS;
+    c->tuning = s.autotune ? MRFG_TUNE_AUTO : MRFG_TUNE_FIXED;
 }
 
 mrfg::SolverControls fromControls(const mrfg_controls *c)
@@ -116,6 +118,10 @@ mrfg::SolverControls fromControls(const mrfg_controls *c)
     s.innerMaxIter = c->inner_max_iter;
     s.maxOuter = c->max_outer;
     s.checkInterval = c->check_interval > 0 ? c->check_interval : 8;
+    if (c->launch_mode != MRFG_LAUNCH_GRAPH && c->launch_mode != MRFG_LAUNCH_KERNELS) throw ArgError("unknown launch mode");
+    s.graphs = c->launch_mode == MRFG_LAUNCH_GRAPH;
+    if (c->tuning != MRFG_TUNE_AUTO && c->tuning != MRFG_TUNE_FIXED) throw ArgError("unknown tuning mode");
+    s.autotune = c->tuning == MRFG_TUNE_AUTO;
     if (s.maxIter < 0 || s.minIter < 0 || s.innerMaxIter < 1 || s.maxOuter < 1) throw ArgError("bad iteration limits");
     return s;
 }
@@ -721,6 +727,32 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w)
     });
 }
 
+int mrfg_tuning(mrfg_ctx *ctx, int *tuned, int *block_threads, int *ell_chunk, double *spmv_ms,
+                double *spmv_ms_default)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        const mrfg::Tuning &t = committedBatch(ctx).tuning();
+        if (tuned) *tuned = t.tuned ? 1 : 0;
+        if (block_threads) *block_threads = 64 * t.slices;
+        if (ell_chunk) *ell_chunk = t.ellChunk;
+        if (spmv_ms) *spmv_ms = t.bestMs;
+        if (spmv_ms_default) *spmv_ms_default = t.defaultMs;
+    });
+}
+
+int mrfg_set_tuning(mrfg_ctx *ctx, int block_threads, int ell_chunk)
+{
+    if (!ctx) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        if (block_threads != 256 && block_threads != 512 && block_threads != 1024)
+            throw ArgError("block_threads must be 256, 512 or 1024");
+        if (ell_chunk != 2 && ell_chunk != 4 && ell_chunk != 8 && ell_chunk != 16)
+            throw ArgError("ell_chunk must be 2, 4, 8 or 16");
+        committedBatch(ctx).forceTuning(block_threads / 64, ell_chunk);
+    });
+}
+
 int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b)
 {
     if (!ctx || !q_a || !q_b) return MRFG_ERR_ARG;
@@ -898,16 +930,17 @@ struct mrfg_server {
 };
 
 int mrfg_server_create(mrfg_server **out, int device, long long hbm_budget_bytes, int n_workers, int max_batch,
-                       const char *fv_solution, const char *entry, int keep_fields)
+                       int n_prep, const char *fv_solution, const char *entry, int keep_fields)
 {
-    if (!out || !fv_solution || !entry || hbm_budget_bytes <= 0 || n_workers < 1 || max_batch < 1)
+    if (!out || !fv_solution || !entry || hbm_budget_bytes <= 0 || n_workers < 1 || max_batch < 1 || n_prep < 0)
         return MRFG_ERR_ARG;
     *out = nullptr;
     try {
         mrfg::Dict d = mrfg::parseDict(fv_solution);
         mrfg::SolverControls c = mrfg::parseControls(d.subDict("solvers").subDict(entry));
         auto *s = new mrfg_server;
-        s->srv.reset(new mrfg::Server(device, (size_t)hbm_budget_bytes, n_workers, max_batch, c, keep_fields != 0));
+        s->srv.reset(new mrfg::Server(device, (size_t)hbm_budget_bytes, n_workers, max_batch, c, keep_fields != 0,
+                                      n_prep));
         *out = s;
         return MRFG_OK;
     } catch (const mrfg::DictError &) {
@@ -935,23 +968,15 @@ int mrfg_server_wait_all(mrfg_server *s)
 
 int mrfg_server_result(mrfg_server *s, int id, mrfg_perf *perf, double *times, int *batch_size)
 {
-    if (!s || id < 0 || id >= s->srv->nRequests()) return MRFG_ERR_ARG;
-    const mrfg::ServeRequest &R = s->srv->request(id);
+    if (!s) return MRFG_ERR_ARG;
+    mrfg::ServeResult R;   // a copy taken under the server's lock
+    if (!s->srv->result(id, R)) return MRFG_ERR_ARG;
     if (R.status == -1) {
         s->err = R.error;
         return MRFG_ERR_DEVICE;
     }
     if (R.status != 2) return MRFG_ERR_STATE;
-    if (perf) {
-        perf->initial_residual = R.perf.initialResidual;
-        perf->final_residual = R.perf.finalResidual;
-        perf->n_iterations = R.perf.nIterations;
-        perf->converged = R.perf.converged;
-        perf->singular = R.perf.singular;
-        perf->n_outer = R.perf.nOuter;
-        perf->seconds = R.perf.seconds;
-        perf->bytes_moved = R.perf.bytes;
-    }
+    if (perf) fillPerf(std::vector<mrfg::SolvePerf>{R.perf}, perf);
     if (times) {
         times[0] = R.tSubmit;
         times[1] = R.tAdmit;
@@ -963,11 +988,8 @@ int mrfg_server_result(mrfg_server *s, int id, mrfg_perf *perf, double *times, i
 
 int mrfg_server_get_psi(mrfg_server *s, int id, int region, double *psi)
 {
-    if (!s || !psi || id < 0 || id >= s->srv->nRequests()) return MRFG_ERR_ARG;
-    const mrfg::ServeRequest &R = s->srv->request(id);
-    if (R.status != 2 || region < 0 || region >= (int)R.psi.size()) return MRFG_ERR_STATE;
-    std::copy(R.psi[region].begin(), R.psi[region].end(), psi);
-    return MRFG_OK;
+    if (!s || !psi) return MRFG_ERR_ARG;
+    return s->srv->copyPsi(id, region, psi) ? MRFG_OK : MRFG_ERR_STATE;
 }
 
 const char *mrfg_server_last_error(const mrfg_server *s) { return s ? s->err.c_str() : ""; }

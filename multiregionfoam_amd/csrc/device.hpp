@@ -49,6 +49,7 @@ struct MatView {
     const int32_t *ifPairOff;
     const int32_t *ifPairCol;
     const T *ifPairW;
+    int ellChunk;   // ELL entries loaded together per row (2, 4, 8 or 16; the launchers' template choice)
 };
 
 }  // namespace mrfg

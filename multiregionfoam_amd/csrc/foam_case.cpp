@@ -816,6 +816,7 @@ struct Options {
     std::string fieldDir;                    // under 0/<region>/ ("" = directly)
     std::string interfaceFile;               // under constant/
     std::string ddt;                         // "" = from fvSchemes
+    std::string div;                         // "" = div(phi,T) from fvSchemes; "upwind" = assemble upwind
     double deltaT = 0.0;                     // 0 = from controlDict
 };
 
@@ -1078,6 +1079,32 @@ std::vector<std::pair<std::string, std::string>> regionList(const std::string &d
 
 }  // namespace
 
+// The div(phi,T) entry of an fvSchemes dictionary (else its default), as
+// words joined by spaces; "" when the file names no convection scheme.
+static std::string divScheme(const Dict &fvSchemes)
+{
+    if (!fvSchemes.isDict("divSchemes")) return "";
+    const Dict &d = fvSchemes.subDict("divSchemes");
+    auto words = [](const std::vector<Value> &toks, size_t from) {
+        std::string out;
+        for (size_t i = from; i < toks.size(); ++i) {
+            if (!toks[i].isWord()) continue;
+            if (!out.empty()) out += " ";
+            out += toks[i].text;
+        }
+        return out;
+    };
+    std::string def, named;
+    for (const Entry &e : d.entries) {
+        if (e.key == "default") def = words(e.tokens, 0);
+        // "div(phi,T)" reads as the word div followed by the list (phi,T)
+        if (e.key == "div" && !e.tokens.empty() && e.tokens[0].isList() && e.tokens[0].items.size() == 1 &&
+            e.tokens[0].items[0].text == "phi,T")
+            named = words(e.tokens, 1);
+    }
+    return named.empty() ? def : named;
+}
+
 CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsText, FoamCaseInfo *info)
 {
     Options opt;
@@ -1086,6 +1113,8 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
         opt.fieldDir = o.wordOrDefault("fieldDir", "");
         opt.interfaceFile = o.wordOrDefault("interfaceProperties", "");
         opt.ddt = o.wordOrDefault("ddt", "");
+        opt.div = o.wordOrDefault("div", "");
+        if (!opt.div.empty() && opt.div != "upwind") throw FoamError("option div must be upwind, not '" + opt.div + "'");
         opt.deltaT = o.numberOrDefault("deltaT", 0.0);
     }
     auto regions = regionList(dir);
@@ -1107,6 +1136,19 @@ CoupledSystem readFoamCase(const std::string &dir, const std::string &optionsTex
         if (ddt == "steadyState") rDeltaT[r] = 0.0;
         else if (ddt == "Euler" || ddt == "backward") rDeltaT[r] = 1.0 / deltaT;   // backward: first step
         else throw FoamError("ddt scheme '" + ddt + "' is not supported (steadyState, Euler, backward)");
+        // convection (transportTemperature.C:133-139, fvm::div(phi, T)): the
+        // upwind part is assembled; a scheme with an explicit correction
+        // (linearUpwind, linear, ...) is refused unless the caller asks for
+        // upwind in its place
+        if (R[r].transport && opt.div.empty()) {
+            std::string fs = dir + "/system/" + regions[r].first + "/fvSchemes";
+            if (!fileExists(fs)) fs = dir + "/system/fvSchemes";
+            const std::string sch = divScheme(parseDict(readFile(fs)));
+            if (!sch.empty() && sch != "Gauss upwind" && sch != "bounded Gauss upwind")
+                throw FoamError("region '" + regions[r].first + "': div(phi,T) scheme '" + sch +
+                                "' is not assembled (Gauss upwind only; the option div upwind assembles upwind "
+                                "convection in its place)");
+        }
     }
 
     // monolithic interfaces

@@ -9,6 +9,8 @@
 // LDU face loop it restates.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dd.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
@@ -17,8 +19,8 @@ namespace mrfg {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kEllChunk = 8;      // ELL entries loaded together per SpMV row
+constexpr int kBlock = 256;       // interface kernels
+constexpr int kMaxBlock = 1024;   // batch kernels: blocks of 4, 8 or 16 slices (autotuned)
 constexpr double kGreat = 1e20;   // lduMatrix::great_
 constexpr double kSmall = 1e-20;  // lduMatrix::small_
 
@@ -33,8 +35,8 @@ constexpr double kSmall = 1e-20;  // lduMatrix::small_
 template <int NV>
 __device__ void blockPartial(double (&v)[NV], double *partials)
 {
-    __shared__ DD sh[kBlock / 64][NV];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ DD sh[kMaxBlock / 64][NV];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     DD d[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) d[j] = waveSum(DD{v[j], 0.0});
@@ -44,7 +46,7 @@ __device__ void blockPartial(double (&v)[NV], double *partials)
     if (threadIdx.x == 0)
         for (int j = 0; j < NV; ++j) {
             DD t = sh[0][j];
-            for (int w = 1; w < kBlock / 64; ++w) t = ddAdd(t, sh[w][j]);
+            for (int w = 1; w < nw; ++w) t = ddAdd(t, sh[w][j]);
             partials[((size_t)blockIdx.x * 4 + j) * 2] = t.hi;
             partials[((size_t)blockIdx.x * 4 + j) * 2 + 1] = t.lo;
         }
@@ -204,27 +206,28 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(int stage, const SysInfo
     }
 }
 
-template <typename T>
+template <typename T, int CH>
 __device__ inline T spmvRow(const MatView<T> &A, int s, int lane, int p, const T *__restrict__ x)
 {
     T y = A.diag[p] * x[p];
     const int e0 = A.ellOff[s], e1 = A.ellOff[s + 1];
-    // chunks of kEllChunk entries: every column, value and gathered x of a
-    // chunk is loaded before the first product (loads at clamped entries, so
-    // they are unconditional), then the products are added in entry order
-    for (int k0 = e0; k0 < e1; k0 += kEllChunk) {
-        int col[kEllChunk];
-        T val[kEllChunk], xv[kEllChunk];
+    // chunks of CH entries (CH autotuned per batch): every column, value and
+    // gathered x of a chunk is loaded before the first product (loads at
+    // clamped entries, so they are unconditional), then the products are
+    // added in entry order -- the row sum does not depend on CH
+    for (int k0 = e0; k0 < e1; k0 += CH) {
+        int col[CH];
+        T val[CH], xv[CH];
 #pragma unroll
-        for (int j = 0; j < kEllChunk; ++j) {
+        for (int j = 0; j < CH; ++j) {
             const size_t e = (size_t)min(k0 + j, e1 - 1) * 64 + lane;
             col[j] = A.ellCol[e];
             val[j] = A.ellVal[e];
         }
 #pragma unroll
-        for (int j = 0; j < kEllChunk; ++j) xv[j] = x[col[j]];
+        for (int j = 0; j < CH; ++j) xv[j] = x[col[j]];
 #pragma unroll
-        for (int j = 0; j < kEllChunk; ++j) {
+        for (int j = 0; j < CH; ++j) {
             const T t = y + val[j] * xv[j];
             y = k0 + j < e1 ? t : y;
         }
@@ -257,16 +260,16 @@ __device__ inline bool blockRow(const BlockEntry *blocks, int &sys, int &s, int 
 // SpMV family
 // ---------------------------------------------------------------------------
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv_plain(MatView<T> A, const BlockEntry *blocks, const T *x, T *y)
+template <typename T, int CH>
+__global__ __launch_bounds__(kMaxBlock) void k_spmv_plain(MatView<T> A, const BlockEntry *blocks, const T *x, T *y)
 {
     int sys, s, lane, p;
-    if (blockRow(blocks, sys, s, lane, p)) y[p] = spmvRow(A, s, lane, p, x);
+    if (blockRow(blocks, sys, s, lane, p)) y[p] = spmvRow<T, CH>(A, s, lane, p, x);
 }
 
 // Setup: Ax = A x0, r = b - Ax, rw = r, p = v = 0; reduce sum(x0).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_setup_residual(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
+template <typename T, int CH>
+__global__ __launch_bounds__(kMaxBlock) void k_setup_residual(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
                                                          SysState *st, double *partials, const T *x,
                                                          const T *b, T *Ax, T *r)
 {
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_residual(MatView<T> A, const B
     if (st[sys].status != kRunning) return;
     double v[1] = {0.0};
     if (ok) {
-        T ax = spmvRow(A, s, lane, p, x);
+        T ax = spmvRow<T, CH>(A, s, lane, p, x);
         Ax[p] = ax;
         r[p] = b[p] - ax;
         v[0] = (double)x[p];
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_residual(MatView<T> A, const B
 // normFactor (lduMatrix::solver::normFactor) and the initial residual; also
 // the first iteration's rho and beta (BiCGStab) or rho seed (PCG).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_setup_norm(const BlockEntry *blocks, const SysInfo *info, SysState *st,
+__global__ __launch_bounds__(kMaxBlock) void k_setup_norm(const BlockEntry *blocks, const SysInfo *info, SysState *st,
                                                      double *partials, const double *sumA, const T *Ax,
                                                      const T *b, T *r, T *rw, T *pv, T *vv, int isPCG)
 {
@@ -309,8 +312,8 @@ __global__ __launch_bounds__(kBlock) void k_setup_norm(const BlockEntry *blocks,
 }
 
 // v = A ph, alpha = rho / (rw . v)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv_alpha(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
+template <typename T, int CH>
+__global__ __launch_bounds__(kMaxBlock) void k_spmv_alpha(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
                                                      SysState *st, double *partials, const T *ph,
                                                      const T *rw, T *v)
 {
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_alpha(MatView<T> A, const Block
     if (st[sys].status != kRunning) return;
     double acc[1] = {0.0};
     if (ok) {
-        T y = spmvRow(A, s, lane, p, ph);
+        T y = spmvRow<T, CH>(A, s, lane, p, ph);
         v[p] = y;
         acc[0] = (double)rw[p] * (double)y;
     }
@@ -327,8 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_alpha(MatView<T> A, const Block
 }
 
 // t = A sh, omega = (t . s) / (t . t)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv_omega(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
+template <typename T, int CH>
+__global__ __launch_bounds__(kMaxBlock) void k_spmv_omega(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
                                                      SysState *st, double *partials, const T *sh,
                                                      const T *sv, T *t)
 {
@@ -337,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(MatView<T> A, const Block
     if (st[sys].status != kRunning) return;
     double acc[2] = {0.0, 0.0};
     if (ok) {
-        T y = spmvRow(A, s, lane, p, sh);
+        T y = spmvRow<T, CH>(A, s, lane, p, sh);
         t[p] = y;
         double yd = (double)y;
         acc[0] = yd * (double)sv[p];
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_omega(MatView<T> A, const Block
 
 // p = r + beta*p - beta*omega*v
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bicg_p(const BlockEntry *blocks, const SysState *st, const T *r, T *pv,
+__global__ __launch_bounds__(kMaxBlock) void k_bicg_p(const BlockEntry *blocks, const SysState *st, const T *r, T *pv,
                                                  const T *v)
 {
     int sys, s, lane, p;
@@ -360,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_p(const BlockEntry *blocks, con
 
 // s = r - alpha*v
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bicg_s(const BlockEntry *blocks, const SysState *st, const T *r,
+__global__ __launch_bounds__(kMaxBlock) void k_bicg_s(const BlockEntry *blocks, const SysState *st, const T *r,
                                                  const T *v, T *sv)
 {
     int sys, s, lane, p;
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_s(const BlockEntry *blocks, con
 
 // x = x + alpha*ph + omega*sh; r = s - omega*t; residual and next rho
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bicg_xr(const BlockEntry *blocks, const SysInfo *info, SysState *st,
+__global__ __launch_bounds__(kMaxBlock) void k_bicg_xr(const BlockEntry *blocks, const SysInfo *info, SysState *st,
                                                   double *partials, T *x, T *r, const T *ph,
                                                   const T *sh, const T *sv, const T *t, const T *rw)
 {
@@ -392,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_xr(const BlockEntry *blocks, co
 
 // PCG: rho = w . r, beta
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_rho(const BlockEntry *blocks, const SysInfo *info, SysState *st,
+__global__ __launch_bounds__(kMaxBlock) void k_pcg_rho(const BlockEntry *blocks, const SysInfo *info, SysState *st,
                                                   double *partials, const T *w, const T *r)
 {
     int sys, s, lane, p;
@@ -405,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_rho(const BlockEntry *blocks, co
 
 // PCG: pA = wA (first iteration) or wA + beta*pA
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_p(const BlockEntry *blocks, const SysState *st, const T *w, T *pA)
+__global__ __launch_bounds__(kMaxBlock) void k_pcg_p(const BlockEntry *blocks, const SysState *st, const T *w, T *pA)
 {
     int sys, s, lane, p;
     bool ok = blockRow(blocks, sys, s, lane, p);
@@ -415,8 +418,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p(const BlockEntry *blocks, cons
 }
 
 // PCG: wA = A pA, alpha = rho / (wA . pA)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_spmv_pcg(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
+template <typename T, int CH>
+__global__ __launch_bounds__(kMaxBlock) void k_spmv_pcg(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
                                                    SysState *st, double *partials, const T *pA, T *w)
 {
     int sys, s, lane, p;
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pcg(MatView<T> A, const BlockEn
     if (st[sys].status != kRunning) return;
     double acc[1] = {0.0};
     if (ok) {
-        T y = spmvRow(A, s, lane, p, pA);
+        T y = spmvRow<T, CH>(A, s, lane, p, pA);
         w[p] = y;
         acc[0] = (double)y * (double)pA[p];
     }
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pcg(MatView<T> A, const BlockEn
 
 // PCG: x += alpha*pA; r -= alpha*wA; residual
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_xr(const BlockEntry *blocks, const SysInfo *info, SysState *st,
+__global__ __launch_bounds__(kMaxBlock) void k_pcg_xr(const BlockEntry *blocks, const SysInfo *info, SysState *st,
                                                  double *partials, T *x, T *r, const T *pA,
                                                  const T *w)
 {
@@ -479,7 +482,8 @@ __global__ void k_refine_prepare(int nSys, const SysState *outer, SysState *inne
 
 // r = b - A x in FP64 and the outer convergence test against the FP64
 // normFactor of the initial guess.
-__global__ __launch_bounds__(kBlock) void k_refine_residual(MatView<double> A, const BlockEntry *blocks,
+template <int CH>
+__global__ __launch_bounds__(kMaxBlock) void k_refine_residual(MatView<double> A, const BlockEntry *blocks,
                                                             const SysInfo *info, SysState *st, double *partials,
                                                             const double *x, const double *b, double *r)
 {
@@ -488,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(MatView<double> A, c
     if (st[sys].status != kRunning) return;
     double acc[1] = {0.0};
     if (ok) {
-        double rp = b[p] - spmvRow(A, s, lane, p, x);
+        double rp = b[p] - spmvRow<double, CH>(A, s, lane, p, x);
         r[p] = rp;
         acc[0] = fabs(rp);
     }
@@ -496,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_residual(MatView<double> A, c
 }
 
 // b32 = (float) r64 for systems whose outer solve is running; x32 = 0
-__global__ __launch_bounds__(kBlock) void k_refine_rhs(const BlockEntry *blocks, const SysState *outer,
+__global__ __launch_bounds__(kMaxBlock) void k_refine_rhs(const BlockEntry *blocks, const SysState *outer,
                                                        const double *r, float *b32, float *x32)
 {
     int sys, s, lane, p;
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_rhs(const BlockEntry *blocks,
 }
 
 // x64 += (double) d32
-__global__ __launch_bounds__(kBlock) void k_refine_update(const BlockEntry *blocks, const SysState *outer, double *x,
+__global__ __launch_bounds__(kMaxBlock) void k_refine_update(const BlockEntry *blocks, const SysState *outer, double *x,
                                                           const float *d)
 {
     int sys, s, lane, p;
@@ -534,20 +538,35 @@ static void fin(const Krylov<T> &K, hipStream_t st)
     k_finalize<finSums(Stage)><<<K.nSys, kFinBlock, 0, st>>>(Stage, K.info, K.state, K.partials);
 }
 
-template <typename T>
-hipError_t launchSpmvPlain(const MatView<T> &A, const BlockEntry *blocks, int nBlk, const T *x, T *y, hipStream_t st)
+// the SpMV kernels instantiated for the ELL chunk the batch was tuned to
+template <typename F>
+static void withChunk(int ch, F &&f)
 {
-    k_spmv_plain<T><<<nBlk, kBlock, 0, st>>>(A, blocks, x, y);
+    switch (ch) {
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 4: f(std::integral_constant<int, 4>()); break;
+    case 16: f(std::integral_constant<int, 16>()); break;
+    default: f(std::integral_constant<int, 8>()); break;
+    }
+}
+
+template <typename T>
+hipError_t launchSpmvPlain(const MatView<T> &A, const BlockEntry *blocks, int nBlk, int threads, const T *x, T *y,
+                           hipStream_t st)
+{
+    withChunk(A.ellChunk, [&](auto c) { k_spmv_plain<T, decltype(c)::value><<<nBlk, threads, 0, st>>>(A, blocks, x, y); });
     MRFG_CHECK_LAUNCH();
 }
 
 template <typename T>
 hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st)
 {
-    k_setup_residual<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.x, K.b,
-                                                   K.ax, K.r);
+    withChunk(A.ellChunk, [&](auto c) {
+        k_setup_residual<T, decltype(c)::value><<<K.nBlk, K.threads, 0, st>>>(A, K.blocks, K.info, K.state,
+                                                                              K.partials, K.x, K.b, K.ax, K.r);
+    });
     fin<kFinXSum>(K, st);
-    k_setup_norm<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.sumA, K.ax, K.b,
+    k_setup_norm<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.sumA, K.ax, K.b,
                                                K.r, pcg ? nullptr : K.rw, pcg ? K.pA : K.pv, pcg ? nullptr : K.v,
                                                pcg ? 1 : 0);
     if (pcg) fin<kFinInitPcg>(K, st);
@@ -571,7 +590,7 @@ hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const 
     auto mark = [&](int which, bool begin) {
         if (probe && probe->which == which) (void)hipEventRecord(begin ? probe->begin : probe->end, st);
     };
-    k_bicg_p<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
+    k_bicg_p<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
     {
         mark(1, true);
         hipError_t e = diluForward<T>(D, K.state, K.pv, K.ph, st);
@@ -583,17 +602,23 @@ hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const 
         mark(2, false);
     }
     mark(3, true);
-    k_spmv_alpha<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.ph, K.rw, K.v);
+    withChunk(A.ellChunk, [&](auto c) {
+        k_spmv_alpha<T, decltype(c)::value><<<K.nBlk, K.threads, 0, st>>>(A, K.blocks, K.info, K.state, K.partials,
+                                                                          K.ph, K.rw, K.v);
+    });
     mark(3, false);
     fin<kFinAlpha>(K, st);
-    k_bicg_s<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.r, K.v, K.sv);
+    k_bicg_s<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.v, K.sv);
     {
         hipError_t e = launchPrecondition(D, K, K.sv, K.sh, st);
         if (e != hipSuccess) return e;
     }
-    k_spmv_omega<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.sh, K.sv, K.t);
+    withChunk(A.ellChunk, [&](auto c) {
+        k_spmv_omega<T, decltype(c)::value><<<K.nBlk, K.threads, 0, st>>>(A, K.blocks, K.info, K.state, K.partials,
+                                                                          K.sh, K.sv, K.t);
+    });
     fin<kFinOmega>(K, st);
-    k_bicg_xr<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.x, K.r, K.ph, K.sh,
+    k_bicg_xr<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.x, K.r, K.ph, K.sh,
                                             K.sv, K.t, K.rw);
     fin<kFinResBicg>(K, st);
     MRFG_CHECK_LAUNCH();
@@ -606,12 +631,15 @@ hipError_t launchPcgIteration(const MatView<T> &A, const DiluView<T> &D, const K
         hipError_t e = launchPrecondition(D, K, K.r, K.w, st);
         if (e != hipSuccess) return e;
     }
-    k_pcg_rho<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.w, K.r);
+    k_pcg_rho<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.w, K.r);
     fin<kFinPcgRho>(K, st);
-    k_pcg_p<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.state, K.w, K.pA);
-    k_spmv_pcg<T><<<K.nBlk, kBlock, 0, st>>>(A, K.blocks, K.info, K.state, K.partials, K.pA, K.w);
+    k_pcg_p<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.w, K.pA);
+    withChunk(A.ellChunk, [&](auto c) {
+        k_spmv_pcg<T, decltype(c)::value><<<K.nBlk, K.threads, 0, st>>>(A, K.blocks, K.info, K.state, K.partials,
+                                                                        K.pA, K.w);
+    });
     fin<kFinPcgAlpha>(K, st);
-    k_pcg_xr<T><<<K.nBlk, kBlock, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.x, K.r, K.pA, K.w);
+    k_pcg_xr<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.x, K.r, K.pA, K.w);
     fin<kFinResPcg>(K, st);
     MRFG_CHECK_LAUNCH();
 }
@@ -625,21 +653,23 @@ hipError_t launchRefinePrepare(int nSys, const SysState *outer, SysState *inner,
 
 hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &K64, hipStream_t st)
 {
-    k_refine_residual<<<K64.nBlk, kBlock, 0, st>>>(A, K64.blocks, K64.info, K64.state, K64.partials,
-                                                   K64.x, K64.b, K64.r);
+    withChunk(A.ellChunk, [&](auto c) {
+        k_refine_residual<decltype(c)::value><<<K64.nBlk, K64.threads, 0, st>>>(A, K64.blocks, K64.info, K64.state,
+                                                                               K64.partials, K64.x, K64.b, K64.r);
+    });
     fin<kFinRefine>(K64, st);
     MRFG_CHECK_LAUNCH();
 }
 
 hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st)
 {
-    k_refine_rhs<<<K64.nBlk, kBlock, 0, st>>>(K64.blocks, K64.state, K64.r, K32.b, K32.x);
+    k_refine_rhs<<<K64.nBlk, K64.threads, 0, st>>>(K64.blocks, K64.state, K64.r, K32.b, K32.x);
     MRFG_CHECK_LAUNCH();
 }
 
 hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st)
 {
-    k_refine_update<<<K64.nBlk, kBlock, 0, st>>>(K64.blocks, K64.state, K64.x, K32.x);
+    k_refine_update<<<K64.nBlk, K64.threads, 0, st>>>(K64.blocks, K64.state, K64.x, K32.x);
     MRFG_CHECK_LAUNCH();
 }
 
@@ -647,7 +677,7 @@ hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K3
 // Time stepping: the ddt terms of transient systems on the device
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_ddt_diag(MatView<double> A, const BlockEntry *blocks, int K,
+__global__ __launch_bounds__(kMaxBlock) void k_ddt_diag(MatView<double> A, const BlockEntry *blocks, int K,
                                                      double coefft, const double *diagBase, const double *a,
                                                      double *diag, double2 *prodPack, double *sumA)
 {
@@ -670,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void k_ddt_diag(MatView<double> A, const Bl
     sumA[p] = r;
 }
 
-__global__ __launch_bounds__(kBlock) void k_ddt_source(const BlockEntry *blocks, int backward,
+__global__ __launch_bounds__(kMaxBlock) void k_ddt_source(const BlockEntry *blocks, int backward,
                                                        const double *sourceBase, const double *a, const double *x,
                                                        const double *xPrev, double *b)
 {
@@ -809,7 +839,8 @@ hipError_t copyProbe(int device, size_t bytes, int reps, double *ms)
 }
 
 #define MRFG_INSTANTIATE(T)                                                                                          \
-    template hipError_t launchSpmvPlain<T>(const MatView<T> &, const BlockEntry *, int, const T *, T *, hipStream_t); \
+    template hipError_t launchSpmvPlain<T>(const MatView<T> &, const BlockEntry *, int, int, const T *, T *,         \
+                                           hipStream_t);                                                             \
     template hipError_t launchSetup<T>(const MatView<T> &, const Krylov<T> &, bool, hipStream_t);                    \
     template hipError_t launchPrecondition<T>(const DiluView<T> &, const Krylov<T> &, const T *, T *, hipStream_t);   \
     template hipError_t launchBicgIteration<T>(const MatView<T> &, const DiluView<T> &, const Krylov<T> &,           \

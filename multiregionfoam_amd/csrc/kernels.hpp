@@ -13,6 +13,7 @@ template <typename T>
 struct Krylov {
     const BlockEntry *blocks = nullptr;
     int nBlk = 0;
+    int threads = 256;   // 64 per slice of a block (the block table's slices per block)
     int nSys = 0;
     const SysInfo *info = nullptr;
     SysState *state = nullptr;
@@ -25,7 +26,8 @@ struct Krylov {
 };
 
 template <typename T>
-hipError_t launchSpmvPlain(const MatView<T> &A, const BlockEntry *blocks, int nBlk, const T *x, T *y, hipStream_t st);
+hipError_t launchSpmvPlain(const MatView<T> &A, const BlockEntry *blocks, int nBlk, int threads, const T *x, T *y,
+                           hipStream_t st);
 template <typename T>
 hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st);
 template <typename T>

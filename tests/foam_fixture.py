@@ -121,6 +121,16 @@ DNA_T = ("DNA\n{\n    T\n    {\n        maxCoupleIter %d;\n        residualContr
          "            maxJumpRes %s;\n            maxFluxRes %s;\n        }\n    }\n}\n")
 
 
+# the tutorial's convection schemes (system/{fluid,solid}/fvSchemes): the
+# fluid's temperature is convected with linearUpwind, which the reader
+# refuses unless told to assemble upwind in its place (div="upwind")
+DIV_SCHEMES = {
+    "fluid": "divSchemes { default Gauss linear; div(phi,U) Gauss upwind; "
+             "div(phi,T) Gauss linearUpwind Gauss linear; }\n",
+    "solid": "divSchemes { default none; }\n",
+}
+
+
 def write_fixture_case(dest, params=None, ddt=None, partitioned=None):
     """params: the test suite's parameter map (genCases.py update_params),
     dict(kf=..., cp=...): the fluid's conductivity (transportProperties and
@@ -185,7 +195,7 @@ def write_fixture_case(dest, params=None, ddt=None, partitioned=None):
         write_dict(os.path.join(dest, "constant", region, "transportProperties"),
                    "".join("%s %s [0 0 0 0 0 0 0] %s;\n" % (k, k, _num(v)) for k, v in sorted(tp.items())))
         write_dict(os.path.join(dest, "system", region, "fvSchemes"),
-                   "ddtSchemes { default %s; }\n" % meta["ddt"][region])
+                   "ddtSchemes { default %s; }\n%s" % (meta["ddt"][region], DIV_SCHEMES[region]))
         f = meta["fields"][region]
         for name in ("T", "k", "U"):
             if name not in f:

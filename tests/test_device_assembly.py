@@ -30,7 +30,7 @@ FV_SOLUTION = ("solvers { Tcoupled { solver BiCGStab; preconditioner DILU; toler
 def _read(gpu, tmp_path_factory, params=None, ddt="steadyState"):
     d = tmp_path_factory.mktemp("case")
     ff.write_fixture_case(str(d), params=params, ddt=None if ddt == "steadyState" else ddt)
-    return gpu.Case.from_foam(str(d), ddt=ddt)
+    return gpu.Case.from_foam(str(d), ddt=ddt, div="upwind")
 
 
 def _assert_matrices_equal(A, sys, case):

@@ -75,8 +75,8 @@ def test_fixture_rebuilds_reference_case(tutorial_dir):
     _lib_or_skip()
     if not os.path.isdir(REF_CASE):
         pytest.skip("reference case not present")
-    a = mrf.Case.from_foam(REF_CASE)
-    b = mrf.Case.from_foam(tutorial_dir)
+    a = mrf.Case.from_foam(REF_CASE, div="upwind")
+    b = mrf.Case.from_foam(tutorial_dir, div="upwind")
     for x, y in zip(a.regions() + a.interfaces(), b.regions() + b.interfaces()):
         assert x.keys() == y.keys()
         for k in x:
@@ -86,7 +86,7 @@ def test_fixture_rebuilds_reference_case(tutorial_dir):
 def test_tutorial_structure(tutorial_dir):
     _lib_or_skip()
     meta, meshes = ff.load_fixture()
-    c = mrf.Case.from_foam(tutorial_dir)
+    c = mrf.Case.from_foam(tutorial_dir, div="upwind")
     R, I = _arrays(c)
     assert [r["nCells"] for r in R] == [13612, 8200]
     assert [r["nFaces"] for r in R] == [len(meshes["fluid"]["neighbour"]), len(meshes["solid"]["neighbour"])]
@@ -106,7 +106,7 @@ def test_tutorial_geometry_and_coefficients(tutorial_dir):
     tutorial against bounding-box geometry of the fixture mesh."""
     _lib_or_skip()
     meta, meshes = ff.load_fixture()
-    c = mrf.Case.from_foam(tutorial_dir)
+    c = mrf.Case.from_foam(tutorial_dir, div="upwind")
     R, I = _arrays(c)
     dt = meta["deltaT"]
     tp = meta["transportProperties"]
@@ -193,7 +193,7 @@ def test_box_case_matches_native_builder(tmp_path, make):
     case = make()
     ff.write_box_case(str(tmp_path), case)
     a = mrf.Case(case)
-    b = mrf.Case.from_foam(str(tmp_path))
+    b = mrf.Case.from_foam(str(tmp_path), div="upwind")
     for x, y in zip(a.regions(), b.regions()):
         for k in ("nCells", "nFaces", "symmetric"):
             assert x[k] == y[k], k
@@ -216,7 +216,7 @@ def test_tutorial_oracle_solve_matches_direct(tutorial_dir):
     direct sparse solve."""
     _lib_or_skip()
     import scipy.sparse.linalg as sla
-    c = mrf.Case.from_foam(tutorial_dir)
+    c = mrf.Case.from_foam(tutorial_dir, div="upwind")
     O = ob.OracleSystem.from_arrays(*_arrays(c))
     psi, perf = O.solve(tol=1e-13, maxIter=1000)
     assert perf["converged"]
@@ -228,10 +228,10 @@ def test_tutorial_oracle_solve_matches_direct(tutorial_dir):
 def test_read_errors(tmp_path):
     _lib_or_skip()
     with pytest.raises(mrf.MrfgError, match="multiRegionProperties"):
-        mrf.Case.from_foam(str(tmp_path))
+        mrf.Case.from_foam(str(tmp_path), div="upwind")
     ff.write_box_case(str(tmp_path), _plate_small())
     with pytest.raises(mrf.MrfgError, match="ddt scheme"):
-        mrf.Case.from_foam(str(tmp_path), ddt="CrankNicolson")
+        mrf.Case.from_foam(str(tmp_path), ddt="CrankNicolson", div="upwind")
 
 
 @pytest.mark.gpu
@@ -244,7 +244,7 @@ def test_tutorial_gpu_matches_oracle(gpu, tutorial_dir):
     fvs = ("solvers { Tcoupled { solver %s; preconditioner { preconditioner %s; } tolerance %s; relTol %s; "
            "minIter %s; maxIter %s; } }" % (t["solver"], t["preconditioner"], t["tolerance"], t["relTol"],
                                               t["minIter"], t["maxIter"]))
-    c = gpu.Case.from_foam(tutorial_dir)
+    c = gpu.Case.from_foam(tutorial_dir, div="upwind")
     A = gpu.CoupledFvMatrix(0)
     A.add_case(c)
     A.commit()
@@ -285,7 +285,7 @@ def test_ddt_split(tutorial_dir):
     a = rho*c*V/dt from bounding-box cell volumes."""
     _lib_or_skip()
     meta, meshes = ff.load_fixture()
-    c = mrf.Case.from_foam(tutorial_dir)
+    c = mrf.Case.from_foam(tutorial_dir, div="upwind")
     tp = meta["transportProperties"]
     for g, (region, R) in enumerate(zip(("fluid", "solid"), c.regions())):
         assert np.array_equal(R["diag"], R["diagBase"] + R["ddtCoeff"])
@@ -293,7 +293,7 @@ def test_ddt_split(tutorial_dir):
         _, _, clo, chi = _hex_geometry(meshes[region])
         rhoC = tp[region]["rho"] * (tp[region]["cp"] if region == "fluid" else tp[region]["cv"])
         np.testing.assert_allclose(R["ddtCoeff"], rhoC * np.prod(chi - clo, axis=1) / meta["deltaT"], rtol=1e-12)
-    s = mrf.Case.from_foam(tutorial_dir, ddt="steadyState")
+    s = mrf.Case.from_foam(tutorial_dir, ddt="steadyState", div="upwind")
     assert all("ddtCoeff" not in R for R in s.regions())
 
 
@@ -303,7 +303,7 @@ def test_time_steps_conserve_energy(tmp_path, scheme):
     sum(a (3 T_n+1 - T_n) / 2) from step to step, to the solver tolerance."""
     _lib_or_skip()
     ff.write_box_case(str(tmp_path), _conduct_pair(), deltaT=0.05)
-    c = mrf.Case.from_foam(str(tmp_path), ddt=scheme)
+    c = mrf.Case.from_foam(str(tmp_path), ddt=scheme, div="upwind")
     O = ob.OracleSystem.from_arrays(*_arrays(c))
     ddt = _ddt(c)
     a = np.concatenate([D["ddtCoeff"] for D in ddt])
@@ -323,10 +323,10 @@ def test_time_steps_reach_steady_state(tutorial_dir):
     tutorial approach the steady system's solution, the first one to 1e-6."""
     _lib_or_skip()
     import scipy.sparse.linalg as sla
-    s = mrf.Case.from_foam(tutorial_dir, ddt="steadyState")
+    s = mrf.Case.from_foam(tutorial_dir, ddt="steadyState", div="upwind")
     Os = ob.OracleSystem.from_arrays(*_arrays(s))
     Ts = sla.spsolve(Os.to_scipy().tocsc(), np.concatenate([r["source"] for r in Os.regions]))
-    c = mrf.Case.from_foam(tutorial_dir, ddt="Euler", deltaT=1e6)
+    c = mrf.Case.from_foam(tutorial_dir, ddt="Euler", deltaT=1e6, div="upwind")
     O = ob.OracleSystem.from_arrays(*_arrays(c))
     steps = O.time_steps(_ddt(c), 2, scheme="Euler", tol=1e-15, maxIter=2000)
     err = [np.linalg.norm(np.concatenate(p) - Ts) / np.linalg.norm(Ts) for p, _ in steps]
@@ -342,7 +342,7 @@ def test_time_steps_gpu_match_oracle(gpu, tutorial_dir):
     t = meta["Tcoupled"]
     kw = dict(tol=float(t["tolerance"]), relTol=float(t["relTol"]), minIter=int(t["minIter"]),
               maxIter=int(t["maxIter"]))
-    c = gpu.Case.from_foam(tutorial_dir)
+    c = gpu.Case.from_foam(tutorial_dir, div="upwind")
     A = gpu.CoupledFvMatrix(0)
     A.add_case(c)
     A.commit()
@@ -390,8 +390,8 @@ def test_rotated_case_assembles_the_same(tmp_path, make):
     case = make()
     ff.write_box_case(str(tmp_path / "a"), case)
     ff.write_box_case(str(tmp_path / "b"), case, rotate=((1.0, 2.0, 3.0), 37.0))
-    a = mrf.Case.from_foam(str(tmp_path / "a"))
-    b = mrf.Case.from_foam(str(tmp_path / "b"))
+    a = mrf.Case.from_foam(str(tmp_path / "a"), div="upwind")
+    b = mrf.Case.from_foam(str(tmp_path / "b"), div="upwind")
     for x, y in zip(a.regions(), b.regions()):
         for k in ("lAddr", "uAddr"):
             assert np.array_equal(x[k], y[k])
@@ -413,7 +413,7 @@ def test_ggi_area_identity(tmp_path, rotate):
     _lib_or_skip()
     case = _covering_pair()
     ff.write_box_case(str(tmp_path), case, rotate=rotate)
-    I = mrf.Case.from_foam(str(tmp_path)).interfaces()[0]
+    I = mrf.Case.from_foam(str(tmp_path), div="upwind").interfaces()[0]
     area = {"A": 0.1 * 0.1, "B": (0.6 / 7) * (0.4 / 5)}   # fluid ymin faces, solid ymax faces
     for s, o in (("A", "B"), ("B", "A")):
         off, idx, w = I["off" + s], I["idx" + s], I["w" + s]
@@ -422,3 +422,34 @@ def test_ggi_area_identity(tmp_path, rotate):
         np.add.at(sent, idx, area[s] * w)
         np.testing.assert_allclose(sent, area[o], rtol=1e-12)
     assert np.diff(I["offA"]).max() > 1 and np.diff(I["offB"]).max() > 1   # non-conformal
+
+
+def test_convection_scheme_is_read_not_assumed(tutorial_dir, tmp_path):
+    """The tutorial convects T with `Gauss linearUpwind Gauss linear`
+    (system/fluid/fvSchemes): its explicit correction is not assembled, so
+    the reader refuses the case rather than assembling upwind silently; the
+    caller may ask for upwind in its place (div="upwind"), and a case naming
+    Gauss upwind, or no convection scheme, reads as it did."""
+    _lib_or_skip()
+    with pytest.raises(mrf.MrfgError, match="linearUpwind"):
+        mrf.Case.from_foam(tutorial_dir)
+    if os.path.isdir(REF_CASE):
+        with pytest.raises(mrf.MrfgError, match="linearUpwind"):
+            mrf.Case.from_foam(REF_CASE)
+    ref = mrf.Case.from_foam(tutorial_dir, div="upwind").regions()
+    d = tmp_path / "upwind"
+    ff.write_fixture_case(str(d))
+    fs = d / "system" / "fluid" / "fvSchemes"
+    fs.write_text(fs.read_text().replace("Gauss linearUpwind Gauss linear", "Gauss upwind"))
+    got = mrf.Case.from_foam(str(d)).regions()
+    for a, b in zip(got, ref):
+        for k in ("diag", "upper", "lower", "source"):
+            assert np.array_equal(a[k], b[k]), k
+    fs.write_text("ddtSchemes { default backward; }\n")
+    got = mrf.Case.from_foam(str(d)).regions()
+    assert np.array_equal(got[0]["upper"], ref[0]["upper"])
+    fs.write_text("ddtSchemes { default backward; }\ndivSchemes { default Gauss linear; }\n")
+    with pytest.raises(mrf.MrfgError, match="Gauss linear"):
+        mrf.Case.from_foam(str(d))
+    with pytest.raises(mrf.MrfgError, match="div"):
+        mrf.Case.from_foam(str(d), div="linear")

@@ -38,7 +38,7 @@ FV_SOLUTION = "solvers { T { solver BiCGStab; preconditioner DILU; tolerance 1e-
 def _read(tmp_path_factory, partitioned=None, params=None, ddt="steadyState"):
     d = tmp_path_factory.mktemp("case")
     ff.write_fixture_case(str(d), params=params, partitioned=partitioned)
-    return mrf.Case.from_foam(str(d), ddt=ddt)
+    return mrf.Case.from_foam(str(d), ddt=ddt, div="upwind")
 
 
 def _ddt(case):
@@ -141,12 +141,12 @@ def test_reader_errors(tmp_path_factory):
     d = tmp_path_factory.mktemp("bad")
     ff.write_fixture_case(str(d), partitioned=dict(accType="broyden", relax=0.75))
     with pytest.raises(mrf.MrfgError, match="broyden"):
-        mrf.Case.from_foam(str(d), ddt="steadyState")
+        mrf.Case.from_foam(str(d), ddt="steadyState", div="upwind")
     d = tmp_path_factory.mktemp("reuse")
     ff.write_fixture_case(str(d), partitioned=dict(accType="IQN-ILS", relax=0.75, couplingReuse=2))
     with pytest.raises(mrf.MrfgError, match="couplingReuse"):
-        mrf.Case.from_foam(str(d), ddt="backward")
-    mrf.Case.from_foam(str(d), ddt="steadyState")   # one time step: no columns to reuse
+        mrf.Case.from_foam(str(d), ddt="backward", div="upwind")
+    mrf.Case.from_foam(str(d), ddt="steadyState", div="upwind")   # one time step: no columns to reuse
 
 
 # ---------------------------------------------------------------------------

@@ -144,10 +144,10 @@ def test_gpu_patch_values_follow_device_assembly(gpu, tmp_path_factory):
     import foam_fixture as ff
     d0 = tmp_path_factory.mktemp("pv0")
     ff.write_fixture_case(str(d0))
-    base = gpu.Case.from_foam(str(d0), ddt="steadyState")
+    base = gpu.Case.from_foam(str(d0), ddt="steadyState", div="upwind")
     d1 = tmp_path_factory.mktemp("pv1")
     ff.write_fixture_case(str(d1), params=dict(kf=2.5, cp=1000.0))
-    ref = gpu.Case.from_foam(str(d1), ddt="steadyState")
+    ref = gpu.Case.from_foam(str(d1), ddt="steadyState", div="upwind")
     A = gpu.CoupledFvMatrix(0)
     A.add_case(base)
     A.commit()

@@ -70,7 +70,7 @@ def _curve(Pr, Re, k, kind):
 def _read(tmp_path_factory, Pr, Re, k):
     d = tmp_path_factory.mktemp("Pr%sRe%sk%s" % (_tag(Pr), Re, k))
     ff.write_fixture_case(str(d), params=suite_params(Pr, Re, k))
-    return mrf.Case.from_foam(str(d), ddt="steadyState")
+    return mrf.Case.from_foam(str(d), ddt="steadyState", div="upwind")
 
 
 def _sample_cells():
@@ -137,7 +137,7 @@ def test_parameters_enter_the_fluid_matrix_linearly(tmp_path_factory):
     for kf, cp in ((2.0, 3.0), (5.0, 7.0), (7.0, 10.0)):
         p = tmp_path_factory.mktemp("lin")
         ff.write_fixture_case(str(p), params=dict(kf=kf, cp=cp))
-        d[kf] = mrf.Case.from_foam(str(p), ddt="steadyState").regions()
+        d[kf] = mrf.Case.from_foam(str(p), ddt="steadyState", div="upwind").regions()
     a, b, s = d[2.0], d[5.0], d[7.0]
     for key in ("lower", "upper", "source"):
         np.testing.assert_allclose(a[0][key] + b[0][key], s[0][key], rtol=1e-12, atol=1e-10)
@@ -219,7 +219,7 @@ def test_suite_assembled_on_device_matches_oracle(gpu, suite, tmp_path_factory):
     oracle's solve of the case read from disk."""
     d = tmp_path_factory.mktemp("base")
     ff.write_fixture_case(str(d))
-    base = gpu.Case.from_foam(str(d), ddt="steadyState")
+    base = gpu.Case.from_foam(str(d), ddt="steadyState", div="upwind")
     n = base.regions()[0]["nCells"]
     A = gpu.CoupledFvMatrix(0)
     for _ in CASES:
