@@ -272,6 +272,19 @@ int mrfg_interface_flux(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *
    of that side.  MRFG_ERR_ARG if the interface has no wall conductances. */
 int mrfg_interface_patch_values(mrfg_ctx *ctx, int sys, int iface, int side, double *t_neighbour, double *t_wall);
 
+/* The FP64 interface heat-flux check of mrfg_interface_flux without any
+   device allocation: no counterpart in the reference; for callers that
+   cannot allocate device memory (code under HIP graph capture, tight
+   serving or time-stepping loops, harnesses that forbid hipMalloc).  Same
+   result as mrfg_interface_flux bit for bit.  The reduction partials and
+   the two results live in a process-lifetime device array of 256 KiB per
+   GPU (taken at mrfg_create), so each side of the interface may have at
+   most 1,048,576 faces (MRFG_ERR_ARG beyond); every call on one GPU shares
+   that array, so calls that overlap in time on one GPU (other threads or
+   contexts) race and must be serialised by the caller.  The interface's
+   face lists are resident from mrfg_commit. */
+int mrfg_interface_flux_noalloc(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b);
+
 /* ---- finite-volume assembly on the device (cases read from disk) ------ */
 
 /* Replace one field of region `region` of committed system `sys`, a case

@@ -130,6 +130,7 @@ def lib():
         "mrfg_bytes": ([vp, C.c_int, C.c_int, F64P, F64P, F64P], C.c_int),
         "mrfg_copy_probe": ([vp, C.c_longlong, C.c_int, F64P], C.c_int),
         "mrfg_interface_flux": ([vp, C.c_int, C.c_int, F64P, F64P], C.c_int),
+        "mrfg_interface_flux_noalloc": ([vp, C.c_int, C.c_int, F64P, F64P], C.c_int),
         "mrfg_set_field": ([vp, C.c_int, C.c_int, C.c_char_p, F64P, C.c_longlong], C.c_int),
         "mrfg_assemble": ([vp], C.c_int),
         "mrfg_assembly_bytes": ([vp, F64P, F64P], C.c_int),
@@ -162,7 +163,7 @@ EXPORTED = (
     "mrfg_case_region_source_base "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_case_interface_wall mrfg_set_interface_wall "
     "mrfg_interface_patch_values mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_tuning mrfg_set_tuning mrfg_bytes mrfg_copy_probe mrfg_interface_flux "
+    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_tuning mrfg_set_tuning mrfg_bytes mrfg_copy_probe mrfg_interface_flux mrfg_interface_flux_noalloc "
     "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
     "mrfg_server_get_psi mrfg_server_last_error").split()

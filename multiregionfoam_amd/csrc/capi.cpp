@@ -10,6 +10,7 @@
 
 #include "batch.hpp"
 #include "dna_run.hpp"
+#include "noalloc.hpp"
 #include "serve.hpp"
 #include "coupled_system.hpp"
 #include "dictionary.hpp"
@@ -165,6 +166,7 @@ int mrfg_create(mrfg_ctx **ctx, int device)
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw std::runtime_error("HIP error: no HIP device");
         if (device < 0 || device >= n) throw ArgError("device index out of range");
         c->batch = std::make_unique<mrfg::Batch>(device);
+        (void)mrfg::fluxScratch(device);   // the allocation-free flux check's scratch, once per device
     });
     if (rc != MRFG_OK) return rc;
     *ctx = c.release();
@@ -803,6 +805,20 @@ int mrfg_interface_patch_values(mrfg_ctx *ctx, int sys, int iface, int side, dou
             throw ArgError("interface index out of range");
         if (side < 0 || side > 1) throw ArgError("side must be 0 (a) or 1 (b)");
         B.interfacePatchValues(sys, iface, side, t_neighbour, t_wall);
+    });
+}
+
+int mrfg_interface_flux_noalloc(mrfg_ctx *ctx, int sys, int iface, double *q_a, double *q_b)
+{
+    if (!ctx || !q_a || !q_b) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        mrfg::Batch &B = committedBatch(ctx);
+        if (iface < 0 || iface >= (int)ctx->systems[sys]->interfaces.size())
+            throw ArgError("interface index out of range");
+        if (hipSetDevice(ctx->device) != hipSuccess) throw std::runtime_error("HIP error: hipSetDevice");
+        double *scratch = mrfg::fluxScratch(ctx->device);   // taken at mrfg_create; cached per device
+        B.interfaceFluxScratch(sys, iface, q_a, q_b, scratch, mrfg::kFluxScratchDoubles);
     });
 }
 
