@@ -518,22 +518,30 @@ def run_batch(m, cases, args):
 
 def run_serving(m, cases, args):
     """Poisson arrivals at 50 req/s of two-region 3-D solves (20k-2M cells,
-    half GGI) submitted on their arrival times to the serving engine; latency
-    = completion - arrival, throughput = requests / (last completion - first
-    arrival); FP64 and mixed runs of the same stream."""
+    half GGI) submitted on their arrival times to the serving engine
+    (preparation threads assemble and lay out each request; device slots,
+    each a resident batch on its own stream, admit prepared requests by HBM
+    footprint and retire each at the convergence check that finds it done).
+    latency = completion - arrival; throughput = requests / (last completion
+    - first arrival), and over the steady middle of the run (requests
+    arriving between 25 % and 75 % of the arrival span, completions in that
+    window); FP64 and mixed runs of the same stream."""
     import numpy as np
     stream = cases.serving_stream(args.requests)
     texts = [(t, cases.to_dict_text(c)) for t, c in stream]
     cells = [sum(int(np.prod(r["nCells"])) for r in c["regions"]) for _, c in stream]
+    span = texts[-1][0]
     res = {}
     for prec in ("fp64", "mixed"):
-        srv = m.Server(device=0, hbm_budget=200 << 30, workers=8, max_batch=8, tolerance=1e-10,
-                       maxIter=3000, precision=prec, innerRelTol=1e-4)
-        # warm-up: one request through every worker
-        for _ in range(8):
+        if prec == "mixed" and args.no_mixed:
+            continue
+        srv = m.Server(device=0, hbm_budget=200 << 30, workers=args.workers, max_batch=args.max_batch,
+                       prep_threads=args.prep, tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=1e-4)
+        # warm-up: one small request through every slot
+        for _ in range(args.workers):
             srv.submit(texts[0][1])
         srv.wait_all()
-        base = 8
+        base = args.workers
         t0 = time.perf_counter()
         for t_arr, txt in texts:
             dt = t0 + t_arr - time.perf_counter()
@@ -542,25 +550,37 @@ def run_serving(m, cases, args):
             srv.submit(txt)
         srv.wait_all()
         rs = [srv.result(base + i) for i in range(len(texts))]
+        tm = [srv.times(base + i) for i in range(len(texts))]
+        mean_ms = lambda a, b: 1e3 * float(np.mean([t[b] - t[a] for t in tm]))
         lat = [r.latency for r in rs]
         first = min(r.t_submit for r in rs)
         last = max(r.t_done for r in rs)
-        res[prec] = dict(requests_s=len(rs) / (last - first), p50_ms=1e3 * pct(lat, 0.5),
-                         p99_ms=1e3 * pct(lat, 0.99), mean_batch=float(np.mean([r.batch_size for r in rs])),
+        lo, hi = first + 0.25 * span, first + 0.75 * span
+        done_mid = sum(1 for r in rs if lo <= r.t_done < hi)
+        res[prec] = dict(requests_s=len(rs) / (last - first), steady_requests_s=done_mid / (hi - lo),
+                         p50_ms=1e3 * pct(lat, 0.5), p99_ms=1e3 * pct(lat, 0.99),
+                         mean_batch=float(np.mean([r.batch_size for r in rs])),
                          converged=sum(r.perf.converged for r in rs),
-                         mean_queue_ms=1e3 * float(np.mean([r.t_start - r.t_submit for r in rs])))
-        progress("serving %s: %.1f req/s, p50 %.0f ms, p99 %.0f ms" % (prec, res[prec]["requests_s"],
-                                                                       res[prec]["p50_ms"], res[prec]["p99_ms"]))
+                         mean_ms={"prepare (host assembly + layout)": mean_ms("submit", "prepared"),
+                                  "wait for a slot": mean_ms("prepared", "admitted"),
+                                  "batch upload (host packing + copies)": mean_ms("admitted", "solve"),
+                                  "solve until retired": mean_ms("solve", "done")},
+                         cells_s=sum(cells) / (last - first))
+        progress("serving %s: %.1f req/s (steady %.1f), p50 %.0f ms, p99 %.0f ms"
+                 % (prec, res[prec]["requests_s"], res[prec]["steady_requests_s"], res[prec]["p50_ms"],
+                    res[prec]["p99_ms"]))
         del srv
     out = {
         "metric": "serving: solves/s and p50/p99 latency at 50 req/s Poisson arrivals; mixed vs FP64",
         "value": res["fp64"]["requests_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.requests,
-        "warmup": 8, "ms_per_step": 1e3 / res["fp64"]["requests_s"], "higher_is_better": True,
+        "warmup": args.workers, "ms_per_step": 1e3 / res["fp64"]["requests_s"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded case generator)",
-        "config": {"workload": "serving: %d requests, Poisson 50 req/s, 20k-2M cells, 50%% GGI" % args.requests,
-                   "total_cells": int(sum(cells)), "arrival_span_s": texts[-1][0],
-                   "workers": 8, "max_batch": 8, "hbm_budget_GiB": 200},
-        "fp64": res["fp64"], "mixed": res["mixed"],
+        "config": {"workload": "serving: %d requests, Poisson 50 req/s, 20k-2M cells, 50%% GGI, tol 1e-10"
+                               % args.requests,
+                   "total_cells": int(sum(cells)), "arrival_span_s": span, "device_slots": args.workers,
+                   "max_batch": args.max_batch, "prep_threads": args.prep or "auto", "hbm_budget_GiB": 200},
+        "fp64": res["fp64"], "mixed": res.get("mixed"),
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
 
@@ -584,7 +604,10 @@ def main():
                          "suite: the reference test suite's nine parameter cases as one batch, fields "
                          "set and assembled on the device, then solved; suite_partitioned: the test suite's "
                          "partitioned_Aitken cases, Dirichlet-Neumann coupling of the nine as one batch")
-    ap.add_argument("--requests", type=int, default=100, help="serving: number of requests")
+    ap.add_argument("--requests", type=int, default=400, help="serving: number of requests")
+    ap.add_argument("--workers", type=int, default=4, help="serving: device slots (resident batches, streams)")
+    ap.add_argument("--max-batch", type=int, default=4, help="serving: requests per slot batch")
+    ap.add_argument("--prep", type=int, default=0, help="serving: host preparation threads (0: automatic)")
     ap.add_argument("--copies", type=int, default=9,
                     help="suite, suite_partitioned: cases in the batch, cycling through the nine parameter sets "
                          "(default 9)")
@@ -607,6 +630,12 @@ def main():
     if args.workload == "batch64":
         return run_batch(m, cases, args)
     if args.workload == "serving":
+        # one run of this workload (8 slots, 4 per batch, 400 requests)
+        # ended in a host segmentation fault whose cause is not identified;
+        # the library's host code is clean under AddressSanitizer and
+        # ThreadSanitizer on the CPU with the same request stream, so the
+        # workload stays disabled until the cause is found
+        sys.exit("the serving workload is disabled: a run ended in a segmentation fault of unknown cause")
         return run_serving(m, cases, args)
     if args.workload == "tutorial":
         return run_tutorial(m, args)
@@ -639,25 +668,35 @@ def main():
     # iterations), from the same initial field as above
     C = m.Case(case)
     regs, ifs = C.regions(), C.interfaces()
-    A.reset_psi()
-    A.solve(fp64)     # graphs and tuning are already in place; warm once
-    dlat = []
-    for _ in range(max(3, args.steps // 2)):
-        d0 = time.perf_counter()
-        A.reset_psi()
+    def refresh():
         for r, R in enumerate(regs):
             A.update_region(0, r, R["diag"], R["upper"], R["source"], lower=None if R["symmetric"] else R["lower"])
         for i, I in enumerate(ifs):
             A.update_interface(0, i, I["coeffA"], I["coeffB"])
+
+    # the first refresh builds the value maps and the pinned staging (once
+    # per committed structure): warm once
+    A.reset_psi()
+    refresh()
+    A.solve(fp64)
+    dlat, dup = [], []
+    for _ in range(max(3, args.steps // 2)):
+        d0 = time.perf_counter()
+        A.reset_psi()
+        refresh()
+        d1 = time.perf_counter()
         dperf = A.solve(fp64)[0]
         dlat.append(time.perf_counter() - d0)
+        dup.append(d1 - d0)
     upload = sum(8 * (2 * R["nCells"] + (1 if R["symmetric"] else 2) * R["nFaces"]) for R in regs) + \
         sum(8 * (I["nA"] + I["nB"]) for I in ifs)
     drop_in = dict(ms_per_step=1e3 * sum(dlat) / len(dlat), solves_s=len(dlat) / sum(dlat),
+                   upload_ms=1e3 * sum(dup) / len(dup),
                    iterations=dperf.n_iterations, same_as_resident=dperf.final_residual == perf.final_residual,
                    bytes_uploaded=upload, steps=len(dlat),
                    note="reset field, upload every region's diag/lower/upper/source and the interface "
-                        "coefficients (pageable host memory), place on the device, factor, solve")
+                        "coefficients (the caller's pageable arrays, staged through pinned memory), place on the device, "
+                        "factor, solve; upload_ms covers the hand-over up to the placement")
     psi_drop = np.concatenate(A.psi(0))
     drop_in["field_equals_resident"] = bool(np.array_equal(psi_drop, psi64))
 

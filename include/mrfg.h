@@ -478,6 +478,11 @@ int mrfg_server_wait_all(mrfg_server *srv);
    (mrfg_server_last_error). */
 int mrfg_server_result(mrfg_server *srv, int request_id, mrfg_perf *perf, double *times, int *batch_size);
 int mrfg_server_get_psi(mrfg_server *srv, int request_id, int region, double *psi);
+/* Where a finished request's time went: times[5] = submitted, prepared
+   (assembled and laid out on the host), admitted to a device slot, solve
+   started (its batch's layout uploaded), done -- seconds since the server
+   started. */
+int mrfg_server_times(mrfg_server *srv, int request_id, double *times);
 const char *mrfg_server_last_error(const mrfg_server *srv);
 
 #ifdef __cplusplus

@@ -143,6 +143,7 @@ def lib():
         "mrfg_server_wait_all": ([vp], C.c_int),
         "mrfg_server_result": ([vp, C.c_int, C.c_void_p, F64P, C.POINTER(C.c_int)], C.c_int),
         "mrfg_server_get_psi": ([vp, C.c_int, C.c_int, F64P], C.c_int),
+        "mrfg_server_times": ([vp, C.c_int, F64P], C.c_int),
         "mrfg_server_last_error": ([vp], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -166,7 +167,7 @@ EXPORTED = (
     "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_tuning mrfg_set_tuning mrfg_bytes mrfg_copy_probe mrfg_interface_flux mrfg_interface_flux_noalloc "
     "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
-    "mrfg_server_get_psi mrfg_server_last_error").split()
+    "mrfg_server_get_psi mrfg_server_times mrfg_server_last_error").split()
 
 
 def _p(a, t):
@@ -780,6 +781,15 @@ class Server:
             raise MrfgError(rc, lib().mrfg_server_last_error(self._h).decode())
         sp = SolverPerformance.from_c(perf)
         return ServeResult(sp, float(t[0]), float(t[1]), float(t[2]), bs.value)
+
+    def times(self, rid):
+        """dict(submit, prepared, admitted, solve, done) of a finished request
+        (seconds since the server started)."""
+        t = np.zeros(5)
+        rc = lib().mrfg_server_times(self._h, rid, _p(t, C.c_double))
+        if rc != MRFG_OK:
+            raise MrfgError(rc, "request %d has not finished" % rid)
+        return dict(zip(("submit", "prepared", "admitted", "solve", "done"), t.tolist()))
 
     def psi(self, rid):
         sizes = self._sizes[rid]

@@ -384,7 +384,7 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
     // (slice, kk, lane); its producer's outs list those inbox addresses.
     {
         const int nSl = slice;
-        const int G = packChunks(K_ + 1, 2), G32 = packChunks(K_ + 1, 4);
+        const int G = packChunks(K_ + 1, 2);
         const int NC = packChunks(K_, 4), H = packChunks(codeInts(K_), 4);
         int rawF = 0, rawB = 0;
         for (size_t e = 0; e < fCode.size(); ++e) {
@@ -398,8 +398,8 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
         auto kkOf = [](int m, int k) { return __builtin_popcount(m & ((1 << k) - 1)); };
         hFPack_.assign((size_t)nSl * G * 64 * 2, 0.0);
         hBPack_.assign((size_t)nSl * G * 64 * 2, 0.0);
-        hFPack32_.assign((size_t)nSl * G32 * 64 * 4, 0.0f);
-        hBPack32_.assign((size_t)nSl * G32 * 64 * 4, 0.0f);
+        hFPack32_.clear();   // derived from the FP64 records when mixed precision first needs them
+        hBPack32_.clear();
         std::vector<double> prod((size_t)nSl * G * 64 * 2, 0.0);
         std::vector<int32_t> fc((size_t)nSl * H * 64 * 4, -1), bc((size_t)nSl * H * 64 * 4, -1);
         // consumers of every published halo value (slot id -> inbox addresses)
@@ -423,11 +423,8 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
                     fw[k / 4] |= fb << (8 * (k % 4));
                     bw[k / 4] |= bb << (8 * (k % 4));
                     size_t e64 = (((size_t)s * G + k / 2) * 64 + l) * 2 + k % 2;
-                    size_t e32 = (((size_t)s * G32 + k / 4) * 64 + l) * 4 + k % 4;
                     hFPack_[e64] = hFVal_[e];
                     hBPack_[e64] = hBVal_[e];
-                    hFPack32_[e32] = (float)hFVal_[e];
-                    hBPack32_[e32] = (float)hBVal_[e];
                     prod[e64] = fProd[e];
                 }
                 // padding slots of the last code int are "none"
@@ -591,6 +588,7 @@ void Batch::ensureFloat()
 {
     if (haveFloat_) return;
     if (hostStale_) refreshHostCopies();
+    else derivePack32();
     dropGraphs();   // FP32 buffers may be allocated anew
     auto toF = [](const std::vector<double> &a) {
         std::vector<float> b(a.size());
@@ -887,13 +885,25 @@ void Batch::runFp64(const SolverControls &c)
     retireAll();
 }
 
+void Batch::copyToHost(void *dst, const void *src, size_t bytes) const
+{
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
+void Batch::copyToDevice(void *dst, const void *src, size_t bytes) const
+{
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+}
+
 void Batch::copyPsiNow(int sys, int region, double *psi)
 {
     const SystemLayout &L = *lay_.at(sys);
     const CoupledSystem &S = *sys_.at(sys);
     const int n = S.regions.at(region).nCells, o = L.regionOffset[region];
     std::vector<double> x(L.nPos);
-    HIP_OK(hipMemcpy(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double));
     for (int c = 0; c < n; ++c) psi[c] = x[L.posOfCell[o + c]];
 }
 
@@ -1148,9 +1158,9 @@ void Batch::setPsi(int sys, int region, const double *psi)
     int n = S.regions.at(region).nCells, o = L.regionOffset[region];
     std::vector<double> x(L.nPos);
     HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double));
     for (int c = 0; c < n; ++c) x[L.posOfCell[o + c]] = psi[c];
-    HIP_OK(hipMemcpy(x64_.p + posBegin(sys), x.data(), L.nPos * sizeof(double), hipMemcpyHostToDevice));
+    copyToDevice(x64_.p + posBegin(sys), x.data(), L.nPos * sizeof(double));
 }
 
 void Batch::getPsi(int sys, int region, double *psi)
@@ -1160,7 +1170,7 @@ void Batch::getPsi(int sys, int region, double *psi)
     int n = S.regions.at(region).nCells, o = L.regionOffset[region];
     std::vector<double> x(L.nPos);
     HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(x.data(), x64_.p + posBegin(sys), L.nPos * sizeof(double));
     for (int c = 0; c < n; ++c) psi[c] = x[L.posOfCell[o + c]];
 }
 
@@ -1175,7 +1185,7 @@ void Batch::spmv(const double *xHost, double *yHost, int sys)
     dy.alloc(nPos_);
     HIP_OK(launchSpmvPlain<double>(view(m64_), blocks_.p, nBlk_, 256, dx.p, dy.p, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
-    HIP_OK(hipMemcpy(y.data(), dy.p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(y.data(), dy.p, nPos_ * sizeof(double));
     for (long long c = 0; c < L.nCells; ++c) yHost[c] = y[posBegin(sys) + L.posOfCell[c]];
 }
 
@@ -1188,7 +1198,7 @@ void Batch::diluFactor(int sys, double *rDHost)
     std::vector<double> r(nPos_);
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
-    HIP_OK(hipMemcpy(r.data(), v64_[11].p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(r.data(), v64_[11].p, nPos_ * sizeof(double));
     for (long long c = 0; c < L.nCells; ++c) rDHost[c] = r[posBegin(sys) + L.posOfCell[c]];
 }
 
@@ -1210,7 +1220,7 @@ void Batch::precondition(const double *rHost, double *wHost, int sys)
     HIP_OK(launchPrecondition<double>(D, K, dr.p, dw.p, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
-    HIP_OK(hipMemcpy(w.data(), dw.p, nPos_ * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(w.data(), dw.p, nPos_ * sizeof(double));
     for (long long c = 0; c < L.nCells; ++c) wHost[c] = w[posBegin(sys) + L.posOfCell[c]];
 }
 
@@ -1358,7 +1368,7 @@ void Batch::precondition32(const float *rHost, float *wHost, int sys)
     HIP_OK(launchPrecondition<float>(D, K, dr.p, dw.p, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
-    HIP_OK(hipMemcpy(w.data(), dw.p, nPos_ * sizeof(float), hipMemcpyDeviceToHost));
+    copyToHost(w.data(), dw.p, nPos_ * sizeof(float));
     for (long long c = 0; c < L.nCells; ++c) wHost[c] = w[posBegin(sys) + L.posOfCell[c]];
 }
 
@@ -1449,7 +1459,7 @@ int Batch::traceDilu(long long *out, int maxWaves)
     HIP_OK(hipStreamSynchronize(stream_));
     checkDeviceError();
     std::vector<unsigned long long> h((size_t)8 * nWaves_);
-    HIP_OK(hipMemcpy(h.data(), tr.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    copyToHost(h.data(), tr.p, h.size() * sizeof(unsigned long long));
     const double nsPerTick = freq > 0 ? 1e6 / freq : 10.0;
     for (int sw = 0; sw < 2; ++sw) {
         unsigned long long t0 = ~0ull;

@@ -89,6 +89,7 @@ struct DevBuf {
         cap = 0;
     }
     void alloc(size_t count);
+    // copy h (pageable: the copy is synchronous for the host) on stream st
     void upload(const std::vector<T> &h, hipStream_t st);
     void zero(hipStream_t st);
 };
@@ -127,6 +128,13 @@ struct RefreshMaps {
     DevBuf<double> faceVal, sideCoeff, diag, source;
     std::vector<char> dirty;     // per system: staged values not yet placed
     std::vector<char> changed;   // per system: values differ from the committed ones
+    // pinned host staging of the values in the device arrays' layout
+    // (faceVal, diag, source, sideCoeff), so each upload is one DMA
+    double *stage = nullptr;
+    RefreshMaps() = default;
+    RefreshMaps(const RefreshMaps &) = delete;
+    RefreshMaps &operator=(const RefreshMaps &) = delete;
+    ~RefreshMaps();
 };
 
 class Batch {
@@ -268,6 +276,11 @@ public:
     void assemblyBytes(double &assemble, double &place) const;
 
 private:
+    // copies ordered on the library stream, waited for (never on the legacy
+    // stream, which must not wait on another slot's stream while that one
+    // captures a graph)
+    void copyToHost(void *dst, const void *src, size_t bytes) const;
+    void copyToDevice(void *dst, const void *src, size_t bytes) const;
     void ensureFloat();
     void factorise();
     void initState(const SolverControls &c, const std::vector<char> *active = nullptr);
@@ -313,6 +326,8 @@ private:
     FvPlace fvPlaceView();
     // the host copies of the FP64 values after the device changed them
     void refreshHostCopies();
+    // the FP32 DILU records (host) from the FP64 ones
+    void derivePack32();
 
     int device_;
     hipStream_t stream_ = nullptr;

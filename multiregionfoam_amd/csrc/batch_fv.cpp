@@ -488,10 +488,10 @@ void Batch::getRegionMatrix(int sys, int region, double *diag, double *lower, do
     const size_t c0 = (size_t)fv_->cellBase[sys] + L.regionOffset[region];
     const size_t f0 = (size_t)fv_->faceBase[sys] + L.faceOffset[region];
     std::vector<double> fvals(2 * (size_t)R.nFaces);
-    HIP_OK(hipMemcpy(diag, fv_->diag.p + c0, R.nCells * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(source, fv_->source.p + c0, R.nCells * sizeof(double), hipMemcpyDeviceToHost));
+    copyToHost(diag, fv_->diag.p + c0, R.nCells * sizeof(double));
+    copyToHost(source, fv_->source.p + c0, R.nCells * sizeof(double));
     if (R.nFaces > 0)
-        HIP_OK(hipMemcpy(fvals.data(), fv_->faceVal.p + 2 * f0, fvals.size() * sizeof(double), hipMemcpyDeviceToHost));
+        copyToHost(fvals.data(), fv_->faceVal.p + 2 * f0, fvals.size() * sizeof(double));
     for (int f = 0; f < R.nFaces; ++f) {
         lower[f] = fvals[2 * (size_t)f];
         upper[f] = fvals[2 * (size_t)f + 1];
@@ -509,21 +509,29 @@ void Batch::getInterfaceCoeff(int sys, int iface, int side, double *coeff)
     for (int i = 0; i < iface; ++i) off += S.interfaces[i].a.nFaces() + S.interfaces[i].b.nFaces();
     if (side) off += S.interfaces[iface].a.nFaces();
     const int n = side ? S.interfaces[iface].b.nFaces() : S.interfaces[iface].a.nFaces();
-    if (n > 0) HIP_OK(hipMemcpy(coeff, fv_->ifCoeff.p + off, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (n > 0) copyToHost(coeff, fv_->ifCoeff.p + off, n * sizeof(double));
 }
 
 void Batch::refreshHostCopies()
 {
     HIP_OK(hipStreamSynchronize(stream_));
-    auto down = [](std::vector<double> &h, const void *d) {
-        if (!h.empty()) HIP_OK(hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    auto down = [this](std::vector<double> &h, const void *d) {
+        if (!h.empty()) copyToHost(h.data(), d, h.size() * sizeof(double));
     };
     down(hEllVal_, m64_.ellVal.p);
     down(hDiag_, m64_.diag.p);
     down(hIfCoeff_, m64_.ifCoeff.p);
     down(hFPack_, m64_.fPack.p);
     down(hBPack_, m64_.bPack.p);
+    derivePack32();
+    hostStale_ = false;
+}
+
+void Batch::derivePack32()
+{
     const int G = packChunks(K_ + 1, 2), G32 = packChunks(K_ + 1, 4);
+    hFPack32_.assign((size_t)nSlices_ * G32 * 64 * 4, 0.0f);
+    hBPack32_.assign((size_t)nSlices_ * G32 * 64 * 4, 0.0f);
     for (int s = 0; s < nSlices_; ++s)
         for (int k = 0; k < K_; ++k)
             for (int l = 0; l < kSlice; ++l) {
@@ -532,7 +540,6 @@ void Batch::refreshHostCopies()
                 hFPack32_[e32] = (float)hFPack_[e64];
                 hBPack32_[e32] = (float)hBPack_[e64];
             }
-    hostStale_ = false;
 }
 
 }  // namespace mrfg

@@ -1002,6 +1002,20 @@ int mrfg_server_result(mrfg_server *s, int id, mrfg_perf *perf, double *times, i
     return MRFG_OK;
 }
 
+int mrfg_server_times(mrfg_server *s, int id, double *times)
+{
+    if (!s || !times) return MRFG_ERR_ARG;
+    mrfg::ServeResult R;
+    if (!s->srv->result(id, R)) return MRFG_ERR_ARG;
+    if (R.status != 2) return MRFG_ERR_STATE;
+    times[0] = R.tSubmit;
+    times[1] = R.tReady;
+    times[2] = R.tAdmit;
+    times[3] = R.tSolve;
+    times[4] = R.tDone;
+    return MRFG_OK;
+}
+
 int mrfg_server_get_psi(mrfg_server *s, int id, int region, double *psi)
 {
     if (!s || !psi) return MRFG_ERR_ARG;

@@ -10,6 +10,7 @@
 // solver's arrays through the maps the layout builder recorded (where every
 // ELL entry, DILU record slot and interface-row face came from).
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 
 #include "batch.hpp"
@@ -23,6 +24,14 @@ namespace mrfg {
         if (e_ != hipSuccess)                                                                              \
             throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
     } while (0)
+
+RefreshMaps::~RefreshMaps()
+{
+    if (stage) {
+        std::lock_guard<std::recursive_mutex> lk(deviceMutex());
+        (void)hipHostFree(stage);
+    }
+}
 
 void Batch::buildRefresh()
 {
@@ -107,6 +116,11 @@ void Batch::buildRefresh()
     F->sideCoeff.upload(sideCoeff.empty() ? std::vector<double>{0.0} : sideCoeff, st);
     F->diag.upload(diag, st);
     F->source.upload(source, st);
+    {
+        std::lock_guard<std::recursive_mutex> lk(deviceMutex());
+        const size_t n = 2 * (size_t)faceBase + 2 * (size_t)cellBase + (size_t)sideBase + 1;
+        HIP_OK(hipHostMalloc((void **)&F->stage, n * sizeof(double), hipHostMallocDefault));
+    }
     HIP_OK(hipStreamSynchronize(st));
     rf_ = std::move(F);
 }
@@ -143,16 +157,22 @@ void Batch::updateRegion(int sys, int region, const double *diag, const double *
     const SystemLayout &L = *lay_[sys];
     const size_t c0 = (size_t)rf_->cellBase[sys] + L.regionOffset[region];
     const size_t f0 = (size_t)rf_->faceBase[sys] + L.faceOffset[region];
-    std::vector<double> fv(2 * (size_t)R.nFaces);
+    // staged in pinned memory in the device arrays' layout, then one DMA each
+    double *sf = rf_->stage + 2 * f0;
+    double *sd = rf_->stage + 2 * (size_t)rf_->nFaces + c0;
+    double *ss = rf_->stage + 2 * (size_t)rf_->nFaces + (size_t)rf_->nCells + c0;
+    const double *lo = lower ? lower : upper;
     for (int f = 0; f < R.nFaces; ++f) {
-        fv[2 * (size_t)f] = lower ? lower[f] : upper[f];
-        fv[2 * (size_t)f + 1] = upper[f];
+        sf[2 * (size_t)f] = lo[f];
+        sf[2 * (size_t)f + 1] = upper[f];
     }
-    HIP_OK(hipMemcpyAsync(rf_->diag.p + c0, diag, R.nCells * sizeof(double), hipMemcpyHostToDevice, stream_));
-    HIP_OK(hipMemcpyAsync(rf_->source.p + c0, source, R.nCells * sizeof(double), hipMemcpyHostToDevice, stream_));
+    std::memcpy(sd, diag, R.nCells * sizeof(double));
+    std::memcpy(ss, source, R.nCells * sizeof(double));
+    HIP_OK(hipMemcpyAsync(rf_->diag.p + c0, sd, R.nCells * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIP_OK(hipMemcpyAsync(rf_->source.p + c0, ss, R.nCells * sizeof(double), hipMemcpyHostToDevice, stream_));
     if (R.nFaces > 0)
-        HIP_OK(hipMemcpyAsync(rf_->faceVal.p + 2 * f0, fv.data(), fv.size() * sizeof(double), hipMemcpyHostToDevice,
-                              stream_));
+        HIP_OK(hipMemcpyAsync(rf_->faceVal.p + 2 * f0, sf, 2 * (size_t)R.nFaces * sizeof(double),
+                              hipMemcpyHostToDevice, stream_));
     HIP_OK(hipStreamSynchronize(stream_));
     rf_->dirty[sys] = 1;
 }
@@ -168,12 +188,17 @@ void Batch::updateInterface(int sys, int iface, const double *coeffA, const doub
     size_t off = rf_->sideBase[sys];
     for (int i = 0; i < iface; ++i) off += S.interfaces[i].a.nFaces() + S.interfaces[i].b.nFaces();
     const CoupledInterface &I = S.interfaces[iface];
-    if (coeffA && I.a.nFaces() > 0)
-        HIP_OK(hipMemcpyAsync(rf_->sideCoeff.p + off, coeffA, I.a.nFaces() * sizeof(double), hipMemcpyHostToDevice,
+    double *st = rf_->stage + 2 * (size_t)rf_->nFaces + 2 * (size_t)rf_->nCells + off;
+    if (coeffA && I.a.nFaces() > 0) {
+        std::memcpy(st, coeffA, I.a.nFaces() * sizeof(double));
+        HIP_OK(hipMemcpyAsync(rf_->sideCoeff.p + off, st, I.a.nFaces() * sizeof(double), hipMemcpyHostToDevice,
                               stream_));
-    if (coeffB && I.b.nFaces() > 0)
-        HIP_OK(hipMemcpyAsync(rf_->sideCoeff.p + off + I.a.nFaces(), coeffB, I.b.nFaces() * sizeof(double),
-                              hipMemcpyHostToDevice, stream_));
+    }
+    if (coeffB && I.b.nFaces() > 0) {
+        std::memcpy(st + I.a.nFaces(), coeffB, I.b.nFaces() * sizeof(double));
+        HIP_OK(hipMemcpyAsync(rf_->sideCoeff.p + off + I.a.nFaces(), st + I.a.nFaces(),
+                              I.b.nFaces() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    }
     HIP_OK(hipStreamSynchronize(stream_));
     rf_->dirty[sys] = 1;
 }

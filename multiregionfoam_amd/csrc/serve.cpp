@@ -95,6 +95,8 @@ bool Server::result(int id, ServeResult &out) const
     out.tSubmit = R.tSubmit;
     out.tAdmit = R.tAdmit;
     out.tDone = R.tDone;
+    out.tReady = R.tReady;
+    out.tSolve = R.tSolve;
     out.batchSize = R.batchSize;
     out.perf = R.perf;
     out.error = R.error;
@@ -158,6 +160,7 @@ void Server::prepare()
             R.sys = std::move(sys);
             R.layout = std::move(lay);
             R.bytes = bytes;
+            R.tReady = now();
             ready_.insert(id);
             cvReady_.notify_all();
         } catch (const std::exception &e) {
@@ -241,6 +244,11 @@ void Server::worker()
         std::string err;
         try {
             batch->setSystems(ptrs, &layouts);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                const double t = now();
+                for (int id : ids) reqs_[id]->tSolve = t;
+            }
             std::vector<SolvePerf> perf;
             const Batch::RetireFn fn = retire;
             batch->solve(ctrl_, perf, nullptr, &fn);

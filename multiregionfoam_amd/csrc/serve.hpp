@@ -34,6 +34,7 @@ namespace mrfg {
 struct ServeRequest {
     std::string text;
     double tSubmit = 0, tAdmit = 0, tDone = 0;   // seconds since the server started
+    double tReady = 0, tSolve = 0;               // prepared; device layout uploaded, solve starts
     int status = 0;                              // 0 waiting, 1 running, 2 done, -1 failed
     int batchSize = 0;
     size_t bytes = 0;
@@ -48,7 +49,7 @@ struct ServeRequest {
 // A copy of a request's public state, taken under the server's lock.
 struct ServeResult {
     int status = 0;
-    double tSubmit = 0, tAdmit = 0, tDone = 0;
+    double tSubmit = 0, tAdmit = 0, tDone = 0, tReady = 0, tSolve = 0;
     int batchSize = 0;
     SolvePerf perf;
     std::string error;

@@ -196,6 +196,8 @@ def test_serving_results_equal_alone(gpu):
         res = srv.result(rid)
         batched += res.batch_size > 1
         assert res.perf.converged and res.t_done >= res.t_start >= res.t_submit
+        t = srv.times(rid)
+        assert t["submit"] <= t["prepared"] <= t["admitted"] <= t["solve"] <= t["done"]
         A = _ctx(gpu, c)
         p = A.solve(tolerance=1e-10, maxIter=1000)[0]
         assert res.perf.n_iterations == p.n_iterations
