@@ -310,6 +310,50 @@ constexpr int kBoxRounds = 4;    // prefetch-ring rounds per unrolled group of s
 // rD in a chunk of its own).
 // LONE: the tile has no neighbouring tile (alone in its region), so it has
 // no inbox to read and no values to deliver.
+// Raw buffer access (gfx9 buffer resource: base, byte size, 32-bit
+// offsets).  Every address of the sweep below is a wave-uniform resource
+// plus a scalar step offset plus a per-lane constant, so a step spends no
+// instructions on 64-bit address arithmetic.  A lane offset of kNoLane lies
+// beyond every resource: the hardware drops such a store, which replaces the
+// per-step exec-mask juggling of conditional stores.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kRawBufferWord3 = 0x00020000;   // 32-bit data format, no swizzle, no index stride
+constexpr int kSysCoherent = 16;              // sc1: the policy of the agent-scope relaxed atomics
+constexpr unsigned kNoLane = 0x80000000u;
+
+// (the arguments are wave-uniform; readfirstlane states it, so the resource
+// lives in scalar registers)
+__device__ inline __amdgpu_buffer_rsrc_t rawBuffer(const void *base, size_t bytes)
+{
+    const unsigned long long a = (unsigned long long)base;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const int n = __builtin_amdgcn_readfirstlane((int)bytes);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), (short)0, n,
+                                             kRawBufferWord3);
+}
+
+template <typename X, int aux>
+__device__ inline X bufLoad(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(X) == 16) return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, aux));
+    else if constexpr (sizeof(X) == 8) return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, aux));
+    else return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, aux));
+}
+
+template <typename X, int aux>
+__device__ inline void bufStore(X v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(X) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, aux);
+    else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, aux);
+}
+
+// KT: the batch's record width K when it is 3 (records of two FP64 / one
+// FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
+// rD in a chunk of its own).
+// LONE: the tile has no neighbouring tile (alone in its region), so it has
+// no inbox to read and no values to deliver.
 template <typename T, int KT, int MODE, int AX, bool LONE>
 __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m, const T *__restrict__ in, T *out,
                                          typename Bits<T>::U *inbox, T *xch, unsigned long long &nSlow)
@@ -330,8 +374,6 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     const int gLast = K / N, eLast = K % N;      // chunk / element of rD (diag)
     const R *recs = MODE == kDiluFactor ? (const R *)V.prodPack : (const R *)(MODE == kDiluBwd ? V.bPack : V.fPack);
 
-    // per-lane constants; every address below is a wave-uniform base plus
-    // one of these 32-bit lane offsets
     const unsigned lane = threadIdx.x;
     const int TB = m[kBmTB], lg = __builtin_ctz(TB), TC = 64 >> lg, TBC = TB + TC;
     const int nA = m[kBmNA], actB = m[kBmActB], actC = m[kBmActC];
@@ -348,36 +390,53 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     const bool useB = edgeB && hasB, useC = edgeC && hasC;
     const unsigned srcB = (kFwd ? lane - 1 : lane + 1) & 63;
     const unsigned srcC = (kFwd ? lane - TB : lane + TB) & 63;
-    const unsigned inB = base + cl, inC = base + TC + bl;   // at step t: inbox + t * TBC
     // where this lane's values go: the consumer's step is t -/+ (TB - 1)
     // (resp. TC - 1), into the consumer's inbox at [cl] (resp. [TC + bl])
     const int toB = kFwd ? nextB : prevB, toC = kFwd ? nextC : prevC;
     const bool sendB = (kFwd ? bl == TB - 1 : bl == 0) && toB >= 0;
     const bool sendC = (kFwd ? cl == TC - 1 : cl == 0) && toC >= 0;
     const int dB = kFwd ? 1 - TB : TB - 1, dC = kFwd ? 1 - TC : TC - 1;
-    const unsigned outB = (toB >= 0 ? V.boxMeta[(size_t)toB * kBoxMeta + kBmInbox] : 0) + cl;
-    const unsigned outC = (toC >= 0 ? V.boxMeta[(size_t)toC * kBoxMeta + kBmInbox] : 0) + TC + bl;
+
+    // the wave's resources: its slices of the records, of in / out (factor:
+    // of rD and of the two record arrays), its own inbox and its consumers'
+    const int rowBytes = G * 64 * (int)sizeof(R);
+    const auto rRec = rawBuffer(recs + (size_t)s0 * G * 64, (size_t)nSteps * rowBytes);
+    const size_t rowsBytes = (size_t)nSteps * 64 * sizeof(T);
+    const auto rIn = MODE == kDiluFactor ? rawBuffer(nullptr, 0) : rawBuffer(in + (size_t)s0 * 64, rowsBytes);
+    const auto rOut = MODE == kDiluFactor ? rawBuffer(nullptr, 0) : rawBuffer(out + (size_t)s0 * 64, rowsBytes);
+    const U *myIn = inbox + base;
+    const auto rInbox = rawBuffer(myIn, LONE ? 0 : (size_t)nSteps * TBC * sizeof(U));
+    auto consumer = [&](int to) {
+        if (LONE || to < 0) return rawBuffer(inbox, 0);
+        const int32_t *mt = V.boxMeta + (size_t)to * kBoxMeta;
+        return rawBuffer(inbox + mt[kBmInbox], (size_t)mt[kBmSteps] * TBC * sizeof(U));
+    };
+    const auto rToB = consumer(toB), rToC = consumer(toC);
+    const unsigned vRec = lane * (unsigned)sizeof(R), vRow = lane * (unsigned)sizeof(T);
+    const unsigned vInB = (unsigned)cl * sizeof(U), vInC = (unsigned)(TC + bl) * sizeof(U);
+    const unsigned vToB = sendB ? (unsigned)cl * sizeof(U) : kNoLane;
+    const unsigned vToC = sendC ? (unsigned)(TC + bl) * sizeof(U) : kNoLane;
 
     struct Step {
         R v[NL];
         T a;
     };
+    // step i of the sweep (clamped: loads past the end re-read the last step)
     auto stepOf = [&](int i) {
         i = i < nSteps ? i : nSteps - 1;
         return kFwd ? i : nSteps - 1 - i;
     };
     auto load = [&](int i, Step &S) {
-        const int s = s0 + stepOf(i);
-        const R *row = recs + (size_t)s * G * 64;
+        const unsigned so = (unsigned)(stepOf(i) * rowBytes);
 #pragma unroll
-        for (int g = 0; g < G2; ++g) S.v[g] = row[g * 64 + lane];
-        if (kExtra) S.v[NL - 1] = row[gLast * 64 + lane];
-        if (MODE != kDiluFactor) S.a = (in + (size_t)s * 64)[lane];
+        for (int g = 0; g < G2; ++g) S.v[g] = bufLoad<R, 0>(rRec, vRec + g * 64 * sizeof(R), so);
+        if (kExtra) S.v[NL - 1] = bufLoad<R, 0>(rRec, vRec + gLast * 64 * sizeof(R), so);
+        if (MODE != kDiluFactor) S.a = bufLoad<T, 0>(rIn, vRow, (unsigned)(stepOf(i) * 64 * sizeof(T)));
     };
     auto loadIn = [&](int i, U &hb, U &hc) {
-        const U *ib = inbox + (size_t)stepOf(i) * TBC;
-        hb = haloLoad(ib, inB);
-        hc = haloLoad(ib, inC);
+        const unsigned so = (unsigned)(stepOf(i) * TBC * sizeof(U));
+        hb = bufLoad<U, kSysCoherent>(rInbox, vInB, so);
+        hc = bufLoad<U, kSysCoherent>(rInbox, vInC, so);
     };
 
     // the exchange values start finite: a missing neighbour's zero
@@ -402,8 +461,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     T yprev = T(0);
     // the loop runs whole groups of steps: a step past the end is invalid for
     // every lane, reads clamped addresses and stores nothing, so the only
-    // conditional code is the wave-uniform slow path and the stores.  The
-    // body is unrolled over kBoxRounds rounds of the prefetch ring, every
+    // conditional code is the wave-uniform slow path and the output store.
+    // The body is unrolled over kBoxRounds rounds of the prefetch ring, every
     // record wait exact (the initial loads above are issued in order for
     // that); 16 steps measured fastest (8 and 32 were slower)
     for (int i0 = 0; i0 < nSteps; i0 += kBoxRounds * kBoxAhead) {
@@ -421,20 +480,20 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             const bool hasA = kFwd ? ia >= 1 : ia + 1 < nA;
             const bool needB = valid & useB, needC = valid & useC;
             const bool miss = !LONE && ((needB & (hb == kSent)) | (needC & (hc == kSent)));
-            if (!LONE && __builtin_expect(__any(miss), 0)) {
+            if (!LONE && __builtin_expect(__builtin_amdgcn_ballot_w64(miss) != 0, 0)) {
                 // rare slow path (wave-uniform): wait for the late values and
                 // reload the inbox values already requested for later steps
                 ++nSlow;
-                const U *ib = inbox + (size_t)t * TBC;
-                if (needB && hb == kSent) hb = haloWait(ib, (int)inB, kSent, V.err);
-                if (needC && hc == kSent) hc = haloWait(ib, (int)inC, kSent, V.err);
+                const U *ib = myIn + (size_t)t * TBC;
+                if (needB && hb == kSent) hb = haloWait(ib, cl, kSent, V.err);
+                if (needC && hc == kSent) hc = haloWait(ib, TC + bl, kSent, V.err);
 #pragma unroll
                 for (int d = 1; d < kBoxInAhead; ++d) {
-                    const U *fb = inbox + (size_t)stepOf(i + d) * TBC;
+                    const U *fb = myIn + (size_t)stepOf(i + d) * TBC;
                     U &vb = hbuf[(u + d) % kBoxAhead];
                     U &vc = cbuf[(u + d) % kBoxAhead];
-                    if (vb == kSent) vb = haloLoad(fb, inB);
-                    if (vc == kSent) vc = haloLoad(fb, inC);
+                    if (vb == kSent) vb = haloLoad(fb, cl);
+                    if (vc == kSent) vc = haloLoad(fb, TC + bl);
                 }
                 __builtin_amdgcn_s_waitcnt(0);
             }
@@ -477,24 +536,32 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             xcN = xch[srcC];
             yprev = yz;
             if (inRange) {
-                const int s = s0 + t;
+                const unsigned so = (unsigned)t * 64 * sizeof(T);
                 if (MODE == kDiluFactor) {
                     const double rd = valid ? 1.0 / (double)y : 0.0;
-                    (V.rDout + (size_t)s * 64)[lane] = rd;
+                    (V.rDout + (size_t)(s0 + t) * 64)[lane] = rd;
                     constexpr int NP = Pack<double>::N;
                     const int GP = packChunks(K + 1, NP);
+                    const int s = s0 + t;
                     ((double *)&V.fPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
                     ((double *)&V.bPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
                 } else {
-                    (out + (size_t)s * 64)[lane] = yz;
+                    bufStore<T, 0>(yz, rOut, vRow, so);
                 }
             }
-            if (!LONE && valid && sendB)
-                __hip_atomic_store(&(inbox + (ptrdiff_t)(t + dB) * TBC)[outB], Bits<T>::to(y), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            if (!LONE && valid && sendC)
-                __hip_atomic_store(&(inbox + (ptrdiff_t)(t + dC) * TBC)[outC], Bits<T>::to(y), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (!LONE) {
+                // the deliveries: every lane stores, those without a value for
+                // a consumer (not on the edge, or an invalid row) at kNoLane.
+                // A valid row's consumer step is valid too, so its offset lies
+                // inside the consumer's inbox; an invalid row's step offset is
+                // only kept non-negative.
+                const U yb = Bits<T>::to(y);
+                const int tb = t + dB, tc = t + dC;
+                bufStore<U, kSysCoherent>(yb, rToB, valid ? vToB : kNoLane,
+                                          (unsigned)((tb > 0 ? tb : 0) * TBC * sizeof(U)));
+                bufStore<U, kSysCoherent>(yb, rToC, valid ? vToC : kNoLane,
+                                          (unsigned)((tc > 0 ? tc : 0) * TBC * sizeof(U)));
+            }
             // the inbox values of a later step first, so that waiting for them
             // leaves the record loads issued after them in flight
             if (!LONE)
