@@ -65,6 +65,11 @@ def kernel_probe(A, reps):
           % (ms, 2 * 256 * 2**20 / (ms * 1e-3) / 1e9), file=sys.stderr, flush=True)
 
 
+# each check interval's iterations replayed as one captured HIP graph (off:
+# kernels launched one by one, e.g. under a profiler's kernel trace)
+GRAPHS = True
+
+
 def progress(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -81,7 +86,7 @@ def run_exchanger(m, cases, args):
     info = A.layout_info(0)
     res = {}
     for prec in ("fp64", "mixed"):
-        ctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
+        ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
                           innerRelTol=1e-4)
         total, lat, perf = run_steps(A, ctrl, args.warmup, args.steps, verbose=True)
         flux = [A.interface_flux(i) for i in range(2)]
@@ -441,14 +446,14 @@ def run_batch(m, cases, args):
     n = len(cs)
     cells = [A.layout_info(q)["cells"] for q in range(n)]
     if args.kernel_probe:
-        A.solve(m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000))   # autotune
+        A.solve(m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000))   # autotune
         kernel_probe(A, args.kernel_reps)
         return
     res = {}
     for prec in ("fp64", "mixed"):
         if prec == "mixed" and args.no_mixed:
             continue
-        ctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=1e-4)
+        ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=1e-4)
         total, lat, perfs = run_steps(A, ctrl, args.warmup, args.steps, all_perf=True, verbose=True)
         res[prec] = dict(systems_s=n * args.steps / total, ms_per_batch=1e3 * total / args.steps,
                          converged=sum(p.converged for p in perfs), max_iterations=max(p.n_iterations for p in perfs),
@@ -468,7 +473,7 @@ def run_batch(m, cases, args):
     dom = max(per_iter, key=per_iter.get)
     launch_bytes = spmv_bytes if dom == "spmv_alpha" else dilu_bytes
     names = {"dilu_fwd": "k_dilu_box<double, 3, 0, false>", "dilu_bwd": "k_dilu_box<double, 3, 1, false>",
-             "spmv_alpha": "k_spmv_alpha<double>"}
+             "spmv_alpha": "k_spmv_alpha<double, %d>" % A.tuning()["ell_chunk"]}
     achieved = launch_bytes / (ks[dom] * 1e-3) / 1e9
     kernels = {k: dict(ms=v, GBs=(spmv_bytes if k == "spmv_alpha" else dilu_bytes) / (v * 1e-3) / 1e9,
                        frac=(spmv_bytes if k == "spmv_alpha" else dilu_bytes) / (v * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -611,11 +616,15 @@ def main():
     ap.add_argument("--copies", type=int, default=9,
                     help="suite, suite_partitioned: cases in the batch, cycling through the nine parameter sets "
                          "(default 9)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch each iteration's kernels one by one instead of replaying HIP graphs")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="only replay the SpMV and DILU kernels on the resident workload, then a plain "
                          "device copy of known size (for rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE "
                          "per dispatch, the copy giving the counter's scale)")
     args = ap.parse_args()
+    global GRAPHS
+    GRAPHS = not args.no_graphs
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus %d is not run "
               "(the multi-GPU sharding is described in DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -655,7 +664,7 @@ def main():
     bm64 = A.bytes_model(0, 8)
     bm32 = A.bytes_model(0, 4)
 
-    fp64 = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=1000)
+    fp64 = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=1000)
     total, lat, perf = run_steps(A, fp64, args.warmup, args.steps)
     psi64 = np.concatenate(A.psi(0))
     ms_step = 1e3 * total / args.steps
@@ -702,7 +711,7 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        mctrl = m.controls(solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
+        mctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
                            innerRelTol=1e-4)
         mt, mlat, mperf = run_steps(A, mctrl, args.warmup, args.steps)
         psim = np.concatenate(A.psi(0))
@@ -729,8 +738,9 @@ def main():
     # the in-solve SpMV also reads the shadow residual for its dot product
     launch_bytes = (bm64["spmv"] + 8 * info["cells"]) if dominant == "spmv" else 0.5 * bm64["dilu"]
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-    kernel_names = {"spmv": "k_spmv_alpha<double>", "dilu_fwd": "k_dilu_box<double, 3, 0>",
-                    "dilu_bwd": "k_dilu_box<double, 3, 1>"}
+    chunk = A.tuning()["ell_chunk"]
+    kernel_names = {"spmv": "k_spmv_alpha<double, %d>" % chunk, "dilu_fwd": "k_dilu_box<double, 3, 0, false>",
+                    "dilu_bwd": "k_dilu_box<double, 3, 1, false>"}
 
     # HBM bytes per launch of that kernel, from the committed rocprofv3 PMC
     # summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain

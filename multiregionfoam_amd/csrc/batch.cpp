@@ -920,6 +920,16 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
     const int nS = (int)sys_.size();
     innerIters.assign(nS, 0);
     fp64Iters.assign(nS, 0);
+    gate_.alloc(std::max(1, nS));
+    std::vector<int8_t> gate(nS, 0);
+    auto putGate = [&]() {
+        HIP_OK(hipMemcpyAsync(gate_.p, gate.data(), nS * sizeof(int8_t), hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    };
+    auto putOuter = [&](const std::vector<SysState> &h) {
+        HIP_OK(hipMemcpyAsync(st64_.p, h.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+    };
     HIP_OK(launchSetup<double>(A64, K64, false, stream_));
     std::vector<SysState> hs, hi(nS);
     readState(hs);
@@ -928,86 +938,111 @@ void Batch::runMixed(const SolverControls &c, std::vector<int> &innerIters, std:
     std::vector<double> prevRes(nS);
     std::vector<char> fallback(nS, 0);
     for (int q = 0; q < nS; ++q) prevRes[q] = hs[q].finalRes;
-    int outer = 0;
-    while (anyRunning(hs) && outer <= c.maxOuter) {
-        HIP_OK(launchRefinePrepare(nS, st64_.p, st32_.p, c.innerRelTol, c.innerMaxIter, stream_));
-        HIP_OK(launchRefineRhs(K64, K32, stream_));
-        HIP_OK(launchSetup<float>(A32, K32, pcg, stream_));
-        int done = 0;
+    // residual history of each running inner solve (stagnation test)
+    std::vector<std::vector<double>> hist(nS);
+    // a fresh inner solve for the gated systems: FP32 state, right-hand side
+    // the FP64 residual, zero correction
+    auto startInner = [&]() {
+        putGate();
+        HIP_OK(launchRefinePrepare(nS, st64_.p, st32_.p, c.innerRelTol, c.innerMaxIter, gate_.p, stream_));
+        HIP_OK(launchRefineRhs(K64, K32, gate_.p, stream_));
+        HIP_OK(launchSetup<float>(A32, K32, pcg, stream_, gate_.p));
+        // the history starts with the inner solve's initial residual
+        HIP_OK(hipMemcpyAsync(hi.data(), st32_.p, nS * sizeof(SysState), hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
+        for (int q = 0; q < nS; ++q)
+            if (gate[q]) hist[q].assign(1, hi[q].finalRes);
+    };
+    // Each system takes its outer step as soon as its own inner solve has
+    // ended (at the check that finds it), and restarts its inner solve,
+    // while the others keep iterating: no system waits for the slowest
+    // inner solve of the batch, and the results are the same as alone.
+    for (int q = 0; q < nS; ++q) gate[q] = hs[q].status == kRunning;
+    startInner();
+    // before the first chunk the histories already hold the current residuals
+    bool firstCheck = true;
+    long long done = 0;
+    const long long budget = (long long)(c.maxOuter + 1) * (c.innerMaxIter + c.checkInterval);
+    for (;;) {
+        HIP_OK(hipMemcpyAsync(hi.data(), st32_.p, nS * sizeof(SysState), hipMemcpyDeviceToHost, stream_));
+        HIP_OK(hipStreamSynchronize(stream_));
         // an inner solve whose best residual over the last kStallWindow
         // checks is not 10 % below its best before them has stagnated (the
         // FP32 system resolves no further): it stops there, and the outer
         // step takes the correction it has
-        std::vector<std::vector<double>> hist(nS);
-        for (;;) {
-            HIP_OK(hipMemcpyAsync(hi.data(), st32_.p, nS * sizeof(SysState), hipMemcpyDeviceToHost, stream_));
-            HIP_OK(hipStreamSynchronize(stream_));
-            bool stopped = false;
-            for (int q = 0; q < nS; ++q) {
-                if (hi[q].status != kRunning) continue;
-                std::vector<double> &h = hist[q];
-                h.push_back(hi[q].finalRes);
-                if ((int)h.size() <= kStallWindow) continue;
-                const double recent = *std::min_element(h.end() - kStallWindow, h.end());
-                const double prior = *std::min_element(h.begin(), h.end() - kStallWindow);
-                if (!(recent < 0.9 * prior)) {
-                    hi[q].status = kMaxIter;
-                    stopped = true;
-                }
-            }
-            if (stopped) {
-                HIP_OK(hipMemcpyAsync(st32_.p, hi.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
-                HIP_OK(hipStreamSynchronize(stream_));
-            }
-            if (!anyRunning(hi) || done > c.innerMaxIter) break;
-            int chunk = std::min(c.checkInterval, c.innerMaxIter - done + 1);
-            runChunk<float>(pcg, chunk, c.graphs, A32, D32, K32);
-            done += chunk;
-        }
-        for (int q = 0; q < nS; ++q)
-            if (hi[q].status != kIdle) innerIters[q] += hi[q].nIter;
-        // an inner solve that broke down or left the FP32 range (singular,
-        // diverged) gives no correction: the system continues in FP64 from
-        // its current field
-        {
-            bool bad = false;
-            for (int q = 0; q < nS; ++q)
-                if (hs[q].status == kRunning && (hi[q].status == kSingular || hi[q].status == kDiverged)) {
-                    fallback[q] = 1;
-                    hs[q].status = kIdle;
-                    bad = true;
-                }
-            if (bad) {
-                HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
-                HIP_OK(hipStreamSynchronize(stream_));
-            }
-        }
-        HIP_OK(launchRefineUpdate(K64, K32, stream_));
-        HIP_OK(launchRefineResidual(A64, K64, stream_));
-        readState(hs);
-        ++outer;
-        bool changed = false;
+        bool stopped = false;
         for (int q = 0; q < nS; ++q) {
-            if (hs[q].status != kRunning) continue;
-            if (!(hs[q].finalRes < kStallFactor * prevRes[q])) {
+            if (hi[q].status != kRunning) continue;
+            std::vector<double> &h = hist[q];
+            if (firstCheck) continue;
+            h.push_back(hi[q].finalRes);
+            if ((int)h.size() <= kStallWindow) continue;
+            const double recent = *std::min_element(h.end() - kStallWindow, h.end());
+            const double prior = *std::min_element(h.begin(), h.end() - kStallWindow);
+            if (!(recent < 0.9 * prior)) {
+                hi[q].status = kMaxIter;
+                stopped = true;
+            }
+        }
+        firstCheck = false;
+        if (stopped) {
+            HIP_OK(hipMemcpyAsync(st32_.p, hi.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
+            HIP_OK(hipStreamSynchronize(stream_));
+        }
+        // the systems whose inner solve has ended; one that broke down or
+        // left the FP32 range (singular, diverged) gives no correction and
+        // continues in FP64 from its current field
+        bool due = false, outerChanged = false;
+        for (int q = 0; q < nS; ++q) {
+            gate[q] = 0;
+            if (hs[q].status != kRunning || hi[q].status == kRunning || hi[q].status == kIdle) continue;
+            innerIters[q] += hi[q].nIter;
+            if (hi[q].status == kSingular || hi[q].status == kDiverged) {
                 fallback[q] = 1;
                 hs[q].status = kIdle;
-                changed = true;
+                outerChanged = true;
+            } else {
+                gate[q] = 1;
+                due = true;
             }
-            prevRes[q] = hs[q].finalRes;
         }
-        if (changed) {
-            HIP_OK(hipMemcpyAsync(st64_.p, hs.data(), nS * sizeof(SysState), hipMemcpyHostToDevice, stream_));
-            HIP_OK(hipStreamSynchronize(stream_));
+        if (outerChanged) putOuter(hs);
+        if (due) {
+            putGate();
+            HIP_OK(launchRefineUpdate(K64, K32, gate_.p, stream_));
+            HIP_OK(launchRefineResidual(A64, K64, gate_.p, stream_));
+            readState(hs);
+            bool changed = false;
+            for (int q = 0; q < nS; ++q) {
+                if (!gate[q] || hs[q].status != kRunning) continue;
+                if (!(hs[q].finalRes < kStallFactor * prevRes[q])) {
+                    fallback[q] = 1;
+                    hs[q].status = kIdle;
+                    changed = true;
+                }
+                prevRes[q] = hs[q].finalRes;
+            }
+            if (changed) putOuter(hs);
+            // systems whose refinement has ended (not those continuing in FP64)
+            for (int q : newlyFinished(hs)) {
+                SolvePerf p = perfFp64(q, hs[q]);
+                p.nIterations = innerIters[q];
+                p.nOuter = hs[q].nIter;
+                p.bytes = p.nIterations * bytesIter(q, 4) + p.nOuter * bytesSpmv(q, 8);
+                (*retire_)(q, p);
+            }
+            bool restart = false;
+            for (int q = 0; q < nS; ++q) {
+                gate[q] = gate[q] && hs[q].status == kRunning;
+                restart = restart || gate[q];
+            }
+            if (restart) startInner();   // the next check follows a chunk
         }
-        // systems whose refinement has ended (not those continuing in FP64)
-        for (int q : newlyFinished(hs)) {
-            SolvePerf p = perfFp64(q, hs[q]);
-            p.nIterations = innerIters[q];
-            p.nOuter = hs[q].nIter;
-            p.bytes = p.nIterations * bytesIter(q, 4) + p.nOuter * bytesSpmv(q, 8);
-            (*retire_)(q, p);
-        }
+        bool running = false;
+        for (int q = 0; q < nS; ++q) running = running || hs[q].status == kRunning;
+        if (!running || done > budget) break;
+        runChunk<float>(pcg, c.checkInterval, c.graphs, A32, D32, K32);
+        done += c.checkInterval;
     }
     for (int q = 0; q < nS; ++q)
         if (hs[q].status == kRunning) fallback[q] = 1;   // outer steps exhausted

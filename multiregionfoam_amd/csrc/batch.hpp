@@ -331,6 +331,7 @@ private:
 
     int device_;
     hipStream_t stream_ = nullptr;
+    DevBuf<int8_t> gate_;   // per system: the mixed refinement's outer step applies
     hipStream_t side_ = nullptr;               // box DILU sweeps beside general ones
     hipEvent_t fork_ = nullptr, join_ = nullptr;
     bool preferLevels_ = false;

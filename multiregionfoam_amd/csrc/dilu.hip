@@ -479,10 +479,58 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             const bool valid = inTile && (unsigned)ia < (unsigned)nA;
             const bool hasA = kFwd ? ia >= 1 : ia + 1 < nA;
             const bool needB = valid & useB, needC = valid & useC;
+            T xb = xbN, xc = xcN;
+            // both exchange reads are issued by every lane (not sunk into the
+            // select), so the step has no divergent code
+            asm volatile("" : "+v"(xb), "+v"(xc));
+            const T vA = yprev;
+            auto slotVal = [&](int q) -> T { return (T)elem(S.v[q / N], q % N); };
+            T last;
+            if (kExtra) last = (T)elem(S.v[NL - 1], eLast);
+            else last = (T)elem(S.v[3 / N], 3 % N);
+            // the row's value from its neighbours' values
+            auto rowValue = [&](U hbv, U hcv) -> T {
+                const T vB = useB ? Bits<T>::from(hbv) : xb;
+                const T vC = useC ? Bits<T>::from(hcv) : xc;
+                T y;
+                if (MODE == kDiluFwd) y = S.a * last;
+                else if (MODE == kDiluBwd) y = S.a;
+                else y = last;
+                // the three terms in slot order, which is the face order.  In
+                // the sweeps a missing neighbour has a zero coefficient and a
+                // finite value (exchanged values of invalid rows are zero), so
+                // its term is an exact zero; the factor divides and selects
+                // instead.
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const T v = q == kSlotA ? vA : q == kSlotB ? vB : vC;
+                    const T c = slotVal(q);
+                    if (MODE == kDiluFactor) {
+                        const bool has = q == kSlotA ? hasA : q == kSlotB ? hasB : hasC;
+                        y = has ? y - c / v : y;
+                    } else {
+                        y = y - last * c * v;
+                    }
+                }
+                return y;
+            };
+            // the value is computed with the prefetched inbox values first and
+            // the exchange started; whether one of them had not arrived is
+            // tested after that, off the step's dependency chain
+            T y = rowValue(hb, hc);
+            T yz = valid ? y : T(0);
+            // all lanes have read the previous step's values (LDS operations
+            // of a wavefront execute in order) before any overwrites them
+            xch[lane] = yz;
+            __builtin_amdgcn_wave_barrier();
+            xbN = xch[srcB];
+            xcN = xch[srcC];
             const bool miss = !LONE && ((needB & (hb == kSent)) | (needC & (hc == kSent)));
             if (!LONE && __builtin_expect(__builtin_amdgcn_ballot_w64(miss) != 0, 0)) {
-                // rare slow path (wave-uniform): wait for the late values and
-                // reload the inbox values already requested for later steps
+                // rare slow path (wave-uniform): wait for the late values,
+                // reload the inbox values already requested for later steps,
+                // and redo the step: the row value, and the exchange (read
+                // again, as it may have taken the provisional values)
                 ++nSlow;
                 const U *ib = myIn + (size_t)t * TBC;
                 if (needB && hb == kSent) hb = haloWait(ib, cl, kSent, V.err);
@@ -496,44 +544,13 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                     if (vc == kSent) vc = haloLoad(fb, TC + bl);
                 }
                 __builtin_amdgcn_s_waitcnt(0);
+                y = rowValue(hb, hc);
+                yz = valid ? y : T(0);
+                xch[lane] = yz;
+                __builtin_amdgcn_wave_barrier();
+                xbN = xch[srcB];
+                xcN = xch[srcC];
             }
-            T xb = xbN, xc = xcN;
-            // both exchange reads are issued by every lane (not sunk into the
-            // select), so the step has no divergent code
-            asm volatile("" : "+v"(xb), "+v"(xc));
-            const T vA = yprev;
-            const T vB = useB ? Bits<T>::from(hb) : xb;
-            const T vC = useC ? Bits<T>::from(hc) : xc;
-            auto slotVal = [&](int q) -> T { return (T)elem(S.v[q / N], q % N); };
-            T last;
-            if (kExtra) last = (T)elem(S.v[NL - 1], eLast);
-            else last = (T)elem(S.v[3 / N], 3 % N);
-            T y;
-            if (MODE == kDiluFwd) y = S.a * last;
-            else if (MODE == kDiluBwd) y = S.a;
-            else y = last;
-            // the three terms in slot order, which is the face order.  In the
-            // sweeps a missing neighbour has a zero coefficient and a finite
-            // value (exchanged values of invalid rows are zero), so its term
-            // is an exact zero; the factor divides and selects instead.
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const T v = q == kSlotA ? vA : q == kSlotB ? vB : vC;
-                const T c = slotVal(q);
-                if (MODE == kDiluFactor) {
-                    const bool has = q == kSlotA ? hasA : q == kSlotB ? hasB : hasC;
-                    y = has ? y - c / v : y;
-                } else {
-                    y = y - last * c * v;
-                }
-            }
-            const T yz = valid ? y : T(0);
-            // all lanes have read the previous step's values (LDS operations
-            // of a wavefront execute in order) before any overwrites them
-            xch[lane] = yz;
-            __builtin_amdgcn_wave_barrier();
-            xbN = xch[srcB];
-            xcN = xch[srcC];
             yprev = yz;
             if (inRange) {
                 const unsigned so = (unsigned)t * 64 * sizeof(T);

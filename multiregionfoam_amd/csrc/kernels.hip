@@ -150,7 +150,7 @@ constexpr int kFinUnroll = 8;
 
 template <int NV>
 __global__ __launch_bounds__(kFinBlock) void k_finalize(int stage, const SysInfo *info, SysState *st,
-                                                        const double *partials)
+                                                        const double *partials, const int8_t *gate)
 {
     const int q = blockIdx.x;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(int stage, const SysInfo
     if (wid != 0) return;
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = waveSum(lane < kFinBlock / 64 ? sh[lane][j] : DD{0.0, 0.0});
-    if (lane == 0 && S.status == kRunning) {
+    if (lane == 0 && S.status == kRunning && (!gate || gate[q])) {
         double out[4] = {0.0, 0.0, 0.0, 0.0};
         for (int j = 0; j < NV; ++j) out[j] = acc[j].hi + acc[j].lo;
         applyStage(stage, S, out);
@@ -271,11 +271,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_spmv_plain(MatView<T> A, const Bl
 template <typename T, int CH>
 __global__ __launch_bounds__(kMaxBlock) void k_setup_residual(MatView<T> A, const BlockEntry *blocks, const SysInfo *info,
                                                          SysState *st, double *partials, const T *x,
-                                                         const T *b, T *Ax, T *r)
+                                                         const T *b, T *Ax, T *r, const int8_t *gate)
 {
     int sys, s, lane, p;
     bool ok = blockRow(blocks, sys, s, lane, p);
-    if (st[sys].status != kRunning) return;
+    if (st[sys].status != kRunning || (gate && !gate[sys])) return;
     double v[1] = {0.0};
     if (ok) {
         T ax = spmvRow<T, CH>(A, s, lane, p, x);
@@ -291,11 +291,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_setup_residual(MatView<T> A, cons
 template <typename T>
 __global__ __launch_bounds__(kMaxBlock) void k_setup_norm(const BlockEntry *blocks, const SysInfo *info, SysState *st,
                                                      double *partials, const double *sumA, const T *Ax,
-                                                     const T *b, T *r, T *rw, T *pv, T *vv, int isPCG)
+                                                     const T *b, T *r, T *rw, T *pv, T *vv, int isPCG,
+                                                     const int8_t *gate)
 {
     int sys, s, lane, p;
     bool ok = blockRow(blocks, sys, s, lane, p);
-    if (st[sys].status != kRunning) return;
+    if (st[sys].status != kRunning || (gate && !gate[sys])) return;
     const double xRef = st[sys].xSum / (double)st[sys].nCells;
     double v[3] = {0.0, 0.0, 0.0};
     if (ok) {
@@ -462,10 +463,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_pcg_xr(const BlockEntry *blocks, 
 
 // Fresh inner state for every system whose outer solve is still running.
 __global__ void k_refine_prepare(int nSys, const SysState *outer, SysState *inner, double innerRelTol,
-                                 int innerMaxIter)
+                                 int innerMaxIter, const int8_t *gate)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nSys) return;
+    if (q >= nSys || (gate && !gate[q])) return;
     SysState s = {};
     s.tol = 0.0;
     // reduce the correction's residual by innerRelTol, or only by as much as
@@ -485,11 +486,12 @@ __global__ void k_refine_prepare(int nSys, const SysState *outer, SysState *inne
 template <int CH>
 __global__ __launch_bounds__(kMaxBlock) void k_refine_residual(MatView<double> A, const BlockEntry *blocks,
                                                             const SysInfo *info, SysState *st, double *partials,
-                                                            const double *x, const double *b, double *r)
+                                                            const double *x, const double *b, double *r,
+                                                            const int8_t *gate)
 {
     int sys, s, lane, p;
     bool ok = blockRow(blocks, sys, s, lane, p);
-    if (st[sys].status != kRunning) return;
+    if (st[sys].status != kRunning || (gate && !gate[sys])) return;
     double acc[1] = {0.0};
     if (ok) {
         double rp = b[p] - spmvRow<double, CH>(A, s, lane, p, x);
@@ -501,22 +503,22 @@ __global__ __launch_bounds__(kMaxBlock) void k_refine_residual(MatView<double> A
 
 // b32 = (float) r64 for systems whose outer solve is running; x32 = 0
 __global__ __launch_bounds__(kMaxBlock) void k_refine_rhs(const BlockEntry *blocks, const SysState *outer,
-                                                       const double *r, float *b32, float *x32)
+                                                       const double *r, float *b32, float *x32, const int8_t *gate)
 {
     int sys, s, lane, p;
     if (!blockRow(blocks, sys, s, lane, p)) return;
-    if (outer[sys].status != kRunning) return;
+    if (outer[sys].status != kRunning || (gate && !gate[sys])) return;
     b32[p] = (float)r[p];
     x32[p] = 0.0f;
 }
 
 // x64 += (double) d32
 __global__ __launch_bounds__(kMaxBlock) void k_refine_update(const BlockEntry *blocks, const SysState *outer, double *x,
-                                                          const float *d)
+                                                          const float *d, const int8_t *gate)
 {
     int sys, s, lane, p;
     if (!blockRow(blocks, sys, s, lane, p)) return;
-    if (outer[sys].status != kRunning) return;
+    if (outer[sys].status != kRunning || (gate && !gate[sys])) return;
     x[p] += (double)d[p];
 }
 
@@ -533,9 +535,9 @@ constexpr int finSums(int stage)
 }
 
 template <int Stage, typename T>
-static void fin(const Krylov<T> &K, hipStream_t st)
+static void fin(const Krylov<T> &K, hipStream_t st, const int8_t *gate = nullptr)
 {
-    k_finalize<finSums(Stage)><<<K.nSys, kFinBlock, 0, st>>>(Stage, K.info, K.state, K.partials);
+    k_finalize<finSums(Stage)><<<K.nSys, kFinBlock, 0, st>>>(Stage, K.info, K.state, K.partials, gate);
 }
 
 // the SpMV kernels instantiated for the ELL chunk the batch was tuned to
@@ -559,18 +561,18 @@ hipError_t launchSpmvPlain(const MatView<T> &A, const BlockEntry *blocks, int nB
 }
 
 template <typename T>
-hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st)
+hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st, const int8_t *gate)
 {
     withChunk(A.ellChunk, [&](auto c) {
         k_setup_residual<T, decltype(c)::value><<<K.nBlk, K.threads, 0, st>>>(A, K.blocks, K.info, K.state,
-                                                                              K.partials, K.x, K.b, K.ax, K.r);
+                                                                              K.partials, K.x, K.b, K.ax, K.r, gate);
     });
-    fin<kFinXSum>(K, st);
+    fin<kFinXSum>(K, st, gate);
     k_setup_norm<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.info, K.state, K.partials, K.sumA, K.ax, K.b,
                                                K.r, pcg ? nullptr : K.rw, pcg ? K.pA : K.pv, pcg ? nullptr : K.v,
-                                               pcg ? 1 : 0);
-    if (pcg) fin<kFinInitPcg>(K, st);
-    else fin<kFinInitBicg>(K, st);
+                                               pcg ? 1 : 0, gate);
+    if (pcg) fin<kFinInitPcg>(K, st, gate);
+    else fin<kFinInitBicg>(K, st, gate);
     MRFG_CHECK_LAUNCH();
 }
 
@@ -645,31 +647,34 @@ hipError_t launchPcgIteration(const MatView<T> &A, const DiluView<T> &D, const K
 }
 
 hipError_t launchRefinePrepare(int nSys, const SysState *outer, SysState *inner, double innerRelTol,
-                               int innerMaxIter, hipStream_t st)
+                               int innerMaxIter, const int8_t *gate, hipStream_t st)
 {
-    k_refine_prepare<<<(nSys + 63) / 64, 64, 0, st>>>(nSys, outer, inner, innerRelTol, innerMaxIter);
+    k_refine_prepare<<<(nSys + 63) / 64, 64, 0, st>>>(nSys, outer, inner, innerRelTol, innerMaxIter, gate);
     MRFG_CHECK_LAUNCH();
 }
 
-hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &K64, hipStream_t st)
+hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &K64, const int8_t *gate,
+                                hipStream_t st)
 {
     withChunk(A.ellChunk, [&](auto c) {
         k_refine_residual<decltype(c)::value><<<K64.nBlk, K64.threads, 0, st>>>(A, K64.blocks, K64.info, K64.state,
-                                                                               K64.partials, K64.x, K64.b, K64.r);
+                                                                               K64.partials, K64.x, K64.b, K64.r,
+                                                                               gate);
     });
-    fin<kFinRefine>(K64, st);
+    fin<kFinRefine>(K64, st, gate);
     MRFG_CHECK_LAUNCH();
 }
 
-hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st)
+hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, const int8_t *gate, hipStream_t st)
 {
-    k_refine_rhs<<<K64.nBlk, K64.threads, 0, st>>>(K64.blocks, K64.state, K64.r, K32.b, K32.x);
+    k_refine_rhs<<<K64.nBlk, K64.threads, 0, st>>>(K64.blocks, K64.state, K64.r, K32.b, K32.x, gate);
     MRFG_CHECK_LAUNCH();
 }
 
-hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st)
+hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, const int8_t *gate,
+                              hipStream_t st)
 {
-    k_refine_update<<<K64.nBlk, K64.threads, 0, st>>>(K64.blocks, K64.state, K64.x, K32.x);
+    k_refine_update<<<K64.nBlk, K64.threads, 0, st>>>(K64.blocks, K64.state, K64.x, K32.x, gate);
     MRFG_CHECK_LAUNCH();
 }
 
@@ -841,7 +846,7 @@ hipError_t copyProbe(int device, size_t bytes, int reps, double *ms)
 #define MRFG_INSTANTIATE(T)                                                                                          \
     template hipError_t launchSpmvPlain<T>(const MatView<T> &, const BlockEntry *, int, int, const T *, T *,         \
                                            hipStream_t);                                                             \
-    template hipError_t launchSetup<T>(const MatView<T> &, const Krylov<T> &, bool, hipStream_t);                    \
+    template hipError_t launchSetup<T>(const MatView<T> &, const Krylov<T> &, bool, hipStream_t, const int8_t *);                    \
     template hipError_t launchPrecondition<T>(const DiluView<T> &, const Krylov<T> &, const T *, T *, hipStream_t);   \
     template hipError_t launchBicgIteration<T>(const MatView<T> &, const DiluView<T> &, const Krylov<T> &,           \
                                                hipStream_t, const IterProbe *);                                      \

@@ -29,7 +29,9 @@ template <typename T>
 hipError_t launchSpmvPlain(const MatView<T> &A, const BlockEntry *blocks, int nBlk, int threads, const T *x, T *y,
                            hipStream_t st);
 template <typename T>
-hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st);
+// gate (optional, one byte per system): only systems with gate[q] != 0 are set up
+hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStream_t st,
+                       const int8_t *gate = nullptr);
 template <typename T>
 hipError_t launchPrecondition(const DiluView<T> &D, const Krylov<T> &K, const T *in, T *out, hipStream_t st);
 // Events recorded around one launch of a BiCGStab iteration (timing only).
@@ -61,11 +63,15 @@ hipError_t launchInterfacePatch(int nFaces, const int32_t *fcPos, const int32_t 
 // writes `bytes` per launch), for calibrating HBM traffic counters.
 hipError_t copyProbe(int device, size_t bytes, int reps, double *ms);
 
+// The refinement steps act on the systems whose outer solve runs and, when
+// a gate is given, whose gate byte is set (one byte per system).
 hipError_t launchRefinePrepare(int nSys, const SysState *outer, SysState *inner, double innerRelTol,
-                               int innerMaxIter, hipStream_t st);
-hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &K64, hipStream_t st);
-hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st);
-hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, hipStream_t st);
+                               int innerMaxIter, const int8_t *gate, hipStream_t st);
+hipError_t launchRefineResidual(const MatView<double> &A, const Krylov<double> &K64, const int8_t *gate,
+                                hipStream_t st);
+hipError_t launchRefineRhs(const Krylov<double> &K64, const Krylov<float> &K32, const int8_t *gate, hipStream_t st);
+hipError_t launchRefineUpdate(const Krylov<double> &K64, const Krylov<float> &K32, const int8_t *gate,
+                              hipStream_t st);
 
 // Time stepping of transient systems (rows of steady systems have a = 0):
 // diag = diagBase + coefft*a, written to the SpMV diagonal and to the factor
