@@ -68,6 +68,8 @@ def kernel_probe(A, reps):
 # each check interval's iterations replayed as one captured HIP graph (off:
 # kernels launched one by one, e.g. under a profiler's kernel trace)
 GRAPHS = True
+# the mixed refinement's inner relative tolerance
+INNER_REL_TOL = 1e-4
 
 
 def progress(msg):
@@ -87,7 +89,7 @@ def run_exchanger(m, cases, args):
     res = {}
     for prec in ("fp64", "mixed"):
         ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
-                          innerRelTol=1e-4)
+                          innerRelTol=INNER_REL_TOL)
         total, lat, perf = run_steps(A, ctrl, args.warmup, args.steps, verbose=True)
         flux = [A.interface_flux(i) for i in range(2)]
         res[prec] = dict(solves_s=args.steps / total, ms_per_step=1e3 * total / args.steps,
@@ -453,7 +455,7 @@ def run_batch(m, cases, args):
     for prec in ("fp64", "mixed"):
         if prec == "mixed" and args.no_mixed:
             continue
-        ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=1e-4)
+        ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=INNER_REL_TOL)
         total, lat, perfs = run_steps(A, ctrl, args.warmup, args.steps, all_perf=True, verbose=True)
         res[prec] = dict(systems_s=n * args.steps / total, ms_per_batch=1e3 * total / args.steps,
                          converged=sum(p.converged for p in perfs), max_iterations=max(p.n_iterations for p in perfs),
@@ -541,7 +543,7 @@ def run_serving(m, cases, args):
         if prec == "mixed" and args.no_mixed:
             continue
         srv = m.Server(device=0, hbm_budget=200 << 30, workers=args.workers, max_batch=args.max_batch,
-                       prep_threads=args.prep, tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=1e-4)
+                       prep_threads=args.prep, tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=INNER_REL_TOL)
         # warm-up: one small request through every slot
         for _ in range(args.workers):
             srv.submit(texts[0][1])
@@ -616,6 +618,8 @@ def main():
     ap.add_argument("--copies", type=int, default=9,
                     help="suite, suite_partitioned: cases in the batch, cycling through the nine parameter sets "
                          "(default 9)")
+    ap.add_argument("--inner-rel-tol", type=float, default=1e-4,
+                    help="relative tolerance of the mixed refinement's FP32 inner solves")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch each iteration's kernels one by one instead of replaying HIP graphs")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -623,8 +627,9 @@ def main():
                          "device copy of known size (for rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE "
                          "per dispatch, the copy giving the counter's scale)")
     args = ap.parse_args()
-    global GRAPHS
+    global GRAPHS, INNER_REL_TOL
     GRAPHS = not args.no_graphs
+    INNER_REL_TOL = args.inner_rel_tol
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus %d is not run "
               "(the multi-GPU sharding is described in DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -712,7 +717,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         mctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
-                           innerRelTol=1e-4)
+                           innerRelTol=INNER_REL_TOL)
         mt, mlat, mperf = run_steps(A, mctrl, args.warmup, args.steps)
         psim = np.concatenate(A.psi(0))
         mixed = dict(value=args.steps / mt, ms_per_step=1e3 * mt / args.steps,

@@ -510,6 +510,12 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
     ifPairOff_.upload(ifPairOff, st);
     ifPairCol_.upload(ifPairCol, st);
     cellOfPos_.upload(cellOfPos, st);
+    {
+        std::vector<unsigned long long> rm(std::max(1, slice), 0ull);
+        for (size_t p = 0; p < cellOfPos.size(); ++p)
+            if (cellOfPos[p] >= 0) rm[p / kSlice] |= 1ull << (p % kSlice);
+        rowMask_.upload(rm, st);
+    }
     blocks_.upload(blocks, st);
     info_.upload(hInfo_, st);
     for (int k = 0; k < 2; ++k) {
@@ -629,6 +635,7 @@ MatView<T> Batch::view(const DevMatrix<T> &m) const
     A.ellVal = m.ellVal.p;
     A.diag = m.diag.p;
     A.ifMask = ifMask_.p;
+    A.rowMask = rowMask_.p;
     A.ifBase = ifBase_.p;
     A.ifRowFace = ifRowFace_.p;
     A.ifCoeff = m.ifCoeff.p;

@@ -362,6 +362,7 @@ private:
     void setTuning(int slices, int ellChunk);
     // DILU schedule
     DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_, waveOut_, boxOrder_, boxMeta_;
+    DevBuf<unsigned long long> rowMask_;   // per slice: lanes holding a cell
     DevBuf<int4> fCodes_, bCodes_;
     DevBuf<double2> prodPack_;
     int haloMaskF_ = 0, haloMaskB_ = 0;

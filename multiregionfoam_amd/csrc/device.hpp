@@ -43,6 +43,7 @@ struct MatView {
     const T *ellVal;
     const T *diag;
     const unsigned long long *ifMask;
+    const unsigned long long *rowMask;   // per slice: the lanes that hold a cell (the others are padding, y = 0)
     const int32_t *ifBase;
     const int32_t *ifRowFace;
     const T *ifCoeff;

@@ -209,6 +209,9 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(int stage, const SysInfo
 template <typename T, int CH>
 __device__ inline T spmvRow(const MatView<T> &A, int s, int lane, int p, const T *__restrict__ x)
 {
+    // a padding position (no cell) has no coefficients: its row is zero, and
+    // its lanes load nothing, so slices cut by a tile's skew fetch less
+    if (!((A.rowMask[s] >> lane) & 1ull)) return T(0);
     T y = A.diag[p] * x[p];
     const int e0 = A.ellOff[s], e1 = A.ellOff[s + 1];
     // chunks of CH entries (CH autotuned per batch): every column, value and
