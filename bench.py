@@ -68,8 +68,9 @@ def kernel_probe(A, reps):
 # each check interval's iterations replayed as one captured HIP graph (off:
 # kernels launched one by one, e.g. under a profiler's kernel trace)
 GRAPHS = True
-# the mixed refinement's inner relative tolerance
+# the mixed refinement's inner relative tolerance and inner preconditioner
 INNER_REL_TOL = 1e-4
+INNER_PRE = "tileDILU"
 
 
 def progress(msg):
@@ -89,7 +90,7 @@ def run_exchanger(m, cases, args):
     res = {}
     for prec in ("fp64", "mixed"):
         ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
-                          innerRelTol=INNER_REL_TOL)
+                          innerRelTol=INNER_REL_TOL, innerPreconditioner=INNER_PRE)
         total, lat, perf = run_steps(A, ctrl, args.warmup, args.steps, verbose=True)
         flux = [A.interface_flux(i) for i in range(2)]
         res[prec] = dict(solves_s=args.steps / total, ms_per_step=1e3 * total / args.steps,
@@ -111,7 +112,7 @@ def run_exchanger(m, cases, args):
                    "cells": info["cells"], "internal_faces": info["faces"], "ggi_pairs": info["pairs"]},
         "mixed": res["mixed"], "fp64": res["fp64"], "mixed_speedup_vs_fp64":
             res["mixed"]["solves_s"] / res["fp64"]["solves_s"],
-        "mixed_rel_l2_vs_fp64": err,
+        "mixed_rel_l2_vs_fp64": err, "inner_preconditioner": INNER_PRE,
     }
     print(json.dumps(out))
 
@@ -455,7 +456,8 @@ def run_batch(m, cases, args):
     for prec in ("fp64", "mixed"):
         if prec == "mixed" and args.no_mixed:
             continue
-        ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec, innerRelTol=INNER_REL_TOL)
+        ctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=3000, precision=prec,
+                          innerRelTol=INNER_REL_TOL, innerPreconditioner=INNER_PRE)
         total, lat, perfs = run_steps(A, ctrl, args.warmup, args.steps, all_perf=True, verbose=True)
         res[prec] = dict(systems_s=n * args.steps / total, ms_per_batch=1e3 * total / args.steps,
                          converged=sum(p.converged for p in perfs), max_iterations=max(p.n_iterations for p in perfs),
@@ -511,7 +513,7 @@ def run_batch(m, cases, args):
                    "systems": n, "cells": int(sum(cells))},
         "fp64": res["fp64"], "mixed": res.get("mixed"),
         "mixed_speedup_vs_fp64": (res["mixed"]["systems_s"] / res["fp64"]["systems_s"]) if "mixed" in res else None,
-        "tuning": t, "kernels_in_solve": kernels,
+        "inner_preconditioner": INNER_PRE, "tuning": t, "kernels_in_solve": kernels,
         "hbm_gbs_algorithmic": sum(it * A.bytes_model(q, 8)["iter"] for q, it in enumerate(res["fp64"]["iterations"]))
         / (res["fp64"]["ms_per_batch"] * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -620,6 +622,9 @@ def main():
                          "(default 9)")
     ap.add_argument("--inner-rel-tol", type=float, default=1e-4,
                     help="relative tolerance of the mixed refinement's FP32 inner solves")
+    ap.add_argument("--inner-pre", choices=("tileDILU", "DILU"), default="tileDILU",
+                    help="preconditioner of the mixed refinement's FP32 inner solves: each box tile's DILU "
+                         "alone (default), or the full DILU")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch each iteration's kernels one by one instead of replaying HIP graphs")
     ap.add_argument("--kernel-probe", action="store_true",
@@ -627,9 +632,10 @@ def main():
                          "device copy of known size (for rocprofv3 --pmc runs: FETCH_SIZE / WRITE_SIZE "
                          "per dispatch, the copy giving the counter's scale)")
     args = ap.parse_args()
-    global GRAPHS, INNER_REL_TOL
+    global GRAPHS, INNER_REL_TOL, INNER_PRE
     GRAPHS = not args.no_graphs
     INNER_REL_TOL = args.inner_rel_tol
+    INNER_PRE = args.inner_pre
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X, so --gpus %d is not run "
               "(the multi-GPU sharding is described in DESIGN.md)" % args.gpus, file=sys.stderr)
@@ -717,14 +723,14 @@ def main():
     mixed = None
     if not args.no_mixed:
         mctrl = m.controls(graphs=GRAPHS, solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
-                           innerRelTol=INNER_REL_TOL)
+                           innerRelTol=INNER_REL_TOL, innerPreconditioner=INNER_PRE)
         mt, mlat, mperf = run_steps(A, mctrl, args.warmup, args.steps)
         psim = np.concatenate(A.psi(0))
         mixed = dict(value=args.steps / mt, ms_per_step=1e3 * mt / args.steps,
                      speedup_vs_fp64=(args.steps / mt) / solves_s,
                      inner_iterations=mperf.n_iterations, outer_steps=mperf.n_outer,
                      final_residual=mperf.final_residual, converged=mperf.converged,
-                     p50_ms=1e3 * pct(mlat, 0.5), p99_ms=1e3 * pct(mlat, 0.99))
+                     p50_ms=1e3 * pct(mlat, 0.5), p99_ms=1e3 * pct(mlat, 0.99), inner_preconditioner=INNER_PRE)
 
     # ---- per-kernel timing (HIP events on the library stream) ----
     # back to back (the kernel alone, its data hot in the caches), and inside

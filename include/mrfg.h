@@ -84,10 +84,17 @@ typedef struct {
                               ELL chunk 2, 4, 8 or 16) on the batch and keeps the fastest;
                               MRFG_TUNE_FIXED (1): 4 slices, chunk 8 (fvSolution key
                               autotune on|off).  Results do not depend on the choice. */
+    int inner_preconditioner; /* mixed: the FP32 inner solve's preconditioner (fvSolution key
+                              refinement { innerPreconditioner tileDILU|DILU; }).
+                              MRFG_INNER_TILE_DILU (0, default): each structured-box tile's DILU
+                              alone (couplings across tile edges left out: the sweeps wait for no
+                              other tile); MRFG_INNER_DILU (1): the full DILU in FP32.  The FP64
+                              refinement sets the result's accuracy either way. */
 } mrfg_controls;
 
 enum { MRFG_LAUNCH_GRAPH = 0, MRFG_LAUNCH_KERNELS = 1 };
 enum { MRFG_TUNE_AUTO = 0, MRFG_TUNE_FIXED = 1 };
+enum { MRFG_INNER_TILE_DILU = 0, MRFG_INNER_DILU = 1 };
 
 /* ---- context --------------------------------------------------------- */
 
@@ -411,6 +418,15 @@ int mrfg_precondition(mrfg_ctx *ctx, int sys, const double *r, double *w);
    solver, with rD rounded from the FP64 factor (no counterpart in the
    reference, which is FP64 only). */
 int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w);
+/* The mixed solver's FP32 inner preconditioner, MRFG_INNER_TILE_DILU (each
+   structured-box tile's DILU alone, couplings across tile edges left out;
+   rD of that tile-local factor) or MRFG_INNER_DILU (as mrfg_precondition_fp32).
+   No counterpart in the reference. */
+int mrfg_precondition_inner(mrfg_ctx *ctx, int sys, int inner_preconditioner, const float *r, float *w);
+/* Per cell of system sys (concatenated region order): the index of the
+   structured-box tile holding it in the DILU schedule, or -1 for a cell
+   scheduled on general waves (the tiles of the tile-local preconditioner). */
+int mrfg_dilu_tiles(mrfg_ctx *ctx, int sys, int *tile_of_cell);
 /* Average device time (ms) of one launch over `reps` launches, from HIP
    events on the context's stream: which 0 = SpMV, 1 = DILU forward sweep,
    2 = DILU backward sweep, 3 = DILU factorisation, each launched back to

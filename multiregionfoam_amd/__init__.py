@@ -27,6 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmrfg.so")
 
 MRFG_OK = 0
+MRFG_INNER_TILE_DILU, MRFG_INNER_DILU = 0, 1   # the mixed solver's inner preconditioners
 ERRORS = {-1: "argument", -2: "device", -3: "dictionary", -4: "state"}
 
 
@@ -53,7 +54,7 @@ class Controls(C.Structure):
                 ("rel_tol", C.c_double), ("min_iter", C.c_int), ("max_iter", C.c_int),
                 ("inner_rel_tol", C.c_double), ("inner_max_iter", C.c_int),
                 ("max_outer", C.c_int), ("check_interval", C.c_int), ("launch_mode", C.c_int),
-                ("tuning", C.c_int)]
+                ("tuning", C.c_int), ("inner_preconditioner", C.c_int)]
 
 
 _lib = None
@@ -122,6 +123,8 @@ def lib():
         "mrfg_dilu_factor": ([vp, C.c_int, F64P], C.c_int),
         "mrfg_precondition": ([vp, C.c_int, F64P, F64P], C.c_int),
         "mrfg_precondition_fp32": ([vp, C.c_int, F32P, F32P], C.c_int),
+        "mrfg_precondition_inner": ([vp, C.c_int, C.c_int, F32P, F32P], C.c_int),
+        "mrfg_dilu_tiles": ([vp, C.c_int, I32P], C.c_int),
         "mrfg_time_kernel": ([vp, C.c_int, C.c_int, F64P], C.c_int),
         "mrfg_layout_info": ([vp, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
         "mrfg_tuning": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), F64P, F64P], C.c_int),
@@ -164,7 +167,7 @@ EXPORTED = (
     "mrfg_case_region_source_base "
     "mrfg_case_interface_sizes mrfg_case_interface mrfg_case_interface_wall mrfg_set_interface_wall "
     "mrfg_interface_patch_values mrfg_add_case mrfg_spmv mrfg_dilu_factor "
-    "mrfg_precondition mrfg_precondition_fp32 mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_tuning mrfg_set_tuning mrfg_bytes mrfg_copy_probe mrfg_interface_flux mrfg_interface_flux_noalloc "
+    "mrfg_precondition mrfg_precondition_fp32 mrfg_precondition_inner mrfg_dilu_tiles mrfg_time_kernel mrfg_trace_dilu mrfg_layout_info mrfg_tuning mrfg_set_tuning mrfg_bytes mrfg_copy_probe mrfg_interface_flux mrfg_interface_flux_noalloc "
     "mrfg_set_field mrfg_assemble mrfg_get_region_matrix mrfg_get_interface_coeff mrfg_assembly_bytes "
     "mrfg_server_create mrfg_server_destroy mrfg_server_submit mrfg_server_wait_all mrfg_server_result "
     "mrfg_server_get_psi mrfg_server_times mrfg_server_last_error").split()
@@ -197,14 +200,14 @@ def parse_controls(fv_solution: str, entry: str) -> Controls:
 
 def _controls_text(solver="BiCGStab", tolerance=1e-10, relTol=0.0, minIter=0, maxIter=1000,
                    precision="fp64", innerRelTol=1e-4, innerMaxIter=1000, maxOuter=50, checkInterval=8,
-                   graphs=True, autotune=True):
+                   graphs=True, autotune=True, innerPreconditioner="tileDILU"):
     """fvSolution text with one solver entry "X" holding the given keys."""
     return ("solvers { X { solver %s; preconditioner { preconditioner Cholesky; } tolerance %r; "
             "relTol %r; minIter %d; maxIter %d; precision %s; checkInterval %d; graphs %s; autotune %s; "
-            "refinement { innerRelTol %r; innerMaxIter %d; maxOuter %d; } } }"
+            "refinement { innerRelTol %r; innerMaxIter %d; maxOuter %d; innerPreconditioner %s; } } }"
             % (solver, float(tolerance), float(relTol), minIter, maxIter, precision, checkInterval,
                "on" if graphs else "off", "on" if autotune else "off", float(innerRelTol), innerMaxIter,
-               maxOuter))
+               maxOuter, innerPreconditioner))
 
 
 def controls(**kw):
@@ -643,6 +646,22 @@ class CoupledFvMatrix:
         w = np.zeros_like(r)
         self._check(lib().mrfg_precondition_fp32(self._h, sys, _p(r, C.c_float), _p(w, C.c_float)))
         return w
+
+    def precondition_inner(self, r, inner="tileDILU", sys=0):
+        """w = M^-1 r with the mixed solver's FP32 inner preconditioner:
+        "tileDILU" (each box tile's DILU alone) or "DILU" (the full DILU)."""
+        mode = {"tileDILU": MRFG_INNER_TILE_DILU, "DILU": MRFG_INNER_DILU}[inner]
+        r = np.ascontiguousarray(np.concatenate(r) if isinstance(r, (list, tuple)) else r, dtype=np.float32)
+        w = np.zeros_like(r)
+        self._check(lib().mrfg_precondition_inner(self._h, sys, mode, _p(r, C.c_float), _p(w, C.c_float)))
+        return w
+
+    def dilu_tiles(self, sys=0):
+        """Per cell (concatenated regions): the DILU schedule's box tile
+        holding it, or -1 for a cell on general waves."""
+        out = np.zeros(sum(self._sizes[sys]), dtype=np.int32)
+        self._check(lib().mrfg_dilu_tiles(self._h, sys, _p(out, C.c_int32)))
+        return out
 
     def interface_flux(self, iface=0, sys=0):
         """(Q_a, Q_b): heat flow into each side of a monolithic interface from

@@ -100,6 +100,12 @@ SolverControls parseControls(const Dict &d)
         c.innerRelTol = r.numberOrDefault("innerRelTol", c.innerRelTol);
         c.innerMaxIter = (int)r.numberOrDefault("innerMaxIter", c.innerMaxIter);
         c.maxOuter = (int)r.numberOrDefault("maxOuter", c.maxOuter);
+        if (r.found("innerPreconditioner")) {
+            const std::string ip = r.word("innerPreconditioner");
+            if (ip == "DILU") c.innerPre = SolverControls::InnerDILU;
+            else if (ip == "tileDILU") c.innerPre = SolverControls::InnerTileDILU;
+            else throw DictError("Unknown innerPreconditioner '" + ip + "'; valid: DILU tileDILU");
+        }
     }
     c.checkInterval = std::max(1, (int)d.numberOrDefault("checkInterval", c.checkInterval));
     if (d.found("autotune")) {
@@ -493,6 +499,8 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
     nGenWaves_ = (int)order.size();
     nBoxWaves_ = (int)boxOrder.size();
     nBoxIn_ = boxIn;
+    hBoxMeta_ = boxMeta;
+    hBoxWaves_.assign(boxOrder.begin(), boxOrder.end());
     boxLone_ = nBoxWaves_ > 0;
     for (int w : boxOrder)
         for (int k : {kBmPrevB, kBmPrevC, kBmNextB, kBmNextC})
@@ -590,11 +598,13 @@ void Batch::setSystems(const std::vector<const CoupledSystem *> &systems,
     HIP_OK(hipStreamSynchronize(st));
 }
 
-void Batch::ensureFloat()
+void Batch::ensureFloat(bool tileLocal)
 {
-    if (haveFloat_) return;
+    tileLocal = tileLocal && nBoxWaves_ > 0;
+    if (haveFloat_ && tileLocal == tileLocal32_) return;
+    tileLocal32_ = tileLocal;
     if (hostStale_) refreshHostCopies();
-    else derivePack32();
+    else derivePack32(tileLocal);
     dropGraphs();   // FP32 buffers may be allocated anew
     auto toF = [](const std::vector<double> &a) {
         std::vector<float> b(a.size());
@@ -658,7 +668,8 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.K = K_;
     D.boxOrder = boxOrder_.p;
     D.nBoxWaves = nBoxWaves_;
-    D.boxLone = boxLone_;
+    // the FP32 records of the tile-local DILU are swept tile by tile
+    D.boxLone = boxLone_ || ((const void *)&m == (const void *)&m32_ && tileLocal32_);
     D.boxMeta = boxMeta_.p;
     D.nBoxIn = nBoxIn_;
     D.fCodes = fCodes_.p;
@@ -779,7 +790,16 @@ void Batch::factorise()
     HIP_OK(mrfg::diluFactor(D, stream_));
     if (haveFloat_) {
         DiluView<float> D32 = diluView(m32_, haloF32_.p, haloB32_.p);
-        HIP_OK(diluConvertRD(D32, v64_[11].p, nSlices_, stream_));
+        const double *rd = v64_[11].p;
+        if (tileLocal32_) {
+            // the general waves' rows keep the full factor's rD
+            rdLocal_.alloc(nPos_);
+            HIP_OK(hipMemcpyAsync(rdLocal_.p, v64_[11].p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice,
+                                  stream_));
+            HIP_OK(diluFactorTileLocal(D, rdLocal_.p, stream_));
+            rd = rdLocal_.p;
+        }
+        HIP_OK(diluConvertRD(D32, rd, nSlices_, stream_));
     }
     factorised_ = true;
 }
@@ -1104,7 +1124,7 @@ void Batch::solve(const SolverControls &c, std::vector<SolvePerf> &perf, const s
     if (active)
         for (size_t q = 0; q < sys_.size(); ++q) retired_[q] = (*active)[q] ? 0 : 1;
     applyRefresh();
-    if (c.precision == SolverControls::Mixed) ensureFloat();
+    if (c.precision == SolverControls::Mixed) ensureFloat(c.innerPre == SolverControls::InnerTileDILU);
     if (c.autotune && !tune_.tuned) autotune();
     if (!c.autotune) setTuning(4, 8);
     HIP_OK(hipStreamSynchronize(stream_));
@@ -1390,10 +1410,23 @@ void Batch::setInterfaceWall(int sys, int iface, int side, const double *kOwn, c
     slot.wall = true;
 }
 
-void Batch::precondition32(const float *rHost, float *wHost, int sys)
+void Batch::tilesOf(int sys, int *tileOfCell) const
+{
+    const SystemLayout &L = *lay_.at(sys);
+    const int w0 = hInfo_.at(sys).waveBegin;
+    std::vector<int> waveOfSlice(L.nSlices, -1);
+    for (int w = 0; w < L.nWaves; ++w)
+        for (int t = 0; t < L.waveSteps[w]; ++t) waveOfSlice[L.waveSlice[w] + t] = w;
+    for (long long c = 0; c < L.nCells; ++c) {
+        const int w = waveOfSlice[L.posOfCell[c] / kSlice];
+        tileOfCell[c] = L.waveBox[w] ? w0 + w : -1;
+    }
+}
+
+void Batch::precondition32(const float *rHost, float *wHost, int sys, bool tileLocal)
 {
     applyRefresh();
-    ensureFloat();
+    ensureFloat(tileLocal);
     factorise();
     const SystemLayout &L = *lay_.at(sys);
     std::vector<float> r(nPos_, 0.0f), w(nPos_);

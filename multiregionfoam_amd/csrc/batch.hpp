@@ -40,6 +40,12 @@ struct SolverControls {
     double innerRelTol = 1e-4;
     int innerMaxIter = 1000;
     int maxOuter = 50;
+    // mixed: the FP32 inner solve's preconditioner.  DILU: the FP64 path's
+    // DILU in FP32.  TileDILU: every box tile's DILU alone, the couplings
+    // across tile edges left out, so the sweeps have no cross-tile waits
+    // (regions on general waves keep the full DILU).  The outer FP64
+    // refinement sets the result's accuracy either way.
+    enum InnerPre { InnerDILU = 0, InnerTileDILU = 1 } innerPre = InnerTileDILU;
     int checkInterval = 8;    // iterations between host convergence checks
     bool graphs = true;       // replay each check interval's iterations as one captured HIP graph
     bool autotune = true;     // time the SpMV launch configurations on the batch once, use the fastest
@@ -187,7 +193,12 @@ public:
     void spmv(const double *xHost, double *yHost, int sys);           // y = A x
     void diluFactor(int sys, double *rDHost);                        // rD per cell
     void precondition(const double *rHost, double *wHost, int sys);   // w = M^-1 r
-    void precondition32(const float *rHost, float *wHost, int sys);   // FP32 records, rD from the FP64 factor
+    // FP32 records, rD from the FP64 factor; tileLocal: the mixed solver's
+    // tile-local DILU (each box tile alone)
+    void precondition32(const float *rHost, float *wHost, int sys, bool tileLocal = false);
+    // per cell of system sys (concatenated regions): the batch-wide index of
+    // its structured-box tile wave, or -1 for a cell on a general wave
+    void tilesOf(int sys, int *tileOfCell) const;
     // Heat flow into each side of interface `iface` of system `sys` from the
     // current FP64 field, on the device: Q = sum coeff*(T_neighbour - T_own)
     // per side (a conserving interface gives qa + qb = 0).
@@ -281,7 +292,9 @@ private:
     // captures a graph)
     void copyToHost(void *dst, const void *src, size_t bytes) const;
     void copyToDevice(void *dst, const void *src, size_t bytes) const;
-    void ensureFloat();
+    // the FP32 copies; tileLocal: box tiles' records without the
+    // coefficients across tile edges, rD of the tile-local factor
+    void ensureFloat(bool tileLocal);
     void factorise();
     void initState(const SolverControls &c, const std::vector<char> *active = nullptr);
     void readState(std::vector<SysState> &out);
@@ -326,8 +339,9 @@ private:
     FvPlace fvPlaceView();
     // the host copies of the FP64 values after the device changed them
     void refreshHostCopies();
-    // the FP32 DILU records (host) from the FP64 ones
-    void derivePack32();
+    // the FP32 DILU records (host) from the FP64 ones (tileLocal: without the
+    // coefficients across box tile edges)
+    void derivePack32(bool tileLocal);
 
     int device_;
     hipStream_t stream_ = nullptr;
@@ -342,6 +356,7 @@ private:
     int nGenWaves_ = 0, nBoxWaves_ = 0, nBoxIn_ = 0;
     bool boxLone_ = false;   // every box tile alone in its region
     bool factorised_ = false, haveFloat_ = false;
+    bool tileLocal32_ = false;   // the FP32 records are those of the tile-local DILU
 
     // structure (shared by both precisions)
     DevBuf<int32_t> ellOff_, ellCol_, ifBase_, ifRowFace_, ifPairOff_, ifPairCol_;
@@ -362,6 +377,8 @@ private:
     void setTuning(int slices, int ellChunk);
     // DILU schedule
     DevBuf<int32_t> order_, waveSlice_, waveSteps_, waveSys_, waveOut_, boxOrder_, boxMeta_;
+    std::vector<int32_t> hBoxMeta_, hBoxWaves_;   // host copies: box tile geometry, the box waves
+    DevBuf<double> rdLocal_;                      // rD of the tile-local factor
     DevBuf<unsigned long long> rowMask_;   // per slice: lanes holding a cell
     DevBuf<int4> fCodes_, bCodes_;
     DevBuf<double2> prodPack_;

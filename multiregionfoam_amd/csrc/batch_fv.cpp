@@ -523,23 +523,44 @@ void Batch::refreshHostCopies()
     down(hIfCoeff_, m64_.ifCoeff.p);
     down(hFPack_, m64_.fPack.p);
     down(hBPack_, m64_.bPack.p);
-    derivePack32();
+    derivePack32(tileLocal32_);
     hostStale_ = false;
 }
 
-void Batch::derivePack32()
+void Batch::derivePack32(bool tileLocal)
 {
     const int G = packChunks(K_ + 1, 2), G32 = packChunks(K_ + 1, 4);
     hFPack32_.assign((size_t)nSlices_ * G32 * 64 * 4, 0.0f);
     hBPack32_.assign((size_t)nSlices_ * G32 * 64 * 4, 0.0f);
+    auto at32 = [G32](int s, int k, int l) { return (((size_t)s * G32 + k / 4) * 64 + l) * 4 + k % 4; };
     for (int s = 0; s < nSlices_; ++s)
         for (int k = 0; k < K_; ++k)
             for (int l = 0; l < kSlice; ++l) {
                 const size_t e64 = (((size_t)s * G + k / 2) * 64 + l) * 2 + k % 2;
-                const size_t e32 = (((size_t)s * G32 + k / 4) * 64 + l) * 4 + k % 4;
-                hFPack32_[e32] = (float)hFPack_[e64];
-                hBPack32_[e32] = (float)hBPack_[e64];
+                hFPack32_[at32(s, k, l)] = (float)hFPack_[e64];
+                hBPack32_[at32(s, k, l)] = (float)hBPack_[e64];
             }
+    if (!tileLocal) return;
+    // The tile-local DILU: a box tile's row whose lower (forward record) or
+    // upper (backward record) neighbour lies in the neighbouring tile drops
+    // that coefficient.  Lane l = bl + TB*cl of a tile holds line (bl, cl); the
+    // neighbour along axis q sits in record slot 2 - q.
+    for (int w : hBoxWaves_) {
+        const int32_t *m = &hBoxMeta_[(size_t)w * kBoxMeta];
+        const int TB = m[kBmTB], TC = kSlice / TB, ax = m[kBmAxis];
+        const int b = ax == 0 ? 1 : 0, c = ax == 2 ? 1 : 2;
+        const int slotB = 2 - b, slotC = 2 - c;
+        for (int t = 0; t < m[kBmSteps]; ++t) {
+            const int s = m[kBmSlice] + t;
+            for (int l = 0; l < kSlice; ++l) {
+                const int bl = l % TB, cl = l / TB;
+                if (bl == 0 && m[kBmPrevB] >= 0) hFPack32_[at32(s, slotB, l)] = 0.0f;
+                if (cl == 0 && m[kBmPrevC] >= 0) hFPack32_[at32(s, slotC, l)] = 0.0f;
+                if (bl == TB - 1 && m[kBmNextB] >= 0) hBPack32_[at32(s, slotB, l)] = 0.0f;
+                if (cl == TC - 1 && m[kBmNextC] >= 0) hBPack32_[at32(s, slotC, l)] = 0.0f;
+            }
+        }
+    }
 }
 
 }  // namespace mrfg

@@ -102,6 +102,7 @@ void toControls(const mrfg::SolverControls &s, mrfg_controls *c)
     c->check_interval = s.checkInterval;
     c->launch_mode = s.graphs ? MRFG_LAUNCH_GRAPH : MRFG_LAUNCH_KERNELS;
     c->tuning = s.autotune ? MRFG_TUNE_AUTO : MRFG_TUNE_FIXED;
+    c->inner_preconditioner = s.innerPre == mrfg::SolverControls::InnerDILU ? MRFG_INNER_DILU : MRFG_INNER_TILE_DILU;
 }
 
 mrfg::SolverControls fromControls(const mrfg_controls *c)
@@ -123,6 +124,10 @@ mrfg::SolverControls fromControls(const mrfg_controls *c)
     s.graphs = c->launch_mode == MRFG_LAUNCH_GRAPH;
     if (c->tuning != MRFG_TUNE_AUTO && c->tuning != MRFG_TUNE_FIXED) throw ArgError("unknown tuning mode");
     s.autotune = c->tuning == MRFG_TUNE_AUTO;
+    if (c->inner_preconditioner != MRFG_INNER_DILU && c->inner_preconditioner != MRFG_INNER_TILE_DILU)
+        throw ArgError("unknown inner preconditioner id");
+    s.innerPre = c->inner_preconditioner == MRFG_INNER_DILU ? mrfg::SolverControls::InnerDILU
+                                                            : mrfg::SolverControls::InnerTileDILU;
     if (s.maxIter < 0 || s.minIter < 0 || s.innerMaxIter < 1 || s.maxOuter < 1) throw ArgError("bad iteration limits");
     return s;
 }
@@ -884,6 +889,25 @@ int mrfg_precondition_fp32(mrfg_ctx *ctx, int sys, const float *r, float *w)
     return guard(ctx, [&] {
         systemOf(ctx, sys);
         committedBatch(ctx).precondition32(r, w, sys);
+    });
+}
+
+int mrfg_precondition_inner(mrfg_ctx *ctx, int sys, int inner_preconditioner, const float *r, float *w)
+{
+    if (!ctx || !r || !w) return MRFG_ERR_ARG;
+    if (inner_preconditioner != MRFG_INNER_TILE_DILU && inner_preconditioner != MRFG_INNER_DILU) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).precondition32(r, w, sys, inner_preconditioner == MRFG_INNER_TILE_DILU);
+    });
+}
+
+int mrfg_dilu_tiles(mrfg_ctx *ctx, int sys, int *tile_of_cell)
+{
+    if (!ctx || !tile_of_cell) return MRFG_ERR_ARG;
+    return guard(ctx, [&] {
+        systemOf(ctx, sys);
+        committedBatch(ctx).tilesOf(sys, tile_of_cell);
     });
 }
 

@@ -23,6 +23,7 @@
 // compiler then counts outstanding loads exactly instead of draining them),
 // and the fast path has no divergent branch.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "dilu.hpp"
 
@@ -321,6 +322,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kRawBufferWord3 = 0x00020000;   // 32-bit data format, no swizzle, no index stride
 constexpr int kSysCoherent = 16;              // sc1: the policy of the agent-scope relaxed atomics
 constexpr unsigned kNoLane = 0x80000000u;
+constexpr int kCmpUnordered = 8;              // the fcmp predicate "unordered": either operand is NaN
 
 // (the arguments are wave-uniform; readfirstlane states it, so the resource
 // lives in scalar registers)
@@ -329,7 +331,10 @@ __device__ inline __amdgpu_buffer_rsrc_t rawBuffer(const void *base, size_t byte
     const unsigned long long a = (unsigned long long)base;
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-    const int n = __builtin_amdgcn_readfirstlane((int)bytes);
+    int n = __builtin_amdgcn_readfirstlane((int)bytes);
+    // every resource keeps its own size register: two resources of equal size
+    // sharing one would be reassembled before each use
+    asm volatile("" : "+s"(n));
     return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), (short)0, n,
                                              kRawBufferWord3);
 }
@@ -378,7 +383,11 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     const int TB = m[kBmTB], lg = __builtin_ctz(TB), TC = 64 >> lg, TBC = TB + TC;
     const int nA = m[kBmNA], actB = m[kBmActB], actC = m[kBmActC];
     const int s0 = m[kBmSlice], nSteps = m[kBmSteps], base = m[kBmInbox];
-    const int prevB = m[kBmPrevB], prevC = m[kBmPrevC], nextB = m[kBmNextB], nextC = m[kBmNextC];
+    // a LONE sweep treats every tile as alone: a tile that has neighbours
+    // then applies the DILU of its own rows only (the tile-local
+    // preconditioner, whose records hold no coefficient across tile edges)
+    const int prevB = LONE ? -1 : m[kBmPrevB], prevC = LONE ? -1 : m[kBmPrevC];
+    const int nextB = LONE ? -1 : m[kBmNextB], nextC = LONE ? -1 : m[kBmNextC];
     const int bl = lane & (TB - 1), cl = lane >> lg;
     const bool inTile = bl < actB && cl < actC;
     // where each dependency comes from: the lane itself (sequential axis),
@@ -400,16 +409,18 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // the wave's resources: its slices of the records, of in / out (factor:
     // of rD and of the two record arrays), its own inbox and its consumers'
     const int rowBytes = G * 64 * (int)sizeof(R);
-    const auto rRec = rawBuffer(recs + (size_t)s0 * G * 64, (size_t)nSteps * rowBytes);
+    const int inRowBytes = TBC * (int)sizeof(U);
+    const R *myRecs = recs + (size_t)s0 * G * 64;
+    const auto rRec = rawBuffer(myRecs, (size_t)nSteps * rowBytes);
     const size_t rowsBytes = (size_t)nSteps * 64 * sizeof(T);
     const auto rIn = MODE == kDiluFactor ? rawBuffer(nullptr, 0) : rawBuffer(in + (size_t)s0 * 64, rowsBytes);
     const auto rOut = MODE == kDiluFactor ? rawBuffer(nullptr, 0) : rawBuffer(out + (size_t)s0 * 64, rowsBytes);
     const U *myIn = inbox + base;
-    const auto rInbox = rawBuffer(myIn, LONE ? 0 : (size_t)nSteps * TBC * sizeof(U));
+    const auto rInbox = rawBuffer(myIn, LONE ? 0 : (size_t)nSteps * inRowBytes);
     auto consumer = [&](int to) {
         if (LONE || to < 0) return rawBuffer(inbox, 0);
         const int32_t *mt = V.boxMeta + (size_t)to * kBoxMeta;
-        return rawBuffer(inbox + mt[kBmInbox], (size_t)mt[kBmSteps] * TBC * sizeof(U));
+        return rawBuffer(inbox + mt[kBmInbox], (size_t)mt[kBmSteps] * inRowBytes);
     };
     const auto rToB = consumer(toB), rToC = consumer(toC);
     const unsigned vRec = lane * (unsigned)sizeof(R), vRow = lane * (unsigned)sizeof(T);
@@ -427,16 +438,47 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
         return kFwd ? i : nSteps - 1 - i;
     };
     auto load = [&](int i, Step &S) {
-        const unsigned so = (unsigned)(stepOf(i) * rowBytes);
+        const int j = stepOf(i);
+        const unsigned so = (unsigned)(j * rowBytes);
 #pragma unroll
         for (int g = 0; g < G2; ++g) S.v[g] = bufLoad<R, 0>(rRec, vRec + g * 64 * sizeof(R), so);
         if (kExtra) S.v[NL - 1] = bufLoad<R, 0>(rRec, vRec + gLast * 64 * sizeof(R), so);
-        if (MODE != kDiluFactor) S.a = bufLoad<T, 0>(rIn, vRow, (unsigned)(stepOf(i) * 64 * sizeof(T)));
+        if (MODE != kDiluFactor) S.a = bufLoad<T, 0>(rIn, vRow, (unsigned)(j * 64 * sizeof(T)));
     };
     auto loadIn = [&](int i, U &hb, U &hc) {
-        const unsigned so = (unsigned)(stepOf(i) * TBC * sizeof(U));
+        const unsigned so = (unsigned)(stepOf(i) * inRowBytes);
         hb = bufLoad<U, kSysCoherent>(rInbox, vInB, so);
         hc = bufLoad<U, kSysCoherent>(rInbox, vInC, so);
+    };
+    // A FULL group (below) addresses every stream of step t through one scalar
+    // offset per array and per-lane constants that hold the rest: the records
+    // and input of step t + dA and the output of step t share the offset of
+    // step min(t, t + dA); the inbox row of step t + dI and the consumers'
+    // rows of steps t + dB and t + dC share that of step t + mOff, the lowest
+    // of them.  The lane constants are non-negative and every address stays
+    // inside its array (a FULL group only addresses steps inside the sweep).
+    const int dA = kFwd ? kBoxAhead : -kBoxAhead, dI = kFwd ? kBoxInAhead : -kBoxInAhead;
+    const int dBe = toB >= 0 ? dB : 0, dCe = toC >= 0 ? dC : 0;
+    const int mOff = min(min(dI, 0), min(dBe, dCe));
+    const unsigned vRecF = vRec + (kFwd ? (unsigned)(dA * rowBytes) : 0u);
+    const unsigned vInF = vRow + (kFwd ? (unsigned)(dA * 64 * sizeof(T)) : 0u);
+    const unsigned vOutF = vRow + (kFwd ? 0u : (unsigned)(-dA * 64 * sizeof(T)));
+    const unsigned vInBF = vInB + (unsigned)((dI - mOff) * inRowBytes);
+    const unsigned vInCF = vInC + (unsigned)((dI - mOff) * inRowBytes);
+    const unsigned vToBF = vToB == kNoLane || !inTile ? kNoLane : vToB + (unsigned)((dB - mOff) * inRowBytes);
+    const unsigned vToCF = vToC == kNoLane || !inTile ? kNoLane : vToC + (unsigned)((dC - mOff) * inRowBytes);
+    auto loadF = [&](int t, Step &S) {
+        const int tm = kFwd ? t : t + dA;
+        const unsigned so = (unsigned)(tm * rowBytes);
+#pragma unroll
+        for (int g = 0; g < G2; ++g) S.v[g] = bufLoad<R, 0>(rRec, vRecF + g * 64 * sizeof(R), so);
+        if (kExtra) S.v[NL - 1] = bufLoad<R, 0>(rRec, vRecF + gLast * 64 * sizeof(R), so);
+        if (MODE != kDiluFactor) S.a = bufLoad<T, 0>(rIn, vInF, (unsigned)(tm * 64 * sizeof(T)));
+    };
+    auto loadInF = [&](int t, U &hb, U &hc) {
+        const unsigned so = (unsigned)((t + mOff) * inRowBytes);
+        hb = bufLoad<U, kSysCoherent>(rInbox, vInBF, so);
+        hc = bufLoad<U, kSysCoherent>(rInbox, vInCF, so);
     };
 
     // the exchange values start finite: a missing neighbour's zero
@@ -459,24 +501,33 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
         __builtin_amdgcn_sched_barrier(0);
     }
     T yprev = T(0);
-    // the loop runs whole groups of steps: a step past the end is invalid for
-    // every lane, reads clamped addresses and stores nothing, so the only
-    // conditional code is the wave-uniform slow path and the output store.
-    // The body is unrolled over kBoxRounds rounds of the prefetch ring, every
-    // record wait exact (the initial loads above are issued in order for
-    // that); 16 steps measured fastest (8 and 32 were slower)
-    for (int i0 = 0; i0 < nSteps; i0 += kBoxRounds * kBoxAhead) {
+    // The loop runs whole groups of steps: a step past the end is invalid for
+    // every lane, reads clamped addresses and stores nothing.  The body is
+    // unrolled over kBoxRounds rounds of the prefetch ring, every record wait
+    // exact (the initial loads above are issued in order for that); 16 steps
+    // measured fastest (8 and 32 were slower).
+    //
+    // A group is FULL when every in-tile row of its steps is valid and its
+    // prefetches stay inside the sweep: there the validity mask is the
+    // loop-invariant tile mask, the output store is unconditional, no
+    // prefetch index is clamped, and every delivery offset lies inside its
+    // consumer's inbox, so a step spends no instructions on those tests.
+    // For the fluid block of the duct that is 9 of every 11 groups.
+    constexpr int kGroup = kBoxRounds * kBoxAhead;
+    const int tLo = (actB - 1) + (actC - 1);   // first step at which every in-tile row is valid
+    auto runGroup = [&](int i0, auto fullTag) {
+        constexpr bool FULL = decltype(fullTag)::value;
 #pragma unroll
-        for (int uu = 0; uu < kBoxRounds * kBoxAhead; ++uu) {
+        for (int uu = 0; uu < kGroup; ++uu) {
             const int u = uu % kBoxAhead;
             const int i = i0 + uu;
-            const bool inRange = i < nSteps;
+            const bool inRange = FULL || i < nSteps;
             Step &S = buf[u];
             U &hb = hbuf[u];
             U &hc = cbuf[u];
             const int t = kFwd ? i : nSteps - 1 - i;
             const int ia = t - bl - cl;
-            const bool valid = inTile && (unsigned)ia < (unsigned)nA;
+            const bool valid = FULL ? inTile : inTile && (unsigned)ia < (unsigned)nA;
             const bool hasA = kFwd ? ia >= 1 : ia + 1 < nA;
             const bool needB = valid & useB, needC = valid & useC;
             T xb = xbN, xc = xcN;
@@ -516,40 +567,52 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             };
             // the value is computed with the prefetched inbox values first and
             // the exchange started; whether one of them had not arrived is
-            // tested after that, off the step's dependency chain
-            T y = rowValue(hb, hc);
-            T yz = valid ? y : T(0);
-            // all lanes have read the previous step's values (LDS operations
-            // of a wavefront execute in order) before any overwrites them
-            xch[lane] = yz;
-            __builtin_amdgcn_wave_barrier();
-            xbN = xch[srcB];
-            xcN = xch[srcC];
-            const bool miss = !LONE && ((needB & (hb == kSent)) | (needC & (hc == kSent)));
-            if (!LONE && __builtin_expect(__builtin_amdgcn_ballot_w64(miss) != 0, 0)) {
-                // rare slow path (wave-uniform): wait for the late values,
-                // reload the inbox values already requested for later steps,
-                // and redo the step: the row value, and the exchange (read
-                // again, as it may have taken the provisional values)
-                ++nSlow;
-                const U *ib = myIn + (size_t)t * TBC;
-                if (needB && hb == kSent) hb = haloWait(ib, cl, kSent, V.err);
-                if (needC && hc == kSent) hc = haloWait(ib, TC + bl, kSent, V.err);
-#pragma unroll
-                for (int d = 1; d < kBoxInAhead; ++d) {
-                    const U *fb = myIn + (size_t)stepOf(i + d) * TBC;
-                    U &vb = hbuf[(u + d) % kBoxAhead];
-                    U &vc = cbuf[(u + d) % kBoxAhead];
-                    if (vb == kSent) vb = haloLoad(fb, cl);
-                    if (vc == kSent) vc = haloLoad(fb, TC + bl);
-                }
-                __builtin_amdgcn_s_waitcnt(0);
+            // tested after that, off the step's dependency chain.  The inbox
+            // sentinel is a NaN, and a NaN operand makes the row's value NaN,
+            // so one test of the value covers both inbox values.  A value that
+            // is NaN for another reason (a diverged system) only takes the
+            // slow path, which then waits for nothing.
+            // The step runs twice at most: a second pass follows a wait for
+            // the late values (wave-uniform), which also reloads the inbox
+            // values already requested for later steps.  The second pass
+            // redoes the row value and the exchange (read again, as it may
+            // have taken the provisional values).
+            T y, yz;
+            for (int pass = 0;; ++pass) {
                 y = rowValue(hb, hc);
                 yz = valid ? y : T(0);
+                // all lanes have read the previous step's values (LDS
+                // operations of a wavefront execute in order) before any
+                // overwrites them
                 xch[lane] = yz;
                 __builtin_amdgcn_wave_barrier();
                 xbN = xch[srcB];
                 xcN = xch[srcC];
+                // the reads stay here, so their latency overlaps the step's
+                // stores and prefetches
+                asm volatile("" ::: "memory");
+                // (invalid rows hold an exact zero, so the test needs no mask)
+                if (LONE || pass || __builtin_expect(__builtin_amdgcn_ballot_w64(yz != yz) == 0, 1)) break;
+                ++nSlow;
+                // (the slow path's addresses are computed here, not hoisted
+                // into every step)
+                int is = i;
+                asm volatile("" : "+s"(is));
+                const U *ib = myIn + (size_t)(kFwd ? is : nSteps - 1 - is) * TBC;
+                if (needB && hb == kSent) hb = haloWait(ib, cl, kSent, V.err);
+                if (needC && hc == kSent) hc = haloWait(ib, TC + bl, kSent, V.err);
+#pragma unroll
+                for (int d = 1; d < kBoxInAhead; ++d) {
+                    const U *fb = myIn + (size_t)stepOf(is + d) * TBC;
+                    U &vb = hbuf[(u + d) % kBoxAhead];
+                    U &vc = cbuf[(u + d) % kBoxAhead];
+                    // every lane loads and selects (no divergent code), so
+                    // the ring keeps its registers
+                    const U nb = haloLoad(fb, cl), nc = haloLoad(fb, TC + bl);
+                    vb = vb == kSent ? nb : vb;
+                    vc = vc == kSent ? nc : vc;
+                }
+                __builtin_amdgcn_s_waitcnt(0);
             }
             yprev = yz;
             if (inRange) {
@@ -557,11 +620,15 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 if (MODE == kDiluFactor) {
                     const double rd = valid ? 1.0 / (double)y : 0.0;
                     (V.rDout + (size_t)(s0 + t) * 64)[lane] = rd;
-                    constexpr int NP = Pack<double>::N;
-                    const int GP = packChunks(K + 1, NP);
-                    const int s = s0 + t;
-                    ((double *)&V.fPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
-                    ((double *)&V.bPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                    if (V.fPack) {   // (a tile-local factor writes rD only)
+                        constexpr int NP = Pack<double>::N;
+                        const int GP = packChunks(K + 1, NP);
+                        const int s = s0 + t;
+                        ((double *)&V.fPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                        ((double *)&V.bPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                    }
+                } else if (FULL) {
+                    bufStore<T, 0>(yz, rOut, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
                 } else {
                     bufStore<T, 0>(yz, rOut, vRow, so);
                 }
@@ -571,23 +638,41 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 // a consumer (not on the edge, or an invalid row) at kNoLane.
                 // A valid row's consumer step is valid too, so its offset lies
                 // inside the consumer's inbox; an invalid row's step offset is
-                // only kept non-negative.
+                // only kept non-negative.  In a FULL group every in-tile row is
+                // valid and every step offset lies inside the consumer's inbox.
                 const U yb = Bits<T>::to(y);
-                const int tb = t + dB, tc = t + dC;
-                bufStore<U, kSysCoherent>(yb, rToB, valid ? vToB : kNoLane,
-                                          (unsigned)((tb > 0 ? tb : 0) * TBC * sizeof(U)));
-                bufStore<U, kSysCoherent>(yb, rToC, valid ? vToC : kNoLane,
-                                          (unsigned)((tc > 0 ? tc : 0) * TBC * sizeof(U)));
+                if (FULL) {
+                    const unsigned so = (unsigned)((t + mOff) * inRowBytes);
+                    bufStore<U, kSysCoherent>(yb, rToB, vToBF, so);
+                    bufStore<U, kSysCoherent>(yb, rToC, vToCF, so);
+                } else {
+                    const int tb = t + dB, tc = t + dC;
+                    bufStore<U, kSysCoherent>(yb, rToB, valid ? vToB : kNoLane,
+                                              (unsigned)((tb > 0 ? tb : 0) * inRowBytes));
+                    bufStore<U, kSysCoherent>(yb, rToC, valid ? vToC : kNoLane,
+                                              (unsigned)((tc > 0 ? tc : 0) * inRowBytes));
+                }
             }
             // the inbox values of a later step first, so that waiting for them
             // leaves the record loads issued after them in flight
-            if (!LONE)
-                loadIn(i + kBoxInAhead, hbuf[(u + kBoxInAhead) % kBoxAhead], cbuf[(u + kBoxInAhead) % kBoxAhead]);
-            load(i + kBoxAhead, S);
+            U &hbF = hbuf[(u + kBoxInAhead) % kBoxAhead], &hcF = cbuf[(u + kBoxInAhead) % kBoxAhead];
+            if (FULL) {
+                if (!LONE) loadInF(t, hbF, hcF);
+                loadF(t, S);
+            } else {
+                if (!LONE) loadIn(i + kBoxInAhead, hbF, hcF);
+                load(i + kBoxAhead, S);
+            }
             // keep each step's prefetches where they are (not sunk towards
             // their use)
             __builtin_amdgcn_sched_barrier(0);
         }
+    };
+    for (int i0 = 0; i0 < nSteps; i0 += kGroup) {
+        const int tMin = kFwd ? i0 : nSteps - kGroup - i0, tMax = kFwd ? i0 + kGroup - 1 : nSteps - 1 - i0;
+        const bool full = tMin >= tLo && tMax <= nA - 1 && i0 + kGroup - 1 + kBoxAhead < nSteps;
+        if (full) runGroup(i0, std::true_type{});
+        else runGroup(i0, std::false_type{});
     }
 }
 
@@ -740,6 +825,18 @@ hipError_t diluFactor(const DiluView<double> &V, hipStream_t s)
                                                 nullptr, 0, s);
     if (e != hipSuccess) return e;
     return launchBox<double, kDiluFactor>(V, nullptr, nullptr, nullptr, (unsigned long long *)V.boxInC, nullptr, 0,
+                                          s);
+}
+
+hipError_t diluFactorTileLocal(const DiluView<double> &V, double *rD, hipStream_t s)
+{
+    if (V.nBoxWaves <= 0) return hipSuccess;
+    DiluView<double> L = V;
+    L.rDout = rD;
+    L.fPack = nullptr;
+    L.bPack = nullptr;
+    L.boxLone = true;
+    return launchBox<double, kDiluFactor>(L, nullptr, nullptr, nullptr, (unsigned long long *)V.boxInC, nullptr, 0,
                                           s);
 }
 

@@ -99,6 +99,11 @@ hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipS
 // Factorisation: rD = 1/d for every row of the batch (FP64), written to
 // rDout and into the forward and backward records.
 hipError_t diluFactor(const DiluView<double> &V, hipStream_t s);
+// The tile-local factor: rD of every box tile's rows with the couplings across
+// tile edges left out (each tile factorised alone), written to rD only (rows
+// of general waves are left as they are).  FP64, from the FP64 product
+// records.
+hipError_t diluFactorTileLocal(const DiluView<double> &V, double *rD, hipStream_t s);
 // rD of the FP32 records from the FP64 factor
 hipError_t diluConvertRD(const DiluView<float> &V32, const double *rD64, int nSlices, hipStream_t s);
 hipError_t fillHalo64(void *h, int n, hipStream_t s);

@@ -67,12 +67,19 @@ def test_parse_controls_reference_syntax():
 def test_parse_controls_gpu_keys():
     c = m.controls(solver="hipBiCGStab", tolerance=1e-10, precision="mixed", innerRelTol=1e-3, maxOuter=7)
     assert c.precision == 1 and c.inner_rel_tol == 1e-3 and c.max_outer == 7
+    # the inner preconditioner: tile-local DILU by default (also the zero value)
+    assert c.inner_preconditioner == m.MRFG_INNER_TILE_DILU == 0
+    d = m.controls(precision="mixed", innerPreconditioner="DILU")
+    assert d.inner_preconditioner == m.MRFG_INNER_DILU
+    e = m.parse_controls("solvers { T { solver BiCGStab; precision mixed; } }", "T")
+    assert e.inner_preconditioner == m.MRFG_INNER_TILE_DILU
 
 
 @pytest.mark.parametrize("bad,msg", [
     ("solvers { T { solver GAMG; } }", "Unknown coupled solver"),
     ("solvers { T { solver BiCGStab; preconditioner { preconditioner FDIC; } } }", "Unknown coupled preconditioner"),
     ("solvers { T { solver BiCGStab; precision half; } }", "Unknown precision"),
+    ("solvers { T { solver BiCGStab; refinement { innerPreconditioner ILU; } } }", "Unknown innerPreconditioner"),
     ("solvers { T { solver BiCGStab tolerance 1e-6; }", "unterminated"),
     ("solvers { }", "undefined"),
 ])

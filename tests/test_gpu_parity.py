@@ -128,17 +128,75 @@ def test_dilu_fp32_records(gpu, name):
         A.solve(tolerance=1e-8, maxIter=200)
 
 
+@pytest.mark.parametrize("inner", ["tileDILU", "DILU"])
 @pytest.mark.parametrize("name", ["plate2d", "duct_ggi", "exchanger3", "duct_tiles", "duct_zlong"])
-def test_mixed_precision_refinement(gpu, name):
+def test_mixed_precision_refinement(gpu, name, inner):
     case = small_cases()[name]
     O = ob.OracleSystem(case)
     psi_ref, perf_ref = O.solve(tol=1e-12, maxIter=2000)
     A = _ctx(gpu, case)
     perf = A.solve(solver="BiCGStab", tolerance=1e-10, maxIter=1000, precision="mixed",
-                   innerRelTol=1e-4)[0]
+                   innerRelTol=1e-4, innerPreconditioner=inner)[0]
     assert perf.converged and perf.n_outer >= 1
     err = rel(np.concatenate(A.psi(0)), np.concatenate(psi_ref))
     assert err <= MIXED_TOL, err
+
+
+def _masked_dilu(O, keep_of_region, r):
+    """DILU (calcReciprocalD / precondition order) of the oracle's regions
+    with the faces of keep_of_region[g] == False left out, in float64."""
+    out = []
+    for g, R in enumerate(O.regions):
+        lo_, up_, l_, u_ = R["lower"], R["upper"], R["lAddr"], R["uAddr"]
+        keep = keep_of_region[g]
+        d = np.array(R["diag"], dtype=np.float64)
+        for f in range(len(l_)):
+            if keep[f]:
+                d[u_[f]] -= up_[f] * lo_[f] / d[l_[f]]
+        rD = 1.0 / d
+        w = rD * r[g]
+        for f in range(len(l_)):
+            if keep[f]:
+                w[u_[f]] -= rD[u_[f]] * lo_[f] * w[l_[f]]
+        for f in range(len(l_) - 1, -1, -1):
+            if keep[f]:
+                w[l_[f]] -= rD[l_[f]] * up_[f] * w[u_[f]]
+        out.append(w)
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("name", ["duct_tiles", "duct_zlong", "exchanger3", "plate2d"])
+def test_tile_local_preconditioner(gpu, name):
+    """The mixed solver's FP32 tile-local DILU equals, within FP32 rounding,
+    the DILU of the same matrix with every face between two different box
+    tiles left out (restated here from the DILU recurrence); cells on general
+    waves keep all their faces.  Its "DILU" mode is the FP32 full DILU."""
+    case = small_cases()[name]
+    O = ob.OracleSystem(case)
+    A = _ctx(gpu, case)
+    tiles = A.dilu_tiles()
+    off = O.offsets()
+    keep, cut = [], 0
+    for g, R in enumerate(O.regions):
+        t = tiles[off[g]:off[g + 1]]
+        tl, tu = t[R["lAddr"]], t[R["uAddr"]]
+        k = ~((tl >= 0) & (tu >= 0) & (tl != tu))
+        keep.append(k)
+        cut += int((~k).sum())
+    rng = np.random.default_rng(11)
+    for rep in range(2):
+        r = [rng.standard_normal(n["nCells"]) for n in O.regions]
+        w_ref = _masked_dilu(O, keep, r)
+        w = A.precondition_inner(np.concatenate(r), "tileDILU").astype(np.float64)
+        assert np.all(np.isfinite(w))
+        assert rel(w, w_ref) < 1e-5, rel(w, w_ref)
+        w_full = A.precondition_inner(np.concatenate(r), "DILU")
+        assert np.array_equal(w_full, A.precondition_fp32(np.concatenate(r).astype(np.float32)))
+        if cut:
+            # the faces left out matter: the full DILU differs
+            w_all = _masked_dilu(O, [np.ones(len(R["lAddr"]), bool) for R in O.regions], r)
+            assert rel(w_all, w_ref) > 1e-3
+        A.solve(tolerance=1e-8, maxIter=200, precision="mixed")
 
 
 @pytest.mark.parametrize("name,balance", [("plate2d", 1e-6), ("duct_conformal", 1e-6),
