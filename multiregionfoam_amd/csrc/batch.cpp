@@ -669,7 +669,9 @@ DiluView<T> Batch::diluView(DevMatrix<T> &m, void *hF, void *hB)
     D.boxOrder = boxOrder_.p;
     D.nBoxWaves = nBoxWaves_;
     // the FP32 records of the tile-local DILU are swept tile by tile
-    D.boxLone = boxLone_ || ((const void *)&m == (const void *)&m32_ && tileLocal32_);
+    const bool local = (const void *)&m == (const void *)&m32_ && tileLocal32_;
+    D.boxLone = boxLone_ || local;
+    D.boxSegs = local ? tileSegments() : 1;
     D.boxMeta = boxMeta_.p;
     D.nBoxIn = nBoxIn_;
     D.fCodes = fCodes_.p;
@@ -796,7 +798,7 @@ void Batch::factorise()
             rdLocal_.alloc(nPos_);
             HIP_OK(hipMemcpyAsync(rdLocal_.p, v64_[11].p, (size_t)nPos_ * sizeof(double), hipMemcpyDeviceToDevice,
                                   stream_));
-            HIP_OK(diluFactorTileLocal(D, rdLocal_.p, stream_));
+            HIP_OK(diluFactorTileLocal(D, rdLocal_.p, tileSegments(), stream_));
             rd = rdLocal_.p;
         }
         HIP_OK(diluConvertRD(D32, rd, nSlices_, stream_));
@@ -1417,9 +1419,23 @@ void Batch::tilesOf(int sys, int *tileOfCell) const
     std::vector<int> waveOfSlice(L.nSlices, -1);
     for (int w = 0; w < L.nWaves; ++w)
         for (int t = 0; t < L.waveSteps[w]; ++t) waveOfSlice[L.waveSlice[w] + t] = w;
+    const int segs = tileSegments();
     for (long long c = 0; c < L.nCells; ++c) {
-        const int w = waveOfSlice[L.posOfCell[c] / kSlice];
-        tileOfCell[c] = L.waveBox[w] ? w0 + w : -1;
+        const int p = L.posOfCell[c], w = waveOfSlice[p / kSlice];
+        if (!L.waveBox[w]) {
+            tileOfCell[c] = -1;
+            continue;
+        }
+        // the segment of the cell's row along the tile's sequential axis
+        const int32_t *m = &L.boxMeta[(size_t)w * kBoxMeta];
+        const int TB = m[kBmTB], lane = p % kSlice, t = p / kSlice - L.waveSlice[w];
+        const int ia = t - lane % TB - lane / TB;
+        int seg = 0, lo, hi;
+        for (; seg < segs - 1; ++seg) {
+            boxSegment(m[kBmNA], seg, segs, lo, hi);
+            if (ia < hi) break;
+        }
+        tileOfCell[c] = (w0 + w) * segs + seg;
     }
 }
 

@@ -560,6 +560,21 @@ void Batch::derivePack32(bool tileLocal)
                 if (cl == TC - 1 && m[kBmNextC] >= 0) hBPack32_[at32(s, slotC, l)] = 0.0f;
             }
         }
+        // the cuts between the tile's segments along its sequential axis:
+        // a segment's first row drops its lower neighbour (slot 2 - axis),
+        // the previous segment's last row its upper neighbour
+        const int segs = tileSegments(), slotA = 2 - ax;
+        for (int sg = 1; sg < segs; ++sg) {
+            int lo, hi;
+            boxSegment(m[kBmNA], sg, segs, lo, hi);
+            if (lo <= 0 || lo >= m[kBmNA]) continue;
+            for (int cl = 0; cl < m[kBmActC]; ++cl)
+                for (int bl = 0; bl < m[kBmActB]; ++bl) {
+                    const int l = bl + TB * cl;
+                    hFPack32_[at32(m[kBmSlice] + lo + bl + cl, slotA, l)] = 0.0f;
+                    hBPack32_[at32(m[kBmSlice] + lo - 1 + bl + cl, slotA, l)] = 0.0f;
+                }
+        }
     }
 }
 

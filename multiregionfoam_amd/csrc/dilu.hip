@@ -359,9 +359,12 @@ __device__ inline void bufStore(X v, __amdgpu_buffer_rsrc_t r, unsigned voff, un
 // rD in a chunk of its own).
 // LONE: the tile has no neighbouring tile (alone in its region), so it has
 // no inbox to read and no values to deliver.
+// seg, segs: a lone sweep of segment seg of segs of the tile's rows (rows
+// of the other segments are not this wave's: it neither uses nor writes them)
 template <typename T, int KT, int MODE, int AX, bool LONE>
 __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m, const T *__restrict__ in, T *out,
-                                         typename Bits<T>::U *inbox, T *xch, unsigned long long &nSlow)
+                                         typename Bits<T>::U *inbox, T *xch, unsigned long long &nSlow, int seg,
+                                         int segs)
 {
     using U = typename Bits<T>::U;
     using R = typename RecordOf<T, MODE>::V;
@@ -390,6 +393,18 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     const int nextB = LONE ? -1 : m[kBmNextB], nextC = LONE ? -1 : m[kBmNextC];
     const int bl = lane & (TB - 1), cl = lane >> lg;
     const bool inTile = bl < actB && cl < actC;
+    // this wave's rows [iaLo, iaHi) along the sequential axis and its steps
+    // tFirst..tLast (the whole tile: 0..nSteps-1); a row at step t is
+    // ia = t - bl - cl
+    int iaLo = 0, iaHi = nA;
+    if (LONE && segs > 1) boxSegment(nA, seg, segs, iaLo, iaHi);
+    const int tLo = (actB - 1) + (actC - 1);   // rows valid from step iaLo + tLo on for every lane
+    const int tFirst = iaLo, tLast = iaHi - 1 + tLo, nLocal = tLast - tFirst + 1;
+    const bool cut = LONE && segs > 1;
+    if (nLocal <= 0) return;   // a segment without rows (nA < segs)
+    // a lane's row at step t that belongs to no segment (padding) or to this
+    // one: the only rows a segment wave writes
+    auto mine = [&](int ia) { return !inTile || ia < 0 || ia >= nA || (ia >= iaLo && ia < iaHi); };
     // where each dependency comes from: the lane itself (sequential axis),
     // a neighbouring lane, or the inbox on the tile's edge
     const bool edgeB = kFwd ? bl == 0 : bl == TB - 1;
@@ -434,8 +449,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     };
     // step i of the sweep (clamped: loads past the end re-read the last step)
     auto stepOf = [&](int i) {
-        i = i < nSteps ? i : nSteps - 1;
-        return kFwd ? i : nSteps - 1 - i;
+        i = i < nLocal ? i : nLocal - 1;
+        return kFwd ? tFirst + i : tLast - i;
     };
     auto load = [&](int i, Step &S) {
         const int j = stepOf(i);
@@ -514,21 +529,20 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // consumer's inbox, so a step spends no instructions on those tests.
     // For the fluid block of the duct that is 9 of every 11 groups.
     constexpr int kGroup = kBoxRounds * kBoxAhead;
-    const int tLo = (actB - 1) + (actC - 1);   // first step at which every in-tile row is valid
     auto runGroup = [&](int i0, auto fullTag) {
         constexpr bool FULL = decltype(fullTag)::value;
 #pragma unroll
         for (int uu = 0; uu < kGroup; ++uu) {
             const int u = uu % kBoxAhead;
             const int i = i0 + uu;
-            const bool inRange = FULL || i < nSteps;
+            const bool inRange = FULL || i < nLocal;
             Step &S = buf[u];
             U &hb = hbuf[u];
             U &hc = cbuf[u];
-            const int t = kFwd ? i : nSteps - 1 - i;
+            const int t = kFwd ? tFirst + i : tLast - i;
             const int ia = t - bl - cl;
-            const bool valid = FULL ? inTile : inTile && (unsigned)ia < (unsigned)nA;
-            const bool hasA = kFwd ? ia >= 1 : ia + 1 < nA;
+            const bool valid = FULL ? inTile : inTile && (unsigned)(ia - iaLo) < (unsigned)(iaHi - iaLo);
+            const bool hasA = kFwd ? ia > iaLo : ia + 1 < iaHi;
             const bool needB = valid & useB, needC = valid & useC;
             T xb = xbN, xc = xcN;
             // both exchange reads are issued by every lane (not sunk into the
@@ -598,7 +612,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 // into every step)
                 int is = i;
                 asm volatile("" : "+s"(is));
-                const U *ib = myIn + (size_t)(kFwd ? is : nSteps - 1 - is) * TBC;
+                const U *ib = myIn + (size_t)(kFwd ? tFirst + is : tLast - is) * TBC;
                 if (needB && hb == kSent) hb = haloWait(ib, cl, kSent, V.err);
                 if (needC && hc == kSent) hc = haloWait(ib, TC + bl, kSent, V.err);
 #pragma unroll
@@ -619,7 +633,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 const unsigned so = (unsigned)t * 64 * sizeof(T);
                 if (MODE == kDiluFactor) {
                     const double rd = valid ? 1.0 / (double)y : 0.0;
-                    (V.rDout + (size_t)(s0 + t) * 64)[lane] = rd;
+                    if (FULL || !cut || mine(ia)) (V.rDout + (size_t)(s0 + t) * 64)[lane] = rd;
                     if (V.fPack) {   // (a tile-local factor writes rD only)
                         constexpr int NP = Pack<double>::N;
                         const int GP = packChunks(K + 1, NP);
@@ -630,7 +644,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 } else if (FULL) {
                     bufStore<T, 0>(yz, rOut, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
                 } else {
-                    bufStore<T, 0>(yz, rOut, vRow, so);
+                    bufStore<T, 0>(yz, rOut, !cut || mine(ia) ? vRow : kNoLane, so);
                 }
             }
             if (!LONE) {
@@ -668,9 +682,9 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             __builtin_amdgcn_sched_barrier(0);
         }
     };
-    for (int i0 = 0; i0 < nSteps; i0 += kGroup) {
-        const int tMin = kFwd ? i0 : nSteps - kGroup - i0, tMax = kFwd ? i0 + kGroup - 1 : nSteps - 1 - i0;
-        const bool full = tMin >= tLo && tMax <= nA - 1 && i0 + kGroup - 1 + kBoxAhead < nSteps;
+    for (int i0 = 0; i0 < nLocal; i0 += kGroup) {
+        const int tMin = kFwd ? tFirst + i0 : tLast - (i0 + kGroup - 1), tMax = kFwd ? tFirst + i0 + kGroup - 1 : tLast - i0;
+        const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kBoxAhead < nLocal;
         if (full) runGroup(i0, std::true_type{});
         else runGroup(i0, std::false_type{});
     }
@@ -684,19 +698,22 @@ __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *
                                                  int nClear)
 {
     __shared__ T xch[64];
-    const int w = V.boxOrder[MODE == kDiluBwd ? V.nBoxWaves - 1 - blockIdx.x : blockIdx.x];
+    // a lone launch may cut every tile into segs segments: segs blocks per tile
+    const int segs = LONE ? V.boxSegs : 1;
+    const int j = (int)blockIdx.x / segs, seg = (int)blockIdx.x - j * segs;
+    const int w = V.boxOrder[MODE == kDiluBwd ? V.nBoxWaves - 1 - j : j];
     const int32_t *m = V.boxMeta + (size_t)w * kBoxMeta;
     const bool active = MODE == kDiluFactor || st[m[kBmSys]].status == kRunning;
     const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
     unsigned long long nSlow = 0;
     if (active && m[kBmSteps] > 0) {
         switch (m[kBmAxis]) {
-        case 0: boxSweep<T, KT, MODE, 0, LONE>(V, m, in, out, inbox, xch, nSlow); break;
-        case 1: boxSweep<T, KT, MODE, 1, LONE>(V, m, in, out, inbox, xch, nSlow); break;
-        default: boxSweep<T, KT, MODE, 2, LONE>(V, m, in, out, inbox, xch, nSlow); break;
+        case 0: boxSweep<T, KT, MODE, 0, LONE>(V, m, in, out, inbox, xch, nSlow, seg, segs); break;
+        case 1: boxSweep<T, KT, MODE, 1, LONE>(V, m, in, out, inbox, xch, nSlow, seg, segs); break;
+        default: boxSweep<T, KT, MODE, 2, LONE>(V, m, in, out, inbox, xch, nSlow, seg, segs); break;
         }
     }
-    if (V.trace && threadIdx.x == 0) {
+    if (V.trace && threadIdx.x == 0 && segs == 1) {
         unsigned long long *tr = V.trace + 4 * ((size_t)V.nWaves + blockIdx.x);
         tr[0] = tStart;
         tr[1] = wall_clock64();
@@ -752,12 +769,14 @@ static hipError_t launchBox(const DiluView<T> &V, const SysState *st, const T *i
     if (V.nBoxWaves <= 0) return hipSuccess;
     const int n = V.nBoxWaves;
     if (V.boxLone) {
-        if (V.K == 3) k_dilu_box<T, 3, MODE, true><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
-        else k_dilu_box<T, 0, MODE, true><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
-    } else {
-        if (V.K == 3) k_dilu_box<T, 3, MODE, false><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
-        else k_dilu_box<T, 0, MODE, false><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        const int nb = n * (V.boxSegs > 1 ? V.boxSegs : 1);
+        if (V.K == 3) k_dilu_box<T, 3, MODE, true><<<nb, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        else k_dilu_box<T, 0, MODE, true><<<nb, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+        return hipGetLastError();
     }
+    if (V.boxSegs > 1) return hipErrorInvalidValue;   // segments only in lone sweeps
+    if (V.K == 3) k_dilu_box<T, 3, MODE, false><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
+    else k_dilu_box<T, 0, MODE, false><<<n, 64, 0, s>>>(V, st, in, out, use, clr, nClr);
     return hipGetLastError();
 }
 
@@ -828,14 +847,17 @@ hipError_t diluFactor(const DiluView<double> &V, hipStream_t s)
                                           s);
 }
 
-hipError_t diluFactorTileLocal(const DiluView<double> &V, double *rD, hipStream_t s)
+hipError_t diluFactorTileLocal(const DiluView<double> &V, double *rD, int segs, hipStream_t s)
 {
     if (V.nBoxWaves <= 0) return hipSuccess;
+    if (segs < 1) return hipErrorInvalidValue;
     DiluView<double> L = V;
     L.rDout = rD;
     L.fPack = nullptr;
     L.bPack = nullptr;
     L.boxLone = true;
+    L.boxSegs = segs;
+    L.trace = nullptr;
     return launchBox<double, kDiluFactor>(L, nullptr, nullptr, nullptr, (unsigned long long *)V.boxInC, nullptr, 0,
                                           s);
 }

@@ -76,6 +76,11 @@ struct DiluView {
     const int32_t *boxOrder = nullptr;
     int nBoxWaves = 0;
     bool boxLone = false;                // every box tile is alone in its region
+    // lone sweeps only (the tile-local preconditioner): each tile's rows are
+    // cut along the sequential axis into boxSegs segments of nA / boxSegs
+    // rows, each swept by a wave of its own with the coupling across the cut
+    // left out
+    int boxSegs = 1;
     const int32_t *boxMeta = nullptr;
     void *boxInF = nullptr, *boxInB = nullptr, *boxInC = nullptr;
     int nBoxIn = 0;
@@ -100,10 +105,17 @@ hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipS
 // rDout and into the forward and backward records.
 hipError_t diluFactor(const DiluView<double> &V, hipStream_t s);
 // The tile-local factor: rD of every box tile's rows with the couplings across
-// tile edges left out (each tile factorised alone), written to rD only (rows
-// of general waves are left as they are).  FP64, from the FP64 product
-// records.
-hipError_t diluFactorTileLocal(const DiluView<double> &V, double *rD, hipStream_t s);
+// tile edges (and across the cuts between its segs segments) left out, each
+// tile segment factorised alone, written to rD only (rows of general waves
+// are left as they are).  FP64, from the FP64 product records.
+hipError_t diluFactorTileLocal(const DiluView<double> &V, double *rD, int segs, hipStream_t s);
+// Segment s of segs of a box tile's nA rows along its sequential axis: rows
+// [lo, hi) (shared by the host's record derivation and the device sweeps)
+__host__ __device__ inline void boxSegment(int nA, int s, int segs, int &lo, int &hi)
+{
+    lo = (int)((long long)s * nA / segs);
+    hi = (int)((long long)(s + 1) * nA / segs);
+}
 // rD of the FP32 records from the FP64 factor
 hipError_t diluConvertRD(const DiluView<float> &V32, const double *rD64, int nSlices, hipStream_t s);
 hipError_t fillHalo64(void *h, int n, hipStream_t s);

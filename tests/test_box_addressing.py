@@ -90,6 +90,48 @@ def replay(n, TB, fwd):
     return full_groups
 
 
+def segment(nA, s, segs):
+    """dilu.hpp boxSegment: rows [lo, hi) of segment s of segs."""
+    return s * nA // segs, (s + 1) * nA // segs
+
+
+def replay_segments(n, TB, fwd, segs):
+    """The lone (tile-local) sweep of each segment of each tile: its steps
+    tFirst..tLast, the FULL condition and the addresses of a FULL step."""
+    tiles, TC = tile_grid(n, TB)
+    full_groups = 0
+    for m in tiles.values():
+        nA, actB, actC, nSteps = m["nA"], m["actB"], m["actC"], m["steps"]
+        tLo = actB - 1 + actC - 1
+        dA = kBoxAhead if fwd else -kBoxAhead
+        covered = set()
+        for sg in range(segs):
+            lo, hi = segment(nA, sg, segs)
+            tFirst, tLast = lo, hi - 1 + tLo
+            nLocal = tLast - tFirst + 1
+            # every step of the segment is a step of the tile
+            assert nLocal <= 0 or (0 <= tFirst and tLast < nSteps)
+            # each row of the tile falls in exactly one segment's steps
+            for ia in range(lo, hi):
+                assert ia not in covered
+                covered.add(ia)
+            for i0 in range(0, max(nLocal, 0), kGroup):
+                tMin = tFirst + i0 if fwd else tLast - (i0 + kGroup - 1)
+                tMax = tFirst + i0 + kGroup - 1 if fwd else tLast - i0
+                if not (tMin >= lo + tLo and tMax <= hi - 1 and i0 + kGroup - 1 + kBoxAhead < nLocal):
+                    continue
+                full_groups += 1
+                for i in range(i0, i0 + kGroup):
+                    t = tFirst + i if fwd else tLast - i
+                    # records / input of the step kBoxAhead later, inside the segment's steps
+                    j = t + dA
+                    assert tFirst <= j <= tLast and (t if fwd else j) >= 0
+                    for bl, cl in itertools.product(range(actB), range(actC)):
+                        assert lo <= t - bl - cl < hi
+        assert covered == set(range(nA))
+    return full_groups
+
+
 def shapes():
     rng = random.Random(7)
     out = [(160, 80, 80), (200, 100, 19), (37, 23, 29), (41, 27, 7), (12, 14, 40), (5, 5, 5), (64, 1, 1),
@@ -111,3 +153,13 @@ def test_duct_fluid_tiles_run_mostly_full_groups():
     tiles, _ = tile_grid((160, 80, 80), 8)
     per_tile = replay((160, 80, 80), 8, True) / len(tiles)
     assert per_tile == 9   # of 11 groups (174 steps)
+
+
+@pytest.mark.parametrize("segs", [2, 3])
+def test_segmented_lone_sweep_addressing(segs):
+    groups = 0
+    for n in shapes():
+        for TB in (1, 4, 8, 16, 64):
+            for fwd in (True, False):
+                groups += replay_segments(n, TB, fwd, segs)
+    assert groups > 0
