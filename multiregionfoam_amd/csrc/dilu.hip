@@ -332,11 +332,11 @@ __device__ inline __amdgpu_buffer_rsrc_t rawBuffer(const void *base, size_t byte
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
     int n = __builtin_amdgcn_readfirstlane((int)bytes);
-    // every resource keeps its own size register: two resources of equal size
+    int w3 = kRawBufferWord3;
+    // every resource keeps its own size and flag registers: two resources
     // sharing one would be reassembled before each use
-    asm volatile("" : "+s"(n));
-    return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), (short)0, n,
-                                             kRawBufferWord3);
+    asm volatile("" : "+s"(n), "+s"(w3));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), (short)0, n, w3);
 }
 
 template <typename X, int aux>
@@ -612,21 +612,42 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 // into every step)
                 int is = i;
                 asm volatile("" : "+s"(is));
-                const U *ib = myIn + (size_t)(kFwd ? tFirst + is : tLast - is) * TBC;
-                if (needB && hb == kSent) hb = haloWait(ib, cl, kSent, V.err);
-                if (needC && hc == kSent) hc = haloWait(ib, TC + bl, kSent, V.err);
+                // Poll rounds: each reloads this step's inbox values and those
+                // already requested for the next steps, all in flight together,
+                // until this step's have arrived.  The later steps' values then
+                // come with the last round instead of a round trip of their own.
+                // Every lane loads and selects (no divergent code), so the ring
+                // keeps its registers.
+                const U *ib[kBoxInAhead];
 #pragma unroll
-                for (int d = 1; d < kBoxInAhead; ++d) {
-                    const U *fb = myIn + (size_t)stepOf(is + d) * TBC;
-                    U &vb = hbuf[(u + d) % kBoxAhead];
-                    U &vc = cbuf[(u + d) % kBoxAhead];
-                    // every lane loads and selects (no divergent code), so
-                    // the ring keeps its registers
-                    const U nb = haloLoad(fb, cl), nc = haloLoad(fb, TC + bl);
-                    vb = vb == kSent ? nb : vb;
-                    vc = vc == kSent ? nc : vc;
+                for (int d = 0; d < kBoxInAhead; ++d) ib[d] = myIn + (size_t)stepOf(is + d) * TBC;
+                for (int spins = 0;; ++spins) {
+                    U nb[kBoxInAhead], nc[kBoxInAhead];
+#pragma unroll
+                    for (int d = 0; d < kBoxInAhead; ++d) {
+                        nb[d] = haloLoad(ib[d], cl);
+                        nc[d] = haloLoad(ib[d], TC + bl);
+                    }
+                    hb = hb == kSent ? nb[0] : hb;
+                    hc = hc == kSent ? nc[0] : hc;
+#pragma unroll
+                    for (int d = 1; d < kBoxInAhead; ++d) {
+                        U &vb = hbuf[(u + d) % kBoxAhead];
+                        U &vc = cbuf[(u + d) % kBoxAhead];
+                        vb = vb == kSent ? nb[d] : vb;
+                        vc = vc == kSent ? nc[d] : vc;
+                    }
+                    const bool late = (needB && hb == kSent) || (needC && hc == kSent);
+                    if (__builtin_amdgcn_ballot_w64(late) == 0) break;
+                    // an expired wait ends every later one at once, so a
+                    // broken schedule ends the kernel instead of hanging it
+                    if (spins >= kPollLimit ||
+                        __hip_atomic_load(V.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        if (threadIdx.x == 0) atomicOr(V.err, 2);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                __builtin_amdgcn_s_waitcnt(0);
             }
             yprev = yz;
             if (inRange) {
