@@ -32,25 +32,46 @@ constexpr double kSmall = 1e-20;  // lduMatrix::small_
 // Block-reduce NV values and store the block's partial sums (slot j as the
 // double-double pair partials[(blockIdx.x * 4 + j) * 2 + {0, 1}]).  No device-scope fence is needed: the
 // partials are read by the finalise kernel launched after this one.
-template <int NV>
+//
+// EXACT = false (the FP32 inner solves of mixed precision, whose terms carry
+// 24-bit values): plain double sums in the same fixed order, about a third of
+// the work; the partial's low part is zero.
+template <int NV, bool EXACT = true>
 __device__ void blockPartial(double (&v)[NV], double *partials)
 {
     __shared__ DD sh[kMaxBlock / 64][NV];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     DD d[NV];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) d[j] = waveSum(DD{v[j], 0.0});
+    for (int j = 0; j < NV; ++j) {
+        if (EXACT) {
+            d[j] = waveSum(DD{v[j], 0.0});
+        } else {
+            double t = v[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            d[j] = DD{t, 0.0};
+        }
+    }
     if (lane == 0)
         for (int j = 0; j < NV; ++j) sh[wid][j] = d[j];
     __syncthreads();
     if (threadIdx.x == 0)
         for (int j = 0; j < NV; ++j) {
             DD t = sh[0][j];
-            for (int w = 1; w < nw; ++w) t = ddAdd(t, sh[w][j]);
+            for (int w = 1; w < nw; ++w) {
+                if (EXACT) t = ddAdd(t, sh[w][j]);
+                else t.hi += sh[w][j].hi;
+            }
             partials[((size_t)blockIdx.x * 4 + j) * 2] = t.hi;
             partials[((size_t)blockIdx.x * 4 + j) * 2 + 1] = t.lo;
         }
 }
+
+// the block sums of a kernel over values T: exact (double-double) for the
+// FP64 path, plain double for the FP32 inner solves
+template <typename T>
+constexpr bool kExactSums = sizeof(T) == sizeof(double);
 
 // lduSolver::stop with SolverPerformance::checkConvergence.  A residual
 // that is not finite (NaN or Inf in the system, the field or the Krylov
@@ -286,7 +307,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_setup_residual(MatView<T> A, cons
         r[p] = b[p] - ax;
         v[0] = (double)x[p];
     }
-    blockPartial<1>(v, partials);
+    blockPartial<1, kExactSums<T>>(v, partials);
 }
 
 // normFactor (lduMatrix::solver::normFactor) and the initial residual; also
@@ -312,7 +333,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_setup_norm(const BlockEntry *bloc
         if (pv) pv[p] = 0;
         if (vv) vv[p] = 0;
     }
-    blockPartial<3>(v, partials);
+    blockPartial<3, kExactSums<T>>(v, partials);
 }
 
 // v = A ph, alpha = rho / (rw . v)
@@ -330,7 +351,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_spmv_alpha(MatView<T> A, const Bl
         v[p] = y;
         acc[0] = (double)rw[p] * (double)y;
     }
-    blockPartial<1>(acc, partials);
+    blockPartial<1, kExactSums<T>>(acc, partials);
 }
 
 // t = A sh, omega = (t . s) / (t . t)
@@ -350,7 +371,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_spmv_omega(MatView<T> A, const Bl
         acc[0] = yd * (double)sv[p];
         acc[1] = yd * yd;
     }
-    blockPartial<2>(acc, partials);
+    blockPartial<2, kExactSums<T>>(acc, partials);
 }
 
 // p = r + beta*p - beta*omega*v
@@ -394,7 +415,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_bicg_xr(const BlockEntry *blocks,
         acc[0] = fabs((double)rn);
         acc[1] = (double)rw[p] * (double)rn;
     }
-    blockPartial<2>(acc, partials);
+    blockPartial<2, kExactSums<T>>(acc, partials);
 }
 
 // PCG: rho = w . r, beta
@@ -407,7 +428,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pcg_rho(const BlockEntry *blocks,
     if (st[sys].status != kRunning) return;
     double acc[1] = {0.0};
     if (ok) acc[0] = (double)w[p] * (double)r[p];
-    blockPartial<1>(acc, partials);
+    blockPartial<1, kExactSums<T>>(acc, partials);
 }
 
 // PCG: pA = wA (first iteration) or wA + beta*pA
@@ -435,7 +456,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_spmv_pcg(MatView<T> A, const Bloc
         w[p] = y;
         acc[0] = (double)y * (double)pA[p];
     }
-    blockPartial<1>(acc, partials);
+    blockPartial<1, kExactSums<T>>(acc, partials);
 }
 
 // PCG: x += alpha*pA; r -= alpha*wA; residual
@@ -455,7 +476,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_pcg_xr(const BlockEntry *blocks, 
         r[p] = rn;
         acc[0] = fabs((double)rn);
     }
-    blockPartial<1>(acc, partials);
+    blockPartial<1, kExactSums<T>>(acc, partials);
 }
 
 // ---------------------------------------------------------------------------
