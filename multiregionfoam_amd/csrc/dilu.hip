@@ -745,6 +745,35 @@ __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *
         for (int q = blockIdx.x * 64 + threadIdx.x; q < nClear; q += gridDim.x * 64) inboxClear[q] = Bits<T>::kSent;
 }
 
+// Both sweeps of lone tiles in one launch: each wave sweeps its tile (or tile
+// segment) forward and then backward, since its backward rows depend only on
+// its own forward rows.  The same per-row operations as the two launches.
+template <typename T, int KT>
+__global__ __launch_bounds__(64) void k_dilu_box_pair(DiluView<T> V, const SysState *st, const T *__restrict__ in,
+                                                      T *out)
+{
+    __shared__ T xch[64];
+    const int segs = V.boxSegs;
+    const int j = (int)blockIdx.x / segs, seg = (int)blockIdx.x - j * segs;
+    const int32_t *m = V.boxMeta + (size_t)V.boxOrder[j] * kBoxMeta;
+    if (st[m[kBmSys]].status != kRunning || m[kBmSteps] <= 0) return;
+    unsigned long long nSlow = 0;
+    using U = typename Bits<T>::U;
+    switch (m[kBmAxis]) {
+    case 0: boxSweep<T, KT, kDiluFwd, 0, true>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    case 1: boxSweep<T, KT, kDiluFwd, 1, true>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    default: boxSweep<T, KT, kDiluFwd, 2, true>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    }
+    // the forward rows this wave stored are read back below (workgroup scope:
+    // the same wave; an agent-scope fence would also write the L2 back)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    switch (m[kBmAxis]) {
+    case 0: boxSweep<T, KT, kDiluBwd, 0, true>(V, m, out, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    case 1: boxSweep<T, KT, kDiluBwd, 1, true>(V, m, out, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    default: boxSweep<T, KT, kDiluBwd, 2, true>(V, m, out, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    }
+}
+
 // rD of the FP32 records from the FP64 factor (one thread per row)
 template <int K>
 __global__ void k_convert_rd(DiluView<float> V, const double *rD64, int nPos)
@@ -907,6 +936,20 @@ hipError_t fillHalo32(void *h, int n, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t diluApplyLone(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s)
+{
+    if (V.nBoxWaves <= 0 || V.nWaves > 0 || !V.boxLone) return hipErrorInvalidValue;
+    const int nb = V.nBoxWaves * (V.boxSegs > 1 ? V.boxSegs : 1);
+    if (V.K == 3) k_dilu_box_pair<T, 3><<<nb, 64, 0, s>>>(V, st, in, out);
+    else k_dilu_box_pair<T, 0><<<nb, 64, 0, s>>>(V, st, in, out);
+    return hipGetLastError();
+}
+
+template hipError_t diluApplyLone<double>(const DiluView<double> &, const SysState *, const double *, double *,
+                                          hipStream_t);
+template hipError_t diluApplyLone<float>(const DiluView<float> &, const SysState *, const float *, float *,
+                                         hipStream_t);
 template hipError_t diluForward<double>(const DiluView<double> &, const SysState *, const double *, double *,
                                         hipStream_t);
 template hipError_t diluForward<float>(const DiluView<float> &, const SysState *, const float *, float *,

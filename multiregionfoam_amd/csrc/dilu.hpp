@@ -101,6 +101,10 @@ template <typename T>
 hipError_t diluForward(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s);
 template <typename T>
 hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipStream_t s);
+// Both sweeps in one launch when every wave is a lone box tile (no general
+// waves): out = M^-1 in, each wave forward then backward over its own rows.
+template <typename T>
+hipError_t diluApplyLone(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s);
 // Factorisation: rD = 1/d for every row of the batch (FP64), written to
 // rDout and into the forward and backward records.
 hipError_t diluFactor(const DiluView<double> &V, hipStream_t s);

@@ -603,6 +603,7 @@ hipError_t launchSetup(const MatView<T> &A, const Krylov<T> &K, bool pcg, hipStr
 template <typename T>
 hipError_t launchPrecondition(const DiluView<T> &D, const Krylov<T> &K, const T *in, T *out, hipStream_t st)
 {
+    if (D.boxLone && D.nWaves == 0 && D.nBoxWaves > 0) return diluApplyLone<T>(D, K.state, in, out, st);
     hipError_t e = diluForward<T>(D, K.state, in, out, st);
     if (e != hipSuccess) return e;
     return diluBackward<T>(D, K.state, out, st);
@@ -617,7 +618,11 @@ hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const 
         if (probe && probe->which == which) (void)hipEventRecord(begin ? probe->begin : probe->end, st);
     };
     k_bicg_p<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
-    {
+    if (!(probe && (probe->which == 1 || probe->which == 2))) {
+        hipError_t e = launchPrecondition(D, K, K.pv, K.ph, st);
+        if (e != hipSuccess) return e;
+    } else {
+        // the two sweeps timed one by one
         mark(1, true);
         hipError_t e = diluForward<T>(D, K.state, K.pv, K.ph, st);
         if (e != hipSuccess) return e;
