@@ -361,10 +361,12 @@ __device__ inline void bufStore(X v, __amdgpu_buffer_rsrc_t r, unsigned voff, un
 // no inbox to read and no values to deliver.
 // seg, segs: a lone sweep of segment seg of segs of the tile's rows (rows
 // of the other segments are not this wave's: it neither uses nor writes them)
-template <typename T, int KT, int MODE, int AX, bool LONE>
+// INP (forward sweeps): the input is BiCGStab's update, computed and stored
+// by the sweep (see SweepInput) instead of read from `in`.
+template <typename T, int KT, int MODE, int AX, bool LONE, int INP = kInpPlain>
 __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m, const T *__restrict__ in, T *out,
                                          typename Bits<T>::U *inbox, T *xch, unsigned long long &nSlow, int seg,
-                                         int segs)
+                                         int segs, const SweepInput<T> &inp = SweepInput<T>{})
 {
     using U = typename Bits<T>::U;
     using R = typename RecordOf<T, MODE>::V;
@@ -428,7 +430,12 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     const R *myRecs = recs + (size_t)s0 * G * 64;
     const auto rRec = rawBuffer(myRecs, (size_t)nSteps * rowBytes);
     const size_t rowsBytes = (size_t)nSteps * 64 * sizeof(T);
-    const auto rIn = MODE == kDiluFactor ? rawBuffer(nullptr, 0) : rawBuffer(in + (size_t)s0 * 64, rowsBytes);
+    const auto rIn = MODE == kDiluFactor ? rawBuffer(nullptr, 0)
+                                          : rawBuffer((INP ? inp.r : in) + (size_t)s0 * 64, rowsBytes);
+    // the fused input's other operands: the vector it is stored to (p, read
+    // first for its old value; s) and v
+    const auto rDst = INP ? rawBuffer(inp.dst + (size_t)s0 * 64, rowsBytes) : rawBuffer(nullptr, 0);
+    const auto rV = INP ? rawBuffer(inp.v + (size_t)s0 * 64, rowsBytes) : rawBuffer(nullptr, 0);
     const auto rOut = MODE == kDiluFactor ? rawBuffer(nullptr, 0) : rawBuffer(out + (size_t)s0 * 64, rowsBytes);
     const U *myIn = inbox + base;
     const auto rInbox = rawBuffer(myIn, LONE ? 0 : (size_t)nSteps * inRowBytes);
@@ -446,6 +453,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     struct Step {
         R v[NL];
         T a;
+        T pOld, vv;   // INP: the update's other operands
     };
     // step i of the sweep (clamped: loads past the end re-read the last step)
     auto stepOf = [&](int i) {
@@ -459,6 +467,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
         for (int g = 0; g < G2; ++g) S.v[g] = bufLoad<R, 0>(rRec, vRec + g * 64 * sizeof(R), so);
         if (kExtra) S.v[NL - 1] = bufLoad<R, 0>(rRec, vRec + gLast * 64 * sizeof(R), so);
         if (MODE != kDiluFactor) S.a = bufLoad<T, 0>(rIn, vRow, (unsigned)(j * 64 * sizeof(T)));
+        if (INP == kInpP) S.pOld = bufLoad<T, 0>(rDst, vRow, (unsigned)(j * 64 * sizeof(T)));
+        if (INP) S.vv = bufLoad<T, 0>(rV, vRow, (unsigned)(j * 64 * sizeof(T)));
     };
     auto loadIn = [&](int i, U &hb, U &hc) {
         const unsigned so = (unsigned)(stepOf(i) * inRowBytes);
@@ -489,6 +499,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
         for (int g = 0; g < G2; ++g) S.v[g] = bufLoad<R, 0>(rRec, vRecF + g * 64 * sizeof(R), so);
         if (kExtra) S.v[NL - 1] = bufLoad<R, 0>(rRec, vRecF + gLast * 64 * sizeof(R), so);
         if (MODE != kDiluFactor) S.a = bufLoad<T, 0>(rIn, vInF, (unsigned)(tm * 64 * sizeof(T)));
+        if (INP == kInpP) S.pOld = bufLoad<T, 0>(rDst, vInF, (unsigned)(tm * 64 * sizeof(T)));
+        if (INP) S.vv = bufLoad<T, 0>(rV, vInF, (unsigned)(tm * 64 * sizeof(T)));
     };
     auto loadInF = [&](int t, U &hb, U &hc) {
         const unsigned so = (unsigned)((t + mOff) * inRowBytes);
@@ -553,13 +565,18 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             T last;
             if (kExtra) last = (T)elem(S.v[NL - 1], eLast);
             else last = (T)elem(S.v[3 / N], 3 % N);
+            // the row's input: BiCGStab's p or s update where fused (the
+            // expressions of k_bicg_p / k_bicg_s), else the loaded value
+            T ain = S.a;
+            if (INP == kInpP) ain = S.a + inp.c1 * S.pOld - inp.c2 * S.vv;
+            else if (INP == kInpS) ain = S.a - inp.c1 * S.vv;
             // the row's value from its neighbours' values
             auto rowValue = [&](U hbv, U hcv) -> T {
                 const T vB = useB ? Bits<T>::from(hbv) : xb;
                 const T vC = useC ? Bits<T>::from(hcv) : xc;
                 T y;
-                if (MODE == kDiluFwd) y = S.a * last;
-                else if (MODE == kDiluBwd) y = S.a;
+                if (MODE == kDiluFwd) y = ain * last;
+                else if (MODE == kDiluBwd) y = ain;
                 else y = last;
                 // the three terms in slot order, which is the face order.  In
                 // the sweeps a missing neighbour has a zero coefficient and a
@@ -664,8 +681,10 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                     }
                 } else if (FULL) {
                     bufStore<T, 0>(yz, rOut, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
+                    if (INP) bufStore<T, 0>(ain, rDst, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
                 } else {
                     bufStore<T, 0>(yz, rOut, !cut || mine(ia) ? vRow : kNoLane, so);
+                    if (INP) bufStore<T, 0>(ain, rDst, !cut || mine(ia) ? vRow : kNoLane, so);
                 }
             }
             if (!LONE) {
@@ -713,10 +732,24 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
 
 // LONE: every box tile of the launch is alone in its region (chosen by the
 // launcher, so a launch with neighbouring tiles runs no lone-tile code).
-template <typename T, int KT, int MODE, bool LONE>
+// INP: a forward sweep computing BiCGStab's update as its input (SweepInput;
+// the scalars are filled in from each wave's system).
+template <typename T>
+__device__ inline SweepInput<T> sweepInputOf(int inpKind, SweepInput<T> in, const SysState &S)
+{
+    if (inpKind == kInpP) {
+        in.c1 = (T)S.beta;
+        in.c2 = (T)(S.beta * S.omega);
+    } else {
+        in.c1 = (T)S.alpha;
+    }
+    return in;
+}
+
+template <typename T, int KT, int MODE, bool LONE, int INP = kInpPlain>
 __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *st, const T *__restrict__ in, T *out,
                                                  typename Bits<T>::U *inbox, typename Bits<T>::U *inboxClear,
-                                                 int nClear)
+                                                 int nClear, SweepInput<T> inpBase = SweepInput<T>{})
 {
     __shared__ T xch[64];
     // a lone launch may cut every tile into segs segments: segs blocks per tile
@@ -728,10 +761,11 @@ __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *
     const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
     unsigned long long nSlow = 0;
     if (active && m[kBmSteps] > 0) {
+        const SweepInput<T> inp = INP ? sweepInputOf(INP, inpBase, st[m[kBmSys]]) : inpBase;
         switch (m[kBmAxis]) {
-        case 0: boxSweep<T, KT, MODE, 0, LONE>(V, m, in, out, inbox, xch, nSlow, seg, segs); break;
-        case 1: boxSweep<T, KT, MODE, 1, LONE>(V, m, in, out, inbox, xch, nSlow, seg, segs); break;
-        default: boxSweep<T, KT, MODE, 2, LONE>(V, m, in, out, inbox, xch, nSlow, seg, segs); break;
+        case 0: boxSweep<T, KT, MODE, 0, LONE, INP>(V, m, in, out, inbox, xch, nSlow, seg, segs, inp); break;
+        case 1: boxSweep<T, KT, MODE, 1, LONE, INP>(V, m, in, out, inbox, xch, nSlow, seg, segs, inp); break;
+        default: boxSweep<T, KT, MODE, 2, LONE, INP>(V, m, in, out, inbox, xch, nSlow, seg, segs, inp); break;
         }
     }
     if (V.trace && threadIdx.x == 0 && segs == 1) {
@@ -748,9 +782,9 @@ __global__ __launch_bounds__(64) void k_dilu_box(DiluView<T> V, const SysState *
 // Both sweeps of lone tiles in one launch: each wave sweeps its tile (or tile
 // segment) forward and then backward, since its backward rows depend only on
 // its own forward rows.  The same per-row operations as the two launches.
-template <typename T, int KT>
+template <typename T, int KT, int INP = kInpPlain>
 __global__ __launch_bounds__(64) void k_dilu_box_pair(DiluView<T> V, const SysState *st, const T *__restrict__ in,
-                                                      T *out)
+                                                      T *out, SweepInput<T> inpBase = SweepInput<T>{})
 {
     __shared__ T xch[64];
     const int segs = V.boxSegs;
@@ -759,10 +793,11 @@ __global__ __launch_bounds__(64) void k_dilu_box_pair(DiluView<T> V, const SysSt
     if (st[m[kBmSys]].status != kRunning || m[kBmSteps] <= 0) return;
     unsigned long long nSlow = 0;
     using U = typename Bits<T>::U;
+    const SweepInput<T> inp = INP ? sweepInputOf(INP, inpBase, st[m[kBmSys]]) : inpBase;
     switch (m[kBmAxis]) {
-    case 0: boxSweep<T, KT, kDiluFwd, 0, true>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs); break;
-    case 1: boxSweep<T, KT, kDiluFwd, 1, true>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs); break;
-    default: boxSweep<T, KT, kDiluFwd, 2, true>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs); break;
+    case 0: boxSweep<T, KT, kDiluFwd, 0, true, INP>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs, inp); break;
+    case 1: boxSweep<T, KT, kDiluFwd, 1, true, INP>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs, inp); break;
+    default: boxSweep<T, KT, kDiluFwd, 2, true, INP>(V, m, in, out, (U *)nullptr, xch, nSlow, seg, segs, inp); break;
     }
     // the forward rows this wave stored are read back below (workgroup scope:
     // the same wave; an agent-scope fence would also write the L2 back)
@@ -946,6 +981,34 @@ hipError_t diluApplyLone(const DiluView<T> &V, const SysState *st, const T *in, 
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t diluApplyFused(const DiluView<T> &V, const SysState *st, int inp, const T *r, T *dst, const T *v, T *out,
+                          hipStream_t s)
+{
+    if (!diluCanFuse(V) || (inp != kInpP && inp != kInpS)) return hipErrorInvalidValue;
+    SweepInput<T> in;
+    in.r = r;
+    in.dst = dst;
+    in.v = v;
+    if (V.boxLone) {
+        const int nb = V.nBoxWaves * (V.boxSegs > 1 ? V.boxSegs : 1);
+        if (inp == kInpP) k_dilu_box_pair<T, 3, kInpP><<<nb, 64, 0, s>>>(V, st, dst, out, in);
+        else k_dilu_box_pair<T, 3, kInpS><<<nb, 64, 0, s>>>(V, st, dst, out, in);
+        return hipGetLastError();
+    }
+    using U = typename Bits<T>::U;
+    const int n = V.nBoxWaves;
+    U *use = (U *)V.boxInF, *clr = (U *)V.boxInB;
+    if (inp == kInpP) k_dilu_box<T, 3, kDiluFwd, false, kInpP><<<n, 64, 0, s>>>(V, st, dst, out, use, clr, V.nBoxIn, in);
+    else k_dilu_box<T, 3, kDiluFwd, false, kInpS><<<n, 64, 0, s>>>(V, st, dst, out, use, clr, V.nBoxIn, in);
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : diluBackward<T>(V, st, out, s);
+}
+
+template hipError_t diluApplyFused<double>(const DiluView<double> &, const SysState *, int, const double *, double *,
+                                           const double *, double *, hipStream_t);
+template hipError_t diluApplyFused<float>(const DiluView<float> &, const SysState *, int, const float *, float *,
+                                          const float *, float *, hipStream_t);
 template hipError_t diluApplyLone<double>(const DiluView<double> &, const SysState *, const double *, double *,
                                           hipStream_t);
 template hipError_t diluApplyLone<float>(const DiluView<float> &, const SysState *, const float *, float *,

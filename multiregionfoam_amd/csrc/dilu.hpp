@@ -95,6 +95,20 @@ struct DiluView {
     hipEvent_t fork = nullptr, join = nullptr;
 };
 
+// A forward sweep of box tiles may compute its input itself, BiCGStab's
+// update of the vector it preconditions, and store that vector (the
+// expressions of k_bicg_p / k_bicg_s, so the results are the same):
+//   kInpP: in = r + c1*dst - c2*v (c1 = beta, c2 = beta*omega), stored to dst (p);
+//   kInpS: in = r - c1*v (c1 = alpha), stored to dst (s).
+enum { kInpPlain = 0, kInpP = 1, kInpS = 2 };
+template <typename T>
+struct SweepInput {
+    const T *r = nullptr;
+    T *dst = nullptr;
+    const T *v = nullptr;
+    T c1 = T(0), c2 = T(0);
+};
+
 // Forward sweep out = (D+L)^-1 in (with the D^-1 scaling of DILU) and the
 // backward sweep in place; systems whose status is not running are skipped.
 template <typename T>
@@ -105,6 +119,19 @@ hipError_t diluBackward(const DiluView<T> &V, const SysState *st, T *inout, hipS
 // waves): out = M^-1 in, each wave forward then backward over its own rows.
 template <typename T>
 hipError_t diluApplyLone(const DiluView<T> &V, const SysState *st, const T *in, T *out, hipStream_t s);
+// The preconditioner applied to BiCGStab's p (inp = kInpP: p = r + beta*p -
+// beta*omega*v) or s (kInpS: s = r - alpha*v) of every system, with the
+// update computed and stored by the forward sweeps (batches of K = 3 box
+// tiles only, no general waves): out = M^-1 dst.  The scalars come from each
+// system's state.
+template <typename T>
+hipError_t diluApplyFused(const DiluView<T> &V, const SysState *st, int inp, const T *r, T *dst, const T *v, T *out,
+                          hipStream_t s);
+template <typename T>
+bool diluCanFuse(const DiluView<T> &V)
+{
+    return V.nWaves == 0 && V.nBoxWaves > 0 && V.K == 3;
+}
 // Factorisation: rD = 1/d for every row of the batch (FP64), written to
 // rDout and into the forward and backward records.
 hipError_t diluFactor(const DiluView<double> &V, hipStream_t s);

@@ -617,11 +617,19 @@ hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const 
     auto mark = [&](int which, bool begin) {
         if (probe && probe->which == which) (void)hipEventRecord(begin ? probe->begin : probe->end, st);
     };
-    k_bicg_p<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
-    if (!(probe && (probe->which == 1 || probe->which == 2))) {
+    // box-only batches: the p and s updates are computed by the forward
+    // sweeps that precondition them (no update kernels)
+    const bool probed = probe && (probe->which == 1 || probe->which == 2);
+    const bool fuse = diluCanFuse(D) && !probed;
+    if (fuse) {
+        hipError_t e = diluApplyFused<T>(D, K.state, kInpP, K.r, K.pv, K.v, K.ph, st);
+        if (e != hipSuccess) return e;
+    } else if (!probed) {
+        k_bicg_p<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
         hipError_t e = launchPrecondition(D, K, K.pv, K.ph, st);
         if (e != hipSuccess) return e;
     } else {
+        k_bicg_p<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.pv, K.v);
         // the two sweeps timed one by one
         mark(1, true);
         hipError_t e = diluForward<T>(D, K.state, K.pv, K.ph, st);
@@ -639,8 +647,11 @@ hipError_t launchBicgIteration(const MatView<T> &A, const DiluView<T> &D, const 
     });
     mark(3, false);
     fin<kFinAlpha>(K, st);
-    k_bicg_s<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.v, K.sv);
-    {
+    if (fuse) {
+        hipError_t e = diluApplyFused<T>(D, K.state, kInpS, K.r, K.sv, K.v, K.sh, st);
+        if (e != hipSuccess) return e;
+    } else {
+        k_bicg_s<T><<<K.nBlk, K.threads, 0, st>>>(K.blocks, K.state, K.r, K.v, K.sv);
         hipError_t e = launchPrecondition(D, K, K.sv, K.sh, st);
         if (e != hipSuccess) return e;
     }
