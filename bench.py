@@ -473,10 +473,14 @@ def run_batch(m, cases, args):
     ks = _inside_solve_kernels(A, args.kernel_reps)
     dilu_bytes = sum(0.5 * A.bytes_model(q, 8)["dilu"] for q in range(n))
     spmv_bytes = sum(A.bytes_model(q, 8)["spmv"] + 8 * cells[q] for q in range(n))
-    per_iter = {"dilu_fwd": 2 * ks["dilu_fwd"], "dilu_bwd": 2 * ks["dilu_bwd"], "spmv_alpha": 2 * ks["spmv_alpha"]}
+    # the roofline kernel: the backward sweep or the SpMV.  In the solve the
+    # forward sweeps of a box-only batch also compute BiCGStab's p / s update
+    # (other kernels, other bytes); the backward sweep is the same kernel in
+    # the solve and in the timing probe
+    per_iter = {"dilu_bwd": 2 * ks["dilu_bwd"], "spmv_alpha": 2 * ks["spmv_alpha"]}
     dom = max(per_iter, key=per_iter.get)
     launch_bytes = spmv_bytes if dom == "spmv_alpha" else dilu_bytes
-    names = {"dilu_fwd": "k_dilu_box<double, 3, 0, false>", "dilu_bwd": "k_dilu_box<double, 3, 1, false>",
+    names = {"dilu_fwd": "k_dilu_box<double, 3, 0, false, 0>", "dilu_bwd": "k_dilu_box<double, 3, 1, false, 0>",
              "spmv_alpha": "k_spmv_alpha<double, %d>" % A.tuning()["ell_chunk"]}
     achieved = launch_bytes / (ks[dom] * 1e-3) / 1e9
     kernels = {k: dict(ms=v, GBs=(spmv_bytes if k == "spmv_alpha" else dilu_bytes) / (v * 1e-3) / 1e9,
@@ -743,15 +747,19 @@ def main():
     s_fwd = A.time_kernel(4, args.kernel_reps)
     s_bwd = A.time_kernel(5, args.kernel_reps)
     s_spmv = A.time_kernel(6, args.kernel_reps)
-    per_iter = {"spmv": 2 * s_spmv, "dilu_fwd": 2 * s_fwd, "dilu_bwd": 2 * s_bwd}
+    # the roofline kernel: the backward sweep or the SpMV (in the solve the
+    # forward sweeps also compute BiCGStab's p / s update: other kernels,
+    # other bytes; the backward sweep is the same kernel in the solve and in
+    # the timing probe)
+    per_iter = {"spmv": 2 * s_spmv, "dilu_bwd": 2 * s_bwd}
     dominant = max(per_iter, key=per_iter.get)
     launch_ms = {"spmv": s_spmv, "dilu_fwd": s_fwd, "dilu_bwd": s_bwd}[dominant]
     # the in-solve SpMV also reads the shadow residual for its dot product
     launch_bytes = (bm64["spmv"] + 8 * info["cells"]) if dominant == "spmv" else 0.5 * bm64["dilu"]
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     chunk = A.tuning()["ell_chunk"]
-    kernel_names = {"spmv": "k_spmv_alpha<double, %d>" % chunk, "dilu_fwd": "k_dilu_box<double, 3, 0, false>",
-                    "dilu_bwd": "k_dilu_box<double, 3, 1, false>"}
+    kernel_names = {"spmv": "k_spmv_alpha<double, %d>" % chunk, "dilu_fwd": "k_dilu_box<double, 3, 0, false, 0>",
+                    "dilu_bwd": "k_dilu_box<double, 3, 1, false, 0>"}
 
     # HBM bytes per launch of that kernel, from the committed rocprofv3 PMC
     # summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain
