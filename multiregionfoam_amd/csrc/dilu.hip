@@ -322,7 +322,6 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kRawBufferWord3 = 0x00020000;   // 32-bit data format, no swizzle, no index stride
 constexpr int kSysCoherent = 16;              // sc1: the policy of the agent-scope relaxed atomics
 constexpr unsigned kNoLane = 0x80000000u;
-constexpr int kCmpUnordered = 8;              // the fcmp predicate "unordered": either operand is NaN
 
 // (the arguments are wave-uniform; readfirstlane states it, so the resource
 // lives in scalar registers)
