@@ -360,10 +360,13 @@ private:
     bool tileLocal32_ = false;   // the FP32 records are those of the tile-local DILU
     // segments per box tile in the tile-local DILU, each swept by a wave of
     // its own (a shorter chain of steps, slightly more inner iterations): as
-    // many as keep the lone waves within kTileSegWaves, at most 3 (duct3d,
-    // 135 tiles: 3 segments, 33.6 ms per mixed solve against 37.1 with 2 and
-    // 39.9 with 1; 4 needed more iterations than they saved)
-    static constexpr int kTileSegWaves = 512;
+    // many as keep the lone waves within kTileSegWaves, at most 3.  The waves
+    // may outnumber the CUs several times over: duct3d (135 tiles) takes 3
+    // segments (33.6 ms per mixed solve against 37.1 with 2 and 39.9 with 1;
+    // 4 needed more iterations than they saved), exchanger3 (585 tiles) 3
+    // (362.8 -> 328.6 ms), batch64 2 (656 -> 558 ms; a budget of 16384 waves,
+    // giving it 3, was slower: 590 ms)
+    static constexpr int kTileSegWaves = 8192;
     int tileSegments() const
     {
         return nBoxWaves_ > 0 ? std::max(1, std::min(3, kTileSegWaves / nBoxWaves_)) : 1;
