@@ -32,7 +32,13 @@ namespace mrfg {
 namespace {
 
 constexpr int kPrefetch = 8;
-constexpr int kWaveRounds = 2;   // prefetch-ring rounds per unrolled group of steps (measured best)
+constexpr int kWaveRounds = 3;   // prefetch-ring rounds per unrolled group of steps
+constexpr int kWaveGroup = kWaveRounds * kPrefetch;
+static_assert(kWaveGroup % kRing == 0, "a group of steps spans whole rounds of the LDS ring");
+// LDS values of a wave's ring: kRing steps, then zeros up to the largest
+// code byte, so that every code byte indexes the ring
+constexpr int kRingSlots = 256;
+static_assert(kRing * 64 <= kCodeInbox, "ring slots lie below the reserved code bytes");
 constexpr int kPollLimit = 1 << 20;
 
 template <typename T>
@@ -94,227 +100,10 @@ struct RecordOf<T, kDiluFactor> {
 
 }  // namespace
 
-// MODE: kDiluFwd / kDiluBwd (T = value type) or kDiluFactor (T = double).
-// HMASK: dependency slots that have an inbox (compile-time, so all inbox
-// prefetch loads are unconditional code).
-template <typename T, int K, int MODE, int HMASK>
-__global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState *st, const T *__restrict__ in,
-                                                  T *out, typename Bits<T>::U *inbox,
-                                                  typename Bits<T>::U *inboxClear, int nClear)
-{
-    using U = typename Bits<T>::U;
-    using R = typename RecordOf<T, MODE>::V;
-    constexpr int N = RecordOf<T, MODE>::N;
-    constexpr int G = packChunks(K + 1, N);      // value chunks per row
-    constexpr int NC = packChunks(K, 4);         // ints holding the byte codes
-    constexpr int H = packChunks(codeInts(K), 4);
-    constexpr int KI = __builtin_popcount(HMASK);
-    constexpr U kSent = Bits<T>::kSent;
-    __shared__ T ring[kRing * 64];
-    const int lane = threadIdx.x;
-    const int w = MODE == kDiluBwd ? V.order[V.nWaves - 1 - blockIdx.x] : V.order[blockIdx.x];
-    const int sys = V.waveSys[w];
-    const bool active = MODE == kDiluFactor || st[sys].status == kRunning;
-    const int s0 = V.waveSlice[w], nSteps = V.waveSteps[w];
-    const int4 *codes = MODE == kDiluBwd ? V.bCodes : V.fCodes;
-    // whether any row of this wave delivers to another wave (wave-uniform)
-    const bool delivers = (V.waveOut[w] & (MODE == kDiluBwd ? 2 : 1)) != 0;
-    const R *recs = MODE == kDiluFactor ? (const R *)V.prodPack : (const R *)(MODE == kDiluBwd ? V.bPack : V.fPack);
-    const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
-    unsigned long long nSlow = 0;
-
-    struct Step {
-        R v[G];
-        int4 c[H];
-        U hv[KI > 0 ? KI : 1];
-        T a;
-    };
-    auto sliceOf = [&](int i) {
-        i = i < nSteps ? i : nSteps - 1;
-        return s0 + (MODE == kDiluBwd ? nSteps - 1 - i : i);
-    };
-    // the row's record, input and inbox values, kPrefetch steps ahead.  Every
-    // address is computable without another load, and steps past the end
-    // re-read the last step, so no load waits for another or is conditional.
-    auto load = [&](int i, Step &S) {
-        const int s = sliceOf(i);
-#pragma unroll
-        for (int g = 0; g < G; ++g) S.v[g] = recs[((size_t)s * G + g) * 64 + lane];
-#pragma unroll
-        for (int h = 0; h < H; ++h) S.c[h] = codes[((size_t)s * H + h) * 64 + lane];
-#pragma unroll
-        for (int kk = 0; kk < KI; ++kk) S.hv[kk] = haloLoad(inbox, ((s * KI + kk) << 6) + lane);
-        if (MODE != kDiluFactor) S.a = in[s * 64 + lane];
-    };
-
-    if (active && nSteps > 0) {
-        Step buf[kPrefetch];
-        // left to the scheduler, which reorders these loads so the loop's
-        // first use waits for all of them; issuing them in consumption order
-        // (as the box kernel does) made the waits exact here but the
-        // tutorial's solve 1.3 % slower
-#pragma unroll
-        for (int u = 0; u < kPrefetch; ++u) load(u, buf[u]);
-        // ring values of a step's dependencies, read as soon as the previous
-        // step has written its slot (a dependency is at most kRing - 1 steps
-        // back, and the slot the step overwrites is never among them), so the
-        // LDS round trip overlaps the rest of the previous step.  A code
-        // without a ring slot reads slot 0, which the select below discards.
-        auto ringSlot = [&](const Step &S, int k) {
-            const int cd = (elem(S.c[k / 16], (k / 4) % 4) >> (8 * (k % 4))) & 0xFF;
-            return cd < 256 - 2 ? cd : 0;
-        };
-        T rv[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) rv[k] = ring[ringSlot(buf[0], k)];
-        // whole groups of steps: a step past the end reads the clamped last
-        // slice and stores nothing (its LDS ring slot is never read), so the
-        // group has no early exits and the record waits stay exact within it;
-        // a group spans kWaveRounds rounds of the prefetch ring (16 steps: 9 %
-        // faster than 8 on the tutorial's chained levels, 32 no faster)
-        for (int i0 = 0; i0 < nSteps; i0 += kWaveRounds * kPrefetch) {
-#pragma unroll
-            for (int uu = 0; uu < kWaveRounds * kPrefetch; ++uu) {
-                const int u = uu % kPrefetch;
-                const int i = i0 + uu;
-                const bool inRange = i < nSteps;
-                {
-                    Step &S = buf[u];
-                    const int s = sliceOf(i);
-                    int code[K];
-                    T c[K];
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        code[k] = (elem(S.c[k / 16], (k / 4) % 4) >> (8 * (k % 4))) & 0xFF;
-                        c[k] = (T)elem(S.v[k / N], k % N);
-                    }
-                    const T last = (T)elem(S.v[K / N], K % N);   // rD, or diag for the factor
-                    // rare slow path (wave-uniform branch): an inbox value not yet
-                    // delivered when it was prefetched is polled here, and the
-                    // inbox prefetches of the following steps are refreshed so
-                    // that one late producer costs one wait, not one per step
-                    bool miss = false;
-#pragma unroll
-                    for (int k = 0; k < K; ++k)
-                        if (HMASK & (1 << k))
-                            miss |= inRange && code[k] == kCodeInbox &&
-                                    S.hv[__builtin_popcount(HMASK & ((1 << k) - 1))] == kSent;
-                    if (__builtin_expect(__any(miss), 0)) {
-                        ++nSlow;
-#pragma unroll
-                        for (int k = 0; k < K; ++k)
-                            if (HMASK & (1 << k)) {
-                                const int kk = __builtin_popcount(HMASK & ((1 << k) - 1));
-                                if (code[k] == kCodeInbox && S.hv[kk] == kSent)
-                                    S.hv[kk] = haloWait(inbox, ((s * KI + kk) << 6) + lane, kSent, V.err);
-                            }
-#pragma unroll
-                        for (int d = 1; d < kPrefetch; ++d) {
-                            Step &F = buf[(u + d) % kPrefetch];
-                            const int sf = sliceOf(i + d);
-#pragma unroll
-                            for (int kk = 0; kk < KI; ++kk)
-                                if (F.hv[kk] == kSent) F.hv[kk] = haloLoad(inbox, ((sf * KI + kk) << 6) + lane);
-                        }
-                        __builtin_amdgcn_s_waitcnt(0);
-                    }
-                    // branch-free fast path: ring or inbox value per dependency;
-                    // a missing dependency contributes an exact zero
-                    T y;
-                    if (MODE == kDiluFwd) y = S.a * last;
-                    else if (MODE == kDiluBwd) y = S.a;
-                    else y = last;
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        T v = rv[k];
-                        if (HMASK & (1 << k)) {
-                            const U h = S.hv[__builtin_popcount(HMASK & ((1 << k) - 1))];
-                            v = code[k] == kCodeInbox ? Bits<T>::from(h) : v;
-                        }
-                        const bool has = code[k] != kCodeNone;
-                        if (MODE == kDiluFactor) y = has ? y - c[k] / v : y;
-                        else y = has ? y - last * c[k] * v : y;
-                    }
-                    const int p = s * 64 + lane;
-                    // the ring is private to this wavefront and LDS operations of
-                    // a wavefront execute in order: no barrier (which would also
-                    // drain the prefetch loads) is needed between steps
-                    ring[(i % kRing) * 64 + lane] = y;
-                    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                    for (int k = 0; k < K; ++k) rv[k] = ring[ringSlot(buf[(u + 1) % kPrefetch], k)];
-                    if (inRange) {
-                        if (MODE == kDiluFactor) {
-                            const double rd = V.cellOfPos[p] >= 0 ? 1.0 / (double)y : 0.0;
-                            V.rDout[p] = rd;
-                            constexpr int NP = Pack<double>::N;
-                            constexpr int GP = packChunks(K + 1, NP);
-                            ((double *)&V.fPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
-                            ((double *)&V.bPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
-                        } else {
-                            out[p] = y;
-                        }
-                        // deliver the value to the inboxes of its consumers in other waves
-                        if (delivers) {
-#pragma unroll
-                            for (int j = 0; j < K; ++j) {
-                                const int o = elem(S.c[(NC + j) / 4], (NC + j) % 4);
-                                if (o >= 0)
-                                    __hip_atomic_store(&inbox[o], Bits<T>::to(y), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
-                            }
-                        }
-                    }
-                    load(i + kPrefetch, S);
-                }
-            }
-        }
-    }
-    if (V.trace && lane == 0) {
-        unsigned long long *tr = V.trace + 4 * (size_t)blockIdx.x;
-        tr[0] = tStart;
-        tr[1] = wall_clock64();
-        tr[2] = nSlow;
-        tr[3] = (unsigned long long)nSteps;
-    }
-    // reset the inbox of the next sweep (nobody reads it during this one)
-    if (inboxClear)
-        for (int q = blockIdx.x * 64 + lane; q < nClear; q += gridDim.x * 64) inboxClear[q] = kSent;
-}
-
-// ---------------------------------------------------------------------------
-// Structured-box tile waves.
-//
-// Lane l = bl + TB*cl of a tile wave owns grid line (bl, cl) of the tile's
-// TB x TC lines along the sequential axis; at step t it computes cell
-// ia = t - bl - cl of its line.  Each of the cell's three dependencies was
-// computed one step earlier (in processing order): along the line by the
-// lane itself (kept in a register), across the line by lane l -/+ 1 and
-// lane l -/+ TB (exchanged through 64 LDS values), and across the tile's
-// edge by the neighbouring tile's wave, which delivers it to this wave's
-// inbox: (TB + TC) values per step, [cl] for the b edge and [TC + bl] for
-// the c edge.  Records hold the three coefficients in fixed slots (slot
-// 2 - axis of the neighbour, which is also the reference's face order), so a
-// step loads its record, its input and two inbox values, all addressed
-// without another load.
-// ---------------------------------------------------------------------------
-
-// Prefetch distances.  A step is bound by its own instruction issue and LDS
-// round trip, not by memory latency, and a deeper inbox prefetch makes every
-// consumer tile lag its producer by that many more steps.
-constexpr int kBoxAhead = 4;     // steps between a record load and its use
-constexpr int kBoxInAhead = 3;   // steps between an inbox load and its use (<= kBoxAhead)
-constexpr int kBoxRounds = 4;    // prefetch-ring rounds per unrolled group of steps (measured best)
-
-// KT: the batch's record width K when it is 3 (records of two FP64 / one
-// FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
-// rD in a chunk of its own).
-// LONE: the tile has no neighbouring tile (alone in its region), so it has
-// no inbox to read and no values to deliver.
 // Raw buffer access (gfx9 buffer resource: base, byte size, 32-bit
-// offsets).  Every address of the sweep below is a wave-uniform resource
-// plus a scalar step offset plus a per-lane constant, so a step spends no
-// instructions on 64-bit address arithmetic.  A lane offset of kNoLane lies
+// offsets).  Every address of the sweeps' fast paths is a wave-uniform
+// resource plus a scalar step offset plus a per-lane constant, so a step
+// spends no instructions on 64-bit address arithmetic.  A lane offset of kNoLane lies
 // beyond every resource: the hardware drops such a store, which replaces the
 // per-step exec-mask juggling of conditional stores.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -352,6 +141,261 @@ __device__ inline void bufStore(X v, __amdgpu_buffer_rsrc_t r, unsigned voff, un
     if constexpr (sizeof(X) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, aux);
     else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, aux);
 }
+
+// MODE: kDiluFwd / kDiluBwd (T = value type) or kDiluFactor (T = double).
+// HMASK: dependency slots that have an inbox (compile-time, so all inbox
+// prefetch loads are unconditional code).
+template <typename T, int K, int MODE, int HMASK>
+__global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState *st, const T *__restrict__ in,
+                                                  T *out, typename Bits<T>::U *inbox,
+                                                  typename Bits<T>::U *inboxClear, int nClear)
+{
+    using U = typename Bits<T>::U;
+    using R = typename RecordOf<T, MODE>::V;
+    constexpr int N = RecordOf<T, MODE>::N;
+    constexpr int G = packChunks(K + 1, N);      // value chunks per row
+    constexpr int NC = packChunks(K, 4);         // ints holding the byte codes
+    constexpr int H = packChunks(codeInts(K), 4);
+    constexpr int KI = __builtin_popcount(HMASK);
+    constexpr U kSent = Bits<T>::kSent;
+    __shared__ T ring[kRingSlots];
+    const int lane = threadIdx.x;
+    const int w = MODE == kDiluBwd ? V.order[V.nWaves - 1 - blockIdx.x] : V.order[blockIdx.x];
+    const int sys = V.waveSys[w];
+    const bool active = MODE == kDiluFactor || st[sys].status == kRunning;
+    const int s0 = V.waveSlice[w], nSteps = V.waveSteps[w];
+    const int4 *codes = MODE == kDiluBwd ? V.bCodes : V.fCodes;
+    // whether any row of this wave delivers to another wave (wave-uniform)
+    const bool delivers = (V.waveOut[w] & (MODE == kDiluBwd ? 2 : 1)) != 0;
+    const R *recs = MODE == kDiluFactor ? (const R *)V.prodPack : (const R *)(MODE == kDiluBwd ? V.bPack : V.fPack);
+    const unsigned long long tStart = V.trace ? wall_clock64() : 0ull;
+    unsigned long long nSlow = 0;
+
+    struct Step {
+        R v[G];
+        int4 c[H];
+        U hv[KI > 0 ? KI : 1];
+        T a;
+    };
+    auto sliceOf = [&](int i) {
+        i = i < nSteps ? i : nSteps - 1;
+        return s0 + (MODE == kDiluBwd ? nSteps - 1 - i : i);
+    };
+    // the row's record, input and inbox values, kPrefetch steps ahead.  Every
+    // address is computable without another load, and steps past the end
+    // re-read the last step, so no load waits for another or is conditional.
+    auto load = [&](int i, Step &S) {
+        const int s = sliceOf(i);
+#pragma unroll
+        for (int g = 0; g < G; ++g) S.v[g] = recs[((size_t)s * G + g) * 64 + lane];
+#pragma unroll
+        for (int h = 0; h < H; ++h) S.c[h] = codes[((size_t)s * H + h) * 64 + lane];
+#pragma unroll
+        for (int kk = 0; kk < KI; ++kk) S.hv[kk] = haloLoad(inbox, ((s * KI + kk) << 6) + lane);
+        if (MODE != kDiluFactor) S.a = in[s * 64 + lane];
+    };
+
+    // FULL groups (below) address a step through per-wave resources: the
+    // wave's rows of the records, codes, input and output, a scalar offset of
+    // the step's slice in the wave and a per-lane constant
+    constexpr bool kCanFull = MODE != kDiluFactor && HMASK == 0;
+    const size_t recBytes = (size_t)nSteps * G * 64 * sizeof(R);
+    const bool fullWave = kCanFull && !delivers && recBytes <= (size_t)INT_MAX;
+    const auto rRec = rawBuffer(recs + (size_t)s0 * G * 64, fullWave ? recBytes : 0);
+    const auto rCode = rawBuffer(codes + (size_t)s0 * H * 64, fullWave ? (size_t)nSteps * H * 64 * sizeof(int4) : 0);
+    const auto rIn = rawBuffer(kCanFull ? in + (size_t)s0 * 64 : nullptr, fullWave ? (size_t)nSteps * 64 * sizeof(T) : 0);
+    const auto rOut = rawBuffer(kCanFull ? out + (size_t)s0 * 64 : nullptr, fullWave ? (size_t)nSteps * 64 * sizeof(T) : 0);
+    const unsigned vRec = lane * (unsigned)sizeof(R), vCode = lane * (unsigned)sizeof(int4),
+                   vRow = lane * (unsigned)sizeof(T);
+    auto loadF = [&](int i, Step &S) {
+        const unsigned j = (unsigned)(MODE == kDiluBwd ? nSteps - 1 - i : i);
+#pragma unroll
+        for (int g = 0; g < G; ++g) S.v[g] = bufLoad<R, 0>(rRec, vRec + g * 64 * sizeof(R), j * G * 64 * sizeof(R));
+#pragma unroll
+        for (int h = 0; h < H; ++h) S.c[h] = bufLoad<int4, 0>(rCode, vCode + h * 64 * sizeof(int4), j * H * 64 * sizeof(int4));
+        S.a = bufLoad<T, 0>(rIn, vRow, j * 64 * sizeof(T));
+    };
+
+    if (active && nSteps > 0) {
+        // ring slots above the wave's kRing steps hold zeros: a code without
+        // a ring slot (kCodeInbox, kCodeNone) reads a finite zero there
+        for (int q = kRing * 64 + lane; q < kRingSlots; q += 64) ring[q] = T(0);
+        __builtin_amdgcn_wave_barrier();
+        Step buf[kPrefetch];
+        // left to the scheduler, which reorders these loads so the loop's
+        // first use waits for all of them; issuing them in consumption order
+        // (as the box kernel does) made the waits exact here but the
+        // tutorial's solve 1.3 % slower
+#pragma unroll
+        for (int u = 0; u < kPrefetch; ++u) load(u, buf[u]);
+        // ring values of a step's dependencies, read as soon as the previous
+        // step has written its slot (a dependency is at most kRing - 1 steps
+        // back, and the slot the step overwrites is never among them), so the
+        // LDS round trip overlaps the rest of the previous step.  The code
+        // byte is the ring slot.
+        auto ringSlot = [&](const Step &S, int k) {
+            return (elem(S.c[k / 16], (k / 4) % 4) >> (8 * (k % 4))) & 0xFF;
+        };
+        T rv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) rv[k] = ring[ringSlot(buf[0], k)];
+        // Whole groups of steps: a step past the end reads the clamped last
+        // slice and stores nothing (its LDS ring slot is never read), so the
+        // group has no early exits and the record waits stay exact within it.
+        // A group spans kWaveRounds rounds of the prefetch ring and a whole
+        // number of ring rounds, so every step's ring slot is a constant.
+        //
+        // A group is FULL when its steps and their prefetches all lie inside
+        // the wave's steps, in a sweep without inbox values or deliveries:
+        // there nothing is clamped, the store is unconditional, every address
+        // is a scalar offset plus a lane constant, and each dependency term is
+        // taken without a select (a missing dependency has a zero coefficient
+        // and reads a zero slot, so for rD >= 0 its term is +0 and leaves the
+        // value unchanged bit for bit).
+        auto runGroup = [&](int i0, auto fullTag) {
+            constexpr bool FULL = decltype(fullTag)::value;
+#pragma unroll
+            for (int uu = 0; uu < kWaveGroup; ++uu) {
+                const int u = uu % kPrefetch;
+                const int i = i0 + uu;
+                const bool inRange = FULL || i < nSteps;
+                Step &S = buf[u];
+                const int s = FULL ? 0 : sliceOf(i);
+                int code[K];
+                T c[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    code[k] = (elem(S.c[k / 16], (k / 4) % 4) >> (8 * (k % 4))) & 0xFF;
+                    c[k] = (T)elem(S.v[k / N], k % N);
+                }
+                const T last = (T)elem(S.v[K / N], K % N);   // rD, or diag for the factor
+                // rare slow path (wave-uniform branch): an inbox value not yet
+                // delivered when it was prefetched is polled here, and the
+                // inbox prefetches of the following steps are refreshed so
+                // that one late producer costs one wait, not one per step
+                if constexpr (!FULL && KI > 0) {
+                    bool miss = false;
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        if (HMASK & (1 << k))
+                            miss |= inRange && code[k] == kCodeInbox &&
+                                    S.hv[__builtin_popcount(HMASK & ((1 << k) - 1))] == kSent;
+                    if (__builtin_expect(__any(miss), 0)) {
+                        ++nSlow;
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            if (HMASK & (1 << k)) {
+                                const int kk = __builtin_popcount(HMASK & ((1 << k) - 1));
+                                if (code[k] == kCodeInbox && S.hv[kk] == kSent)
+                                    S.hv[kk] = haloWait(inbox, ((s * KI + kk) << 6) + lane, kSent, V.err);
+                            }
+#pragma unroll
+                        for (int d = 1; d < kPrefetch; ++d) {
+                            Step &F = buf[(u + d) % kPrefetch];
+                            const int sf = sliceOf(i + d);
+#pragma unroll
+                            for (int kk = 0; kk < KI; ++kk)
+                                if (F.hv[kk] == kSent) F.hv[kk] = haloLoad(inbox, ((sf * KI + kk) << 6) + lane);
+                        }
+                        __builtin_amdgcn_s_waitcnt(0);
+                    }
+                }
+                // branch-free: ring or inbox value per dependency; a missing
+                // dependency contributes an exact zero
+                T y;
+                if (MODE == kDiluFwd) y = S.a * last;
+                else if (MODE == kDiluBwd) y = S.a;
+                else y = last;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    T v = rv[k];
+                    if (HMASK & (1 << k)) {
+                        const U h = S.hv[__builtin_popcount(HMASK & ((1 << k) - 1))];
+                        v = code[k] == kCodeInbox ? Bits<T>::from(h) : v;
+                    }
+                    const bool has = code[k] != kCodeNone;
+                    if (MODE == kDiluFactor) y = has ? y - c[k] / v : y;
+                    else if (FULL) y = y - last * c[k] * v;
+                    else y = has ? y - last * c[k] * v : y;
+                }
+                // the ring is private to this wavefront and LDS operations of
+                // a wavefront execute in order: no barrier (which would also
+                // drain the prefetch loads) is needed between steps
+                ring[(uu % kRing) * 64 + lane] = y;
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int k = 0; k < K; ++k) rv[k] = ring[ringSlot(buf[(u + 1) % kPrefetch], k)];
+                if (FULL) {
+                    bufStore<T, 0>(y, rOut, vRow, (unsigned)(MODE == kDiluBwd ? nSteps - 1 - i : i) * 64 * sizeof(T));
+                    loadF(i + kPrefetch, S);
+                    continue;
+                }
+                const int p = s * 64 + lane;
+                if (inRange) {
+                    if (MODE == kDiluFactor) {
+                        const double rd = V.cellOfPos[p] >= 0 ? 1.0 / (double)y : 0.0;
+                        V.rDout[p] = rd;
+                        constexpr int NP = Pack<double>::N;
+                        constexpr int GP = packChunks(K + 1, NP);
+                        ((double *)&V.fPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                        ((double *)&V.bPack[((size_t)s * GP + K / NP) * 64 + lane])[K % NP] = rd;
+                    } else {
+                        out[p] = y;
+                    }
+                    // deliver the value to the inboxes of its consumers in other waves
+                    if (delivers) {
+#pragma unroll
+                        for (int j = 0; j < K; ++j) {
+                            const int o = elem(S.c[(NC + j) / 4], (NC + j) % 4);
+                            if (o >= 0)
+                                __hip_atomic_store(&inbox[o], Bits<T>::to(y), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                    }
+                }
+                load(i + kPrefetch, S);
+            }
+        };
+        for (int i0 = 0; i0 < nSteps; i0 += kWaveGroup) {
+            if (kCanFull && fullWave && i0 + kWaveGroup - 1 + kPrefetch < nSteps) runGroup(i0, std::true_type{});
+            else runGroup(i0, std::false_type{});
+        }
+    }
+    if (V.trace && lane == 0) {
+        unsigned long long *tr = V.trace + 4 * (size_t)blockIdx.x;
+        tr[0] = tStart;
+        tr[1] = wall_clock64();
+        tr[2] = nSlow;
+        tr[3] = (unsigned long long)nSteps;
+    }
+    // reset the inbox of the next sweep (nobody reads it during this one)
+    if (inboxClear)
+        for (int q = blockIdx.x * 64 + lane; q < nClear; q += gridDim.x * 64) inboxClear[q] = kSent;
+}
+
+// ---------------------------------------------------------------------------
+// Structured-box tile waves.
+//
+// Lane l = bl + TB*cl of a tile wave owns grid line (bl, cl) of the tile's
+// TB x TC lines along the sequential axis; at step t it computes cell
+// ia = t - bl - cl of its line.  Each of the cell's three dependencies was
+// computed one step earlier (in processing order): along the line by the
+// lane itself (kept in a register), across the line by lane l -/+ 1 and
+// lane l -/+ TB (exchanged through 64 LDS values), and across the tile's
+// edge by the neighbouring tile's wave, which delivers it to this wave's
+// inbox: (TB + TC) values per step, [cl] for the b edge and [TC + bl] for
+// the c edge.  Records hold the three coefficients in fixed slots (slot
+// 2 - axis of the neighbour, which is also the reference's face order), so a
+// step loads its record, its input and two inbox values, all addressed
+// without another load.
+// ---------------------------------------------------------------------------
+
+// Prefetch distances.  A step is bound by its own instruction issue and LDS
+// round trip, not by memory latency, and a deeper inbox prefetch makes every
+// consumer tile lag its producer by that many more steps.
+constexpr int kBoxAhead = 4;     // steps between a record load and its use
+constexpr int kBoxInAhead = 3;   // steps between an inbox load and its use (<= kBoxAhead)
+constexpr int kBoxRounds = 4;    // prefetch-ring rounds per unrolled group of steps (measured best)
 
 // KT: the batch's record width K when it is 3 (records of two FP64 / one
 // FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
