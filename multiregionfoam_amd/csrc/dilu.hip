@@ -449,7 +449,10 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     if (nLocal <= 0) return;   // a segment without rows (nA < segs)
     // a lane's row at step t that belongs to no segment (padding) or to this
     // one: the only rows a segment wave writes
-    auto mine = [&](int ia) { return !inTile || ia < 0 || ia >= nA || (ia >= iaLo && ia < iaHi); };
+    // (bitwise, so that it compiles to a few compares and no exec-mask branches)
+    auto mine = [&](int ia) {
+        return !inTile | ((unsigned)ia >= (unsigned)nA) | ((unsigned)(ia - iaLo) < (unsigned)(iaHi - iaLo));
+    };
     // where each dependency comes from: the lane itself (sequential axis),
     // a neighbouring lane, or the inbox on the tile's edge
     const bool edgeB = kFwd ? bl == 0 : bl == TB - 1;
@@ -584,10 +587,10 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // consumer's inbox, so a step spends no instructions on those tests.
     // For the fluid block of the duct that is 9 of every 11 groups.
     constexpr int kGroup = kBoxRounds * kBoxAhead;
-    auto runGroup = [&](int i0, auto fullTag) {
+    auto runGroup = [&](int i0, auto fullTag, auto stepsTag) {
         constexpr bool FULL = decltype(fullTag)::value;
 #pragma unroll
-        for (int uu = 0; uu < kGroup; ++uu) {
+        for (int uu = 0; uu < decltype(stepsTag)::value; ++uu) {
             const int u = uu % kBoxAhead;
             const int i = i0 + uu;
             const bool inRange = FULL || i < nLocal;
@@ -726,8 +729,10 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                     bufStore<T, 0>(yz, rOut, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
                     if (INP) bufStore<T, 0>(ain, rDst, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
                 } else {
-                    bufStore<T, 0>(yz, rOut, !cut || mine(ia) ? vRow : kNoLane, so);
-                    if (INP) bufStore<T, 0>(ain, rDst, !cut || mine(ia) ? vRow : kNoLane, so);
+                    // (arithmetic select: no branch, no exec-mask moves)
+                    const unsigned vSt = vRow | (unsigned)!(!cut | mine(ia)) * kNoLane;
+                    bufStore<T, 0>(yz, rOut, vSt, so);
+                    if (INP) bufStore<T, 0>(ain, rDst, vSt, so);
                 }
             }
             if (!LONE) {
@@ -765,11 +770,22 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             __builtin_amdgcn_sched_barrier(0);
         }
     };
-    for (int i0 = 0; i0 < nLocal; i0 += kGroup) {
-        const int tMin = kFwd ? tFirst + i0 : tLast - (i0 + kGroup - 1), tMax = kFwd ? tFirst + i0 + kGroup - 1 : tLast - i0;
-        const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kBoxAhead < nLocal;
-        if (full) runGroup(i0, std::true_type{});
-        else runGroup(i0, std::false_type{});
+    // the last steps (at most kGroup - kBoxAhead) run in groups of kBoxAhead,
+    // so a sweep runs at most kBoxAhead - 1 steps past its end
+    using GroupSteps = std::integral_constant<int, kGroup>;
+    using TailSteps = std::integral_constant<int, kBoxAhead>;
+    for (int i0 = 0; i0 < nLocal;) {
+        if (nLocal - i0 > kGroup - kBoxAhead) {
+            const int tMin = kFwd ? tFirst + i0 : tLast - (i0 + kGroup - 1);
+            const int tMax = kFwd ? tFirst + i0 + kGroup - 1 : tLast - i0;
+            const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kBoxAhead < nLocal;
+            if (full) runGroup(i0, std::true_type{}, GroupSteps{});
+            else runGroup(i0, std::false_type{}, GroupSteps{});
+            i0 += kGroup;
+        } else {
+            runGroup(i0, std::false_type{}, TailSteps{});
+            i0 += kBoxAhead;
+        }
     }
 }
 
