@@ -395,7 +395,11 @@ __global__ __launch_bounds__(64) void k_dilu_wave(DiluView<T> V, const SysState 
 // consumer tile lag its producer by that many more steps.
 constexpr int kBoxAhead = 4;     // steps between a record load and its use
 constexpr int kBoxInAhead = 3;   // steps between an inbox load and its use (<= kBoxAhead)
-constexpr int kBoxRounds = 4;    // prefetch-ring rounds per unrolled group of steps (measured best)
+// A lone sweep has no inbox: a deeper prefetch costs it only registers, and
+// keeps more of its HBM loads in flight (a wave's steps wait on HBM latency
+// when the records are not in cache).
+constexpr int kBoxAheadLone = 8;
+constexpr int kBoxGroup = 16;    // steps per unrolled group (measured best)
 
 // KT: the batch's record width K when it is 3 (records of two FP64 / one
 // FP32 chunk, rD in the last), or 0 for a wider batch (K read from the view,
@@ -419,6 +423,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     constexpr int NL = G2 + (kExtra ? 1 : 0);
     constexpr U kSent = Bits<T>::kSent;
     constexpr bool kFwd = MODE != kDiluBwd;
+    constexpr int kAhead = LONE ? kBoxAheadLone : kBoxAhead;   // record prefetch distance (steps)
     // record slot of the dependency along the sequential axis and along b, c
     constexpr int kAxB = AX == 0 ? 1 : 0;
     constexpr int kSlotA = 2 - AX, kSlotB = 2 - kAxB;   // the c neighbour holds the third slot
@@ -528,7 +533,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // rows of steps t + dB and t + dC share that of step t + mOff, the lowest
     // of them.  The lane constants are non-negative and every address stays
     // inside its array (a FULL group only addresses steps inside the sweep).
-    const int dA = kFwd ? kBoxAhead : -kBoxAhead, dI = kFwd ? kBoxInAhead : -kBoxInAhead;
+    const int dA = kFwd ? kAhead : -kAhead, dI = kFwd ? kBoxInAhead : -kBoxInAhead;
     const int dBe = toB >= 0 ? dB : 0, dCe = toC >= 0 ? dC : 0;
     const int mOff = min(min(dI, 0), min(dBe, dCe));
     const unsigned vRecF = vRec + (kFwd ? (unsigned)(dA * rowBytes) : 0u);
@@ -561,14 +566,14 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // the neighbours' values for the next step, read as soon as they are
     // written (below), so the LDS round trip overlaps the rest of the step
     T xbN = xch[srcB], xcN = xch[srcC];
-    Step buf[kBoxAhead];
-    U hbuf[kBoxAhead], cbuf[kBoxAhead];
+    Step buf[kAhead];
+    U hbuf[kAhead], cbuf[kAhead];
 #pragma unroll
-    for (int u = 0; u < kBoxAhead; ++u) hbuf[u] = cbuf[u] = kSent;   // a lone tile never loads them
+    for (int u = 0; u < kAhead; ++u) hbuf[u] = cbuf[u] = kSent;   // a lone tile never loads them
     // the first steps' loads in the order they are consumed (not reordered):
     // the loop's first use then waits only for the loads it needs
 #pragma unroll
-    for (int u = 0; u < kBoxAhead; ++u) {
+    for (int u = 0; u < kAhead; ++u) {
         if (!LONE && u < kBoxInAhead) loadIn(u, hbuf[u], cbuf[u]);
         load(u, buf[u]);
         __builtin_amdgcn_sched_barrier(0);
@@ -576,7 +581,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     T yprev = T(0);
     // The loop runs whole groups of steps: a step past the end is invalid for
     // every lane, reads clamped addresses and stores nothing.  The body is
-    // unrolled over kBoxRounds rounds of the prefetch ring, every record wait
+    // unrolled over whole rounds of the prefetch ring, every record wait
     // exact (the initial loads above are issued in order for that); 16 steps
     // measured fastest (8 and 32 were slower).
     //
@@ -586,12 +591,14 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // prefetch index is clamped, and every delivery offset lies inside its
     // consumer's inbox, so a step spends no instructions on those tests.
     // For the fluid block of the duct that is 9 of every 11 groups.
-    constexpr int kGroup = kBoxRounds * kBoxAhead;
-    auto runGroup = [&](int i0, auto fullTag, auto stepsTag) {
+    constexpr int kGroup = kBoxGroup;
+    static_assert(kGroup % kAhead == 0, "a group spans whole rounds of the prefetch ring");
+    // (offTag: the prefetch-ring slot of the group's first step)
+    auto runGroup = [&](int i0, auto fullTag, auto stepsTag, auto offTag) {
         constexpr bool FULL = decltype(fullTag)::value;
 #pragma unroll
         for (int uu = 0; uu < decltype(stepsTag)::value; ++uu) {
-            const int u = uu % kBoxAhead;
+            const int u = (decltype(offTag)::value + uu) % kAhead;
             const int i = i0 + uu;
             const bool inRange = FULL || i < nLocal;
             Step &S = buf[u];
@@ -603,9 +610,6 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             const bool hasA = kFwd ? ia > iaLo : ia + 1 < iaHi;
             const bool needB = valid & useB, needC = valid & useC;
             T xb = xbN, xc = xcN;
-            // both exchange reads are issued by every lane (not sunk into the
-            // select), so the step has no divergent code
-            asm volatile("" : "+v"(xb), "+v"(xc));
             const T vA = yprev;
             auto slotVal = [&](int q) -> T { return (T)elem(S.v[q / N], q % N); };
             T last;
@@ -616,29 +620,48 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             T ain = S.a;
             if (INP == kInpP) ain = S.a + inp.c1 * S.pOld - inp.c2 * S.vv;
             else if (INP == kInpS) ain = S.a - inp.c1 * S.vv;
+            // the parts of the row value that need no neighbour value (the
+            // sweeps' first term and coefficient products)
+            T y0 = T(0), lc[3] = {T(0), T(0), T(0)}, pA = T(0);
+            if (MODE != kDiluFactor) {
+                y0 = MODE == kDiluFwd ? ain * last : ain;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) lc[q] = last * slotVal(q);
+                pA = lc[kSlotA] * vA;   // the term along the sequential axis
+            }
+            // both exchange reads are issued by every lane (not sunk into the
+            // select), so the step has no divergent code; the products above
+            // are operands too, so they are computed before the wait for the
+            // exchange instead of after it, on the step's dependency chain
+            asm volatile("" : "+v"(xb), "+v"(xc), "+v"(y0), "+v"(lc[0]), "+v"(lc[1]), "+v"(lc[2]), "+v"(pA));
             // the row's value from its neighbours' values
             auto rowValue = [&](U hbv, U hcv) -> T {
                 const T vB = useB ? Bits<T>::from(hbv) : xb;
                 const T vC = useC ? Bits<T>::from(hcv) : xc;
-                T y;
-                if (MODE == kDiluFwd) y = ain * last;
-                else if (MODE == kDiluBwd) y = ain;
-                else y = last;
-                // the three terms in slot order, which is the face order.  In
-                // the sweeps a missing neighbour has a zero coefficient and a
-                // finite value (exchanged values of invalid rows are zero), so
-                // its term is an exact zero; the factor divides and selects
+                T y = MODE == kDiluFactor ? last : y0;
+                // the three terms in slot order, which is the face order:
+                // y - (last*c)*v each (the products first, so that the second
+                // is not issued behind the first subtraction).  In the sweeps
+                // a missing neighbour has a zero coefficient and a finite
+                // value (exchanged values of invalid rows are zero), so its
+                // term is an exact zero; the factor divides and selects
                 // instead.
+                T pr[3];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     const T v = q == kSlotA ? vA : q == kSlotB ? vB : vC;
-                    const T c = slotVal(q);
                     if (MODE == kDiluFactor) {
                         const bool has = q == kSlotA ? hasA : q == kSlotB ? hasB : hasC;
-                        y = has ? y - c / v : y;
+                        y = has ? y - slotVal(q) / v : y;
                     } else {
-                        y = y - last * c * v;
+                        pr[q] = q == kSlotA ? pA : lc[q] * v;
                     }
+                }
+                if (MODE != kDiluFactor) {
+                    // (kept apart: otherwise the scheduler interleaves them again)
+                    asm volatile("" : "+v"(pr[kSlotB]), "+v"(pr[3 - kSlotA - kSlotB]));
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) y = y - pr[q];
                 }
                 return y;
             };
@@ -695,8 +718,8 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                     hc = hc == kSent ? nc[0] : hc;
 #pragma unroll
                     for (int d = 1; d < kBoxInAhead; ++d) {
-                        U &vb = hbuf[(u + d) % kBoxAhead];
-                        U &vc = cbuf[(u + d) % kBoxAhead];
+                        U &vb = hbuf[(u + d) % kAhead];
+                        U &vc = cbuf[(u + d) % kAhead];
                         vb = vb == kSent ? nb[d] : vb;
                         vc = vc == kSent ? nc[d] : vc;
                     }
@@ -757,34 +780,43 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
             }
             // the inbox values of a later step first, so that waiting for them
             // leaves the record loads issued after them in flight
-            U &hbF = hbuf[(u + kBoxInAhead) % kBoxAhead], &hcF = cbuf[(u + kBoxInAhead) % kBoxAhead];
+            U &hbF = hbuf[(u + kBoxInAhead) % kAhead], &hcF = cbuf[(u + kBoxInAhead) % kAhead];
             if (FULL) {
                 if (!LONE) loadInF(t, hbF, hcF);
                 loadF(t, S);
             } else {
                 if (!LONE) loadIn(i + kBoxInAhead, hbF, hcF);
-                load(i + kBoxAhead, S);
+                load(i + kAhead, S);
             }
             // keep each step's prefetches where they are (not sunk towards
             // their use)
             __builtin_amdgcn_sched_barrier(0);
         }
     };
-    // the last steps (at most kGroup - kBoxAhead) run in groups of kBoxAhead,
-    // so a sweep runs at most kBoxAhead - 1 steps past its end
+    // the last steps (at most kGroup - kTail) run in groups of kTail, so a
+    // sweep runs at most kTail - 1 steps past its end
+    constexpr int kTail = 4;
+    static_assert(kAhead % kTail == 0, "tail groups start on a prefetch-ring slot that is a multiple of kTail");
     using GroupSteps = std::integral_constant<int, kGroup>;
-    using TailSteps = std::integral_constant<int, kBoxAhead>;
+    using TailSteps = std::integral_constant<int, kTail>;
+    using Off0 = std::integral_constant<int, 0>;
     for (int i0 = 0; i0 < nLocal;) {
-        if (nLocal - i0 > kGroup - kBoxAhead) {
+        if (nLocal - i0 > kGroup - kTail) {
             const int tMin = kFwd ? tFirst + i0 : tLast - (i0 + kGroup - 1);
             const int tMax = kFwd ? tFirst + i0 + kGroup - 1 : tLast - i0;
-            const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kBoxAhead < nLocal;
-            if (full) runGroup(i0, std::true_type{}, GroupSteps{});
-            else runGroup(i0, std::false_type{}, GroupSteps{});
+            const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kAhead < nLocal;
+            if (full) runGroup(i0, std::true_type{}, GroupSteps{}, Off0{});
+            else runGroup(i0, std::false_type{}, GroupSteps{}, Off0{});
             i0 += kGroup;
         } else {
-            runGroup(i0, std::false_type{}, TailSteps{});
-            i0 += kBoxAhead;
+            if constexpr (kAhead == kTail) {
+                runGroup(i0, std::false_type{}, TailSteps{}, Off0{});
+            } else {
+                static_assert(kAhead == 2 * kTail, "tail groups start at ring slot 0 or kTail");
+                if (i0 % kAhead == 0) runGroup(i0, std::false_type{}, TailSteps{}, Off0{});
+                else runGroup(i0, std::false_type{}, TailSteps{}, std::integral_constant<int, kTail>{});
+            }
+            i0 += kTail;
         }
     }
 }

@@ -19,6 +19,7 @@ import random
 import pytest
 
 kBoxAhead, kBoxInAhead, kGroup = 4, 3, 16
+kBoxAheadLone = 8   # lone sweeps (no inbox) prefetch records further ahead
 
 
 def tangential(a):
@@ -97,13 +98,14 @@ def segment(nA, s, segs):
 
 def replay_segments(n, TB, fwd, segs):
     """The lone (tile-local) sweep of each segment of each tile: its steps
-    tFirst..tLast, the FULL condition and the addresses of a FULL step."""
+    tFirst..tLast, the FULL condition and the addresses of a FULL step (lone
+    sweeps prefetch kBoxAheadLone steps ahead)."""
     tiles, TC = tile_grid(n, TB)
     full_groups = 0
     for m in tiles.values():
         nA, actB, actC, nSteps = m["nA"], m["actB"], m["actC"], m["steps"]
         tLo = actB - 1 + actC - 1
-        dA = kBoxAhead if fwd else -kBoxAhead
+        dA = kBoxAheadLone if fwd else -kBoxAheadLone
         covered = set()
         for sg in range(segs):
             lo, hi = segment(nA, sg, segs)
@@ -118,12 +120,12 @@ def replay_segments(n, TB, fwd, segs):
             for i0 in range(0, max(nLocal, 0), kGroup):
                 tMin = tFirst + i0 if fwd else tLast - (i0 + kGroup - 1)
                 tMax = tFirst + i0 + kGroup - 1 if fwd else tLast - i0
-                if not (tMin >= lo + tLo and tMax <= hi - 1 and i0 + kGroup - 1 + kBoxAhead < nLocal):
+                if not (tMin >= lo + tLo and tMax <= hi - 1 and i0 + kGroup - 1 + kBoxAheadLone < nLocal):
                     continue
                 full_groups += 1
                 for i in range(i0, i0 + kGroup):
                     t = tFirst + i if fwd else tLast - i
-                    # records / input of the step kBoxAhead later, inside the segment's steps
+                    # records / input of the step kBoxAheadLone later, inside the segment's steps
                     j = t + dA
                     assert tFirst <= j <= tLast and (t if fwd else j) >= 0
                     for bl, cl in itertools.product(range(actB), range(actC)):
