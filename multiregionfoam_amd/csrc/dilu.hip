@@ -593,9 +593,12 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     // For the fluid block of the duct that is 9 of every 11 groups.
     constexpr int kGroup = kBoxGroup;
     static_assert(kGroup % kAhead == 0, "a group spans whole rounds of the prefetch ring");
-    // (offTag: the prefetch-ring slot of the group's first step)
-    auto runGroup = [&](int i0, auto fullTag, auto stepsTag, auto offTag) {
+    // (offTag: the prefetch-ring slot of the group's first step; cutTag: the
+    // sweep is a segment of its tile, so rows of other segments are not
+    // stored)
+    auto runGroup = [&](int i0, auto fullTag, auto stepsTag, auto offTag, auto cutTag) {
         constexpr bool FULL = decltype(fullTag)::value;
+        constexpr bool CUT = decltype(cutTag)::value;
 #pragma unroll
         for (int uu = 0; uu < decltype(stepsTag)::value; ++uu) {
             const int u = (decltype(offTag)::value + uu) % kAhead;
@@ -740,7 +743,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                 const unsigned so = (unsigned)t * 64 * sizeof(T);
                 if (MODE == kDiluFactor) {
                     const double rd = valid ? 1.0 / (double)y : 0.0;
-                    if (FULL || !cut || mine(ia)) (V.rDout + (size_t)(s0 + t) * 64)[lane] = rd;
+                    if (FULL || !CUT || mine(ia)) (V.rDout + (size_t)(s0 + t) * 64)[lane] = rd;
                     if (V.fPack) {   // (a tile-local factor writes rD only)
                         constexpr int NP = Pack<double>::N;
                         const int GP = packChunks(K + 1, NP);
@@ -753,7 +756,7 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
                     if (INP) bufStore<T, 0>(ain, rDst, vOutF, (unsigned)((kFwd ? t : t + dA) * 64 * sizeof(T)));
                 } else {
                     // (arithmetic select: no branch, no exec-mask moves)
-                    const unsigned vSt = vRow | (unsigned)!(!cut | mine(ia)) * kNoLane;
+                    const unsigned vSt = CUT ? vRow | (unsigned)!mine(ia) * kNoLane : vRow;
                     bufStore<T, 0>(yz, rOut, vSt, so);
                     if (INP) bufStore<T, 0>(ain, rDst, vSt, so);
                 }
@@ -800,24 +803,33 @@ __device__ __forceinline__ void boxSweep(const DiluView<T> &V, const int32_t *m,
     using GroupSteps = std::integral_constant<int, kGroup>;
     using TailSteps = std::integral_constant<int, kTail>;
     using Off0 = std::integral_constant<int, 0>;
-    for (int i0 = 0; i0 < nLocal;) {
-        if (nLocal - i0 > kGroup - kTail) {
-            const int tMin = kFwd ? tFirst + i0 : tLast - (i0 + kGroup - 1);
-            const int tMax = kFwd ? tFirst + i0 + kGroup - 1 : tLast - i0;
-            const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kAhead < nLocal;
-            if (full) runGroup(i0, std::true_type{}, GroupSteps{}, Off0{});
-            else runGroup(i0, std::false_type{}, GroupSteps{}, Off0{});
-            i0 += kGroup;
-        } else {
-            if constexpr (kAhead == kTail) {
-                runGroup(i0, std::false_type{}, TailSteps{}, Off0{});
+    auto sweep = [&](auto cutTag) {
+        for (int i0 = 0; i0 < nLocal;) {
+            if (nLocal - i0 > kGroup - kTail) {
+                const int tMin = kFwd ? tFirst + i0 : tLast - (i0 + kGroup - 1);
+                const int tMax = kFwd ? tFirst + i0 + kGroup - 1 : tLast - i0;
+                const bool full = tMin >= iaLo + tLo && tMax <= iaHi - 1 && i0 + kGroup - 1 + kAhead < nLocal;
+                if (full) runGroup(i0, std::true_type{}, GroupSteps{}, Off0{}, cutTag);
+                else runGroup(i0, std::false_type{}, GroupSteps{}, Off0{}, cutTag);
+                i0 += kGroup;
             } else {
-                static_assert(kAhead == 2 * kTail, "tail groups start at ring slot 0 or kTail");
-                if (i0 % kAhead == 0) runGroup(i0, std::false_type{}, TailSteps{}, Off0{});
-                else runGroup(i0, std::false_type{}, TailSteps{}, std::integral_constant<int, kTail>{});
+                if constexpr (kAhead == kTail) {
+                    runGroup(i0, std::false_type{}, TailSteps{}, Off0{}, cutTag);
+                } else {
+                    static_assert(kAhead == 2 * kTail, "tail groups start at ring slot 0 or kTail");
+                    if (i0 % kAhead == 0) runGroup(i0, std::false_type{}, TailSteps{}, Off0{}, cutTag);
+                    else runGroup(i0, std::false_type{}, TailSteps{}, std::integral_constant<int, kTail>{}, cutTag);
+                }
+                i0 += kTail;
             }
-            i0 += kTail;
         }
+    };
+    // (only lone sweeps are ever cut)
+    if constexpr (LONE) {
+        if (cut) sweep(std::true_type{});
+        else sweep(std::false_type{});
+    } else {
+        sweep(std::false_type{});
     }
 }
 
